@@ -1,0 +1,162 @@
+"""PHOLD events/s at 2^20 LPs on one MI355X.
+
+A step is one complete PHOLD simulation of the workload: 2^20 LPs, 32 initial
+events per LP (2^25 events in flight), end time 50, float32 timestamps
+ts' = (ts + 0.1) + Exp(1), destination self with p = 0.5 else uniform, Philox
+streams from seed 42.  The queue is populated outside the timed region (the
+reference times the simulation loop only, src/des.cu:148-215); the timed region
+of a step is the whole step loop, driven by hipGraph replays.
+
+Beside the JSON line's value:
+  roofline      the dominant kernel's algorithmic bytes per launch / its mean
+                launch duration (HIP events on the queue's stream), against 8 TB/s
+  cpu_baseline  the sequential std::priority_queue oracle on a bounded sample of
+                the same workload (end time 0.5), one host core
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+RECORD = 16          # bytes per event record
+METRIC = "PHOLD events/s @2^20 LPs; batched A* queries/s + p99 latency vs plain"
+
+WORKLOADS = {
+    "phold_2p20_k32_t50": dict(num_lps=1 << 20, init_per_lp=32, end_time=50.0),
+    "phold_1024_k16_t100": dict(num_lps=1024, init_per_lp=16, end_time=100.0),
+}
+QUEUES = {
+    "laq": dict(capacity_per_lp=128, stage_per_lp=16),
+}
+
+
+def make(gpq, kind, wl, stats=False, checks=False):
+    q = gpq.Queue(kind, num_lps=wl["num_lps"], lookahead=0.1, horizon=wl["end_time"], block_threads=256,
+                  collect_stats=stats, **QUEUES[kind])
+    m = gpq.Phold(q, init_per_lp=wl["init_per_lp"], seed=42, end_time=wl["end_time"], checks=checks)
+    m.model_init()
+    return q, m
+
+
+def roofline(gpq, kind, wl, log):
+    """Mean launch durations of the model step kernel and the queue post kernel from a
+    HIP-event-instrumented run, and their algorithmic bytes from the device work
+    counters of a second (identical, deterministic) run."""
+    q, m = make(gpq, kind, wl)
+    prof = m.profile(batch=64)
+    m.model_finish(); q.queue_finish()
+    q, m = make(gpq, kind, wl, stats=True)
+    before = q.stats()
+    rep = gpq.des_run(m, mode="graph", batch=64)
+    after = q.stats()
+    m.model_finish(); q.queue_finish()
+    d = {k: after[k] - before[k] for k in after}
+    launches = max(d["post_launches"], 1)
+    L = wl["num_lps"]
+    # queue post: sorted insert of k records displacing d (16*(2k+2d)) + one head word per LP
+    post_bytes = (RECORD * (2 * d["inserted"] + 2 * d["displaced"]) + 4 * d["scanned_lps"] + d["aux_bytes"]
+                  + 4 * launches) / launches
+    # model step: one head word per LP, and per committed event the head record (16), the
+    # per-LP state read+write (count 8, hash 16, ring indices 16, head 4 + next head 4),
+    # the staging counter (4) and the staged record (16)
+    events_per_step = rep["events"] / max(prof["steps"], 1)
+    step_bytes = 4 * L + 88 * events_per_step
+    kernels = {
+        "queue_post": {"us": prof["post_kernel_us"], "alg_bytes": post_bytes},
+        "model_step": {"us": prof["step_kernel_us"], "alg_bytes": step_bytes},
+    }
+    for k in kernels.values():
+        k["gbps"] = k["alg_bytes"] / (k["us"] * 1e-6) / 1e9 if k["us"] > 0 else 0.0
+    dom = max(kernels, key=lambda k: kernels[k]["us"])
+    log("profile: %s" % json.dumps({"prof": prof, "counters": d, "kernels": kernels}))
+    return dom, kernels, d
+
+
+def cpu_baseline(wl, log):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    sample_T = 0.5
+    res, _, _ = Oracle().phold("seq", wl["num_lps"], wl["init_per_lp"], sample_T)
+    log("cpu_baseline: %d events in %.2fs" % (res.events, res.seconds))
+    return {"value": res.events / res.seconds, "unit": "events/s", "cores": 1, "kind": "port",
+            "sample": "sequential std::priority_queue oracle, %d LPs x %d events, end time %.2f "
+                      "(%d events), event loop only" % (wl["num_lps"], wl["init_per_lp"], sample_T, res.events)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--queue", default="laq", choices=sorted(QUEUES))
+    ap.add_argument("--workload", default="phold_2p20_k32_t50", choices=sorted(WORKLOADS))
+    ap.add_argument("--batch", type=int, default=64, help="steps per hipGraph replay")
+    ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not provided "
+              "(the multi-GPU sharding is a design note only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    def log(msg):
+        print("[bench] " + msg, file=sys.stderr, flush=True)
+
+    import gpupq_amd as gpq
+    wl = WORKLOADS[args.workload]
+    golden = None
+    gpath = os.path.join(ROOT, "tests", "golden", "phold_golden.json")
+    if os.path.exists(gpath):
+        golden = json.load(open(gpath)).get(args.workload)
+
+    total_events, total_s, hashes = 0, 0.0, set()
+    for i in range(args.warmup + args.steps):
+        q, m = make(gpq, args.queue, wl)
+        q.sync()
+        t0 = time.perf_counter()
+        rep = gpq.des_run(m, mode="graph", batch=args.batch)
+        dt = time.perf_counter() - t0
+        h = m.trace_hash()
+        hashes.add(h)
+        m.model_finish()
+        q.queue_finish()
+        timed = i >= args.warmup
+        log("step %d%s: %d events, %d windows, %.4fs (%.3e events/s), hash %016x" %
+            (i, "" if timed else " (warmup)", rep["events"], rep["windows"], dt, rep["events"] / dt, h))
+        if timed:
+            total_events += rep["events"]
+            total_s += dt
+    value = total_events / total_s
+    parity = "no-golden"
+    if golden is not None:
+        parity = "match" if hashes == {int(golden["trace_hash"], 16)} else "MISMATCH"
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1000.0 * total_s / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "queue": args.queue, "num_lps": wl["num_lps"],
+                   "init_per_lp": wl["init_per_lp"], "end_time": wl["end_time"], "lookahead": 0.1,
+                   "drive": "hipgraph x%d" % args.batch, "events_per_step": total_events // args.steps,
+                   "trace_hash_vs_cpu": parity},
+    }
+    if not args.no_roofline:
+        dom, kernels, counters = roofline(gpq, args.queue, wl, log)
+        k = kernels[dom]
+        line["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS,
+                            "unit": "GB/s", "frac": k["gbps"] / HBM_PEAK_GBPS, "traffic": None,
+                            "alg_bytes_per_launch": k["alg_bytes"], "mean_launch_us": k["us"],
+                            "other_kernels": {n: {"achieved": v["gbps"], "mean_launch_us": v["us"]}
+                                              for n, v in kernels.items() if n != dom}}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(wl, log)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,213 @@
+"""gpupq_amd -- MI355X-native data-parallel priority queues and the PHOLD / A*
+models that drive them.
+
+The names mirror the reference's host interfaces:
+  queue.h  (src/queue.h:86-115): Queue.queue_init / queue_post_init / queue_pre /
+           queue_post / queue_get_min_ts / queue_finish / queue_check_phold
+  model.h  (src/model.h:9-19):  Phold.model_init / model_handle_next /
+           model_get_events / model_finish
+  des.h    (src/des.h:3-5):     des_init / des_run / des_finish
+All work runs in hand-written HIP kernels in libgpupq.so (gfx950); importing
+this package without the built library raises ImportError.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import DesReport, Event, GpqError, PholdConfig, QueueConfig, check
+
+__all__ = ["Queue", "Phold", "des_init", "des_run", "des_finish", "GpqError", "lib", "event_dtype"]
+
+lib = _lib.load()
+
+# numpy view of gpq_event
+event_dtype = np.dtype([("ts", "<u4"), ("dst", "<u4"), ("src", "<u4"), ("seq", "<u4")])
+
+
+def f32_bits(x):
+    return int(np.float32(x).view(np.uint32))
+
+
+def bits_f32(u):
+    return float(np.uint32(u).view(np.float32))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Queue:
+    """One queue handle.  Reference: queue_init() (src/queue.h:87) with the
+    compile-time choices of src/config.h as constructor arguments."""
+
+    def __init__(self, kind="laq", num_lps=1024, capacity_per_lp=64, stage_per_lp=16, node_size=0,
+                 heap_nodes=0, total_capacity=0, lookahead=0.1, horizon=float("inf"), block_threads=256,
+                 collect_stats=False):
+        self.kind = kind
+        self.num_lps = num_lps
+        cfg = QueueConfig(_lib.QUEUE_KINDS[kind], num_lps, capacity_per_lp, stage_per_lp, node_size,
+                          heap_nodes, total_capacity, lookahead, horizon, block_threads, 1 if collect_stats else 0)
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        check(lib.gpq_queue_create(ctypes.byref(cfg), ctypes.byref(h)), "queue_init")
+        self.handle = h
+
+    # -- queue.h
+    def queue_post_init(self):
+        check(lib.gpq_queue_post_init(self.handle), "queue_post_init")
+
+    def queue_pre(self):
+        check(lib.gpq_queue_pre(self.handle), "queue_pre")
+
+    def queue_post(self):
+        check(lib.gpq_queue_post(self.handle), "queue_post")
+
+    def queue_get_min_ts(self):
+        out = ctypes.c_float()
+        check(lib.gpq_queue_get_min_ts(self.handle, ctypes.byref(out)), "queue_get_min_ts")
+        return out.value
+
+    def queue_check_phold(self):
+        """Pending item count (conservation check, src/heap_queue.cu:1908-1945)."""
+        out = ctypes.c_uint64()
+        check(lib.gpq_queue_pending(self.handle, ctypes.byref(out)), "queue_check_phold")
+        return out.value
+
+    def error_flags(self):
+        out = ctypes.c_uint32()
+        check(lib.gpq_queue_error_flags(self.handle, ctypes.byref(out)), "error_flags")
+        return out.value
+
+    def stats(self):
+        """Work counters of the post kernels (needs collect_stats=True for all but post_launches)."""
+        st = _lib.QueueStats()
+        check(lib.gpq_queue_stats_get(self.handle, ctypes.byref(st)), "stats")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def sync(self):
+        check(lib.gpq_queue_sync(self.handle), "sync")
+
+    def queue_finish(self):
+        if self.handle:
+            check(lib.gpq_queue_destroy(self.handle), "queue_finish")
+            self.handle = None
+
+    close = queue_finish
+
+    # -- device verbs driven from the host (src/queue.cuh:46-70)
+    def insert(self, events):
+        ev = np.ascontiguousarray(events, dtype=event_dtype)
+        check(lib.gpq_queue_insert_host(self.handle, _ptr(ev), len(ev)), "queue_insert")
+
+    def peek(self, set_done=False, max_items=None):
+        max_items = max_items or max(self.num_lps, 1) * 4
+        out = np.zeros(max_items, dtype=event_dtype)
+        idx = np.zeros(max_items, dtype=np.int32)
+        found = ctypes.c_uint32()
+        check(lib.gpq_queue_peek_host(self.handle, 1 if set_done else 0, _ptr(out), _ptr(idx), max_items,
+                                      ctypes.byref(found)), "queue_peek")
+        n = min(found.value, max_items)
+        order = np.lexsort((out["seq"][:n], out["src"][:n], out["ts"][:n]))
+        return out[:n][order], idx[:n][order]
+
+    def lp_contents(self, lp, cap=4096):
+        out = np.zeros(cap, dtype=event_dtype)
+        n = ctypes.c_uint32()
+        check(lib.gpq_queue_lp_contents(self.handle, lp, _ptr(out), cap, ctypes.byref(n)), "lp_contents")
+        return out[:min(n.value, cap)]
+
+    def queue_length(self, lp):
+        n = ctypes.c_uint32()
+        check(lib.gpq_queue_length(self.handle, lp, ctypes.byref(n)), "queue_length")
+        return n.value
+
+    def queue_clear(self, lp):
+        check(lib.gpq_queue_clear_lp(self.handle, lp), "queue_clear")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.queue_finish()
+
+    def __del__(self):
+        try:
+            self.queue_finish()
+        except Exception:
+            pass
+
+
+class Phold:
+    """PHOLD model on a queue (src/phold.cu, src/model.h:9-19)."""
+
+    def __init__(self, queue, init_per_lp=16, seed=42, end_time=None, lookahead=None, p_self=0.5, checks=True):
+        self.queue = queue
+        lookahead = queue.cfg.lookahead if lookahead is None else lookahead
+        end_time = queue.cfg.horizon if end_time is None else end_time
+        self.cfg = PholdConfig(queue.num_lps, init_per_lp, seed, lookahead, end_time,
+                               min(int(round(p_self * 2 ** 32)), 0xFFFFFFFF), 1 if checks else 0)
+        h = ctypes.c_void_p()
+        check(lib.gpq_phold_create(queue.handle, ctypes.byref(self.cfg), ctypes.byref(h)), "phold_create")
+        self.handle = h
+
+    def model_init(self):
+        check(lib.gpq_phold_init(self.handle), "model_init")
+
+    def model_handle_next(self):
+        check(lib.gpq_phold_handle_next(self.handle), "model_handle_next")
+
+    def model_get_events(self):
+        out = ctypes.c_uint64()
+        check(lib.gpq_phold_get_events(self.handle, ctypes.byref(out)), "model_get_events")
+        return out.value
+
+    def model_finish(self):
+        if self.handle:
+            check(lib.gpq_phold_finish(self.handle), "model_finish")
+            check(lib.gpq_phold_destroy(self.handle), "model_finish")
+            self.handle = None
+
+    def lp_state(self):
+        n = self.queue.num_lps
+        cnt = np.zeros(n, dtype=np.uint32)
+        hsh = np.zeros(n, dtype=np.uint64)
+        check(lib.gpq_phold_lp_state(self.handle, _ptr(cnt), _ptr(hsh)), "lp_state")
+        return cnt, hsh
+
+    def trace_hash(self):
+        out = ctypes.c_uint64()
+        check(lib.gpq_phold_trace_hash(self.handle, ctypes.byref(out)), "trace_hash")
+        return out.value
+
+    def profile(self, batch=64):
+        """Runs the step loop with HIP events around every kernel; mean durations in us."""
+        p = _lib.KernelProfile()
+        check(lib.gpq_des_profile(self.handle, batch, ctypes.byref(p)), "des_profile")
+        return {k: getattr(p, k) for k, _ in p._fields_}
+
+    def __del__(self):
+        try:
+            self.model_finish()
+        except Exception:
+            pass
+
+
+def des_init(kind="laq", **queue_kwargs):
+    """des_init() (src/des.cu:74-87): creates the configured queue."""
+    return Queue(kind=kind, **queue_kwargs)
+
+
+def des_run(model, mode="graph", batch=32):
+    """des_run() (src/des.cu:134-221): steps until no pending event is below end_time.
+    Returns the report as a dict (events/s as in src/des.cu:95-96)."""
+    rep = DesReport()
+    check(lib.gpq_des_run(model.handle, _lib.DRIVE_MODES[mode], batch, ctypes.byref(rep)), "des_run")
+    return {k: getattr(rep, k) for k, _ in DesReport._fields_}
+
+
+def des_finish(model):
+    """des_finish() (src/des.cu:89-132): releases the model and its queue."""
+    q = model.queue
+    model.model_finish()
+    q.queue_finish()

@@ -1,0 +1,118 @@
+"""ctypes declarations of include/gpupq.h.  Loading fails loudly when the HIP
+library has not been built: there is no CPU fallback."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpupq.so")
+
+c_u32 = ctypes.c_uint32
+c_u64 = ctypes.c_uint64
+c_i32 = ctypes.c_int32
+c_f32 = ctypes.c_float
+c_void_p = ctypes.c_void_p
+
+GPQ_OK, GPQ_ERR_INVALID, GPQ_ERR_CAPACITY, GPQ_ERR_HIP, GPQ_ERR_NOMEM, GPQ_ERR_STATE, GPQ_ERR_UNSUPPORTED = range(7)
+QUEUE_KINDS = {"laq": 0, "lsq": 1, "lhq": 2, "hq": 3, "gaq": 4}
+DRIVE_MODES = {"plain": 0, "graph": 1}
+
+
+class Event(ctypes.Structure):
+    _fields_ = [("ts", c_u32), ("dst", c_u32), ("src", c_u32), ("seq", c_u32)]
+
+
+class QueueConfig(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("num_lps", c_u32), ("capacity_per_lp", c_u32),
+                ("stage_per_lp", c_u32), ("node_size", c_u32), ("heap_nodes", c_u32),
+                ("total_capacity", c_u64), ("lookahead", c_f32), ("horizon", c_f32),
+                ("block_threads", c_u32), ("collect_stats", c_u32)]
+
+
+class PholdConfig(ctypes.Structure):
+    _fields_ = [("num_lps", c_u32), ("init_per_lp", c_u32), ("seed", c_u32), ("lookahead", c_f32),
+                ("end_time", c_f32), ("p_self_u32", c_u32), ("checks", c_u32)]
+
+
+class DesReport(ctypes.Structure):
+    _fields_ = [("steps", c_u64), ("windows", c_u64), ("events", c_u64), ("sim_seconds", ctypes.c_double),
+                ("events_per_second", ctypes.c_double), ("final_min_ts", c_f32), ("error_flags", c_u32)]
+
+
+class QueueStats(ctypes.Structure):
+    _fields_ = [("post_launches", c_u64), ("inserted", c_u64), ("displaced", c_u64), ("merged_lps", c_u64),
+                ("scanned_lps", c_u64), ("aux_bytes", c_u64)]
+
+
+class KernelProfile(ctypes.Structure):
+    _fields_ = [("steps", c_u64), ("events", c_u64), ("step_kernel_us", ctypes.c_double),
+                ("post_kernel_us", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+# (name, restype, argtypes) for every entry point of include/gpupq.h
+SIGNATURES = [
+    ("gpq_status_string", ctypes.c_char_p, [ctypes.c_int]),
+    ("gpq_last_error", ctypes.c_char_p, []),
+    ("gpq_version", ctypes.c_char_p, []),
+    ("gpq_queue_create", ctypes.c_int, [ctypes.POINTER(QueueConfig), ctypes.POINTER(c_void_p)]),
+    ("gpq_queue_destroy", ctypes.c_int, [c_void_p]),
+    ("gpq_queue_post_init", ctypes.c_int, [c_void_p]),
+    ("gpq_queue_pre", ctypes.c_int, [c_void_p]),
+    ("gpq_queue_post", ctypes.c_int, [c_void_p]),
+    ("gpq_queue_get_min_ts", ctypes.c_int, [c_void_p, ctypes.POINTER(c_f32)]),
+    ("gpq_queue_pending", ctypes.c_int, [c_void_p, ctypes.POINTER(c_u64)]),
+    ("gpq_queue_error_flags", ctypes.c_int, [c_void_p, ctypes.POINTER(c_u32)]),
+    ("gpq_queue_sync", ctypes.c_int, [c_void_p]),
+    ("gpq_queue_insert_host", ctypes.c_int, [c_void_p, c_void_p, c_u32]),
+    ("gpq_queue_peek_host", ctypes.c_int, [c_void_p, ctypes.c_int, c_void_p, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
+    ("gpq_queue_lp_contents", ctypes.c_int, [c_void_p, c_u32, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
+    ("gpq_queue_length", ctypes.c_int, [c_void_p, c_u32, ctypes.POINTER(c_u32)]),
+    ("gpq_queue_clear_lp", ctypes.c_int, [c_void_p, c_u32]),
+    ("gpq_phold_create", ctypes.c_int, [c_void_p, ctypes.POINTER(PholdConfig), ctypes.POINTER(c_void_p)]),
+    ("gpq_phold_destroy", ctypes.c_int, [c_void_p]),
+    ("gpq_phold_init", ctypes.c_int, [c_void_p]),
+    ("gpq_phold_handle_next", ctypes.c_int, [c_void_p]),
+    ("gpq_phold_get_events", ctypes.c_int, [c_void_p, ctypes.POINTER(c_u64)]),
+    ("gpq_phold_finish", ctypes.c_int, [c_void_p]),
+    ("gpq_phold_lp_state", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    ("gpq_phold_trace_hash", ctypes.c_int, [c_void_p, ctypes.POINTER(c_u64)]),
+    ("gpq_des_run", ctypes.c_int, [c_void_p, ctypes.c_int, c_u32, ctypes.POINTER(DesReport)]),
+    ("gpq_queue_stats_get", ctypes.c_int, [c_void_p, ctypes.POINTER(QueueStats)]),
+    ("gpq_des_profile", ctypes.c_int, [c_void_p, c_u32, ctypes.POINTER(KernelProfile)]),
+    ("gpq_selftest_exp", ctypes.c_int, [c_u32, c_u32, c_void_p]),
+    ("gpq_selftest_philox", ctypes.c_int, [c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_void_p]),
+]
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads libgpupq.so and declares every entry point; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError("gpupq_amd: %s is not built (run `python -m gpupq_amd.build`); "
+                          "there is no fallback implementation" % path)
+    lib = ctypes.CDLL(path)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class GpqError(RuntimeError):
+    def __init__(self, status, where, message):
+        super().__init__("%s: %s (%s)" % (where, message, _status_name(status)))
+        self.status = status
+
+
+def _status_name(s):
+    return {0: "ok", 1: "invalid", 2: "capacity", 3: "hip", 4: "nomem", 5: "state", 6: "unsupported"}.get(s, str(s))
+
+
+def check(status, where):
+    if status != GPQ_OK:
+        msg = _lib.gpq_last_error().decode() if _lib is not None else ""
+        raise GpqError(status, where, msg)

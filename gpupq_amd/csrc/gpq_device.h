@@ -1,0 +1,184 @@
+// gpq_device.h -- device-side vocabulary shared by every queue and model kernel:
+// the event record and its total order, the counter-based random stream, the
+// Exp(1) sampler, the trace hash and the wave64 helpers.  gfx950 only.
+//
+// Reference counterparts: src/queue_item.h:11-32 (event, operator<),
+// src/phold.cu:29-34 (rand_exponential), src/util.cuh:13-16 (get_lp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpq {
+
+constexpr int kWave = 64;
+constexpr uint32_t kInfBits = 0x7F800000u;   // +inf as float bits: "no event"
+
+// 16-byte event record.  ts holds float32 bits; every timestamp is >= 0, so the
+// unsigned order of the bits is the float order.  dst names the LP that owns the
+// event (the reference's `node`, one node per LP); (src, seq) is unique per event.
+struct __attribute__((aligned(16))) Event {
+  uint32_t ts;
+  uint32_t dst;
+  uint32_t src;
+  uint32_t seq;
+};
+
+// Total order (ts, src, seq); the reference compares ts only (src/queue_item.h:22-28).
+__device__ __forceinline__ bool key_less(const Event& a, const Event& b) {
+  if (a.ts != b.ts) return a.ts < b.ts;
+  if (a.src != b.src) return a.src < b.src;
+  return a.seq < b.seq;
+}
+
+__device__ __forceinline__ Event load_event(const Event* p) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  return Event{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ void store_event(Event* p, const Event& e) {
+  *reinterpret_cast<uint4*>(p) = make_uint4(e.ts, e.dst, e.src, e.seq);
+}
+
+__device__ __forceinline__ float bits_to_f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t f_to_bits(float f) { return __float_as_uint(f); }
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11): 10 rounds of two 32x32->64 products,
+// Weyl key bumps between rounds.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
+#pragma unroll
+  for (int round = 0; round < 10; ++round) {
+    if (round) { k.x += 0x9E3779B9u; k.y += 0xBB67AE85u; }
+    const uint32_t lo_a = 0xD2511F53u * c.x;
+    const uint32_t hi_a = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo_b = 0xCD9E8D57u * c.z;
+    const uint32_t hi_b = __umulhi(0xCD9E8D57u, c.z);
+    c = make_uint4(hi_b ^ c.y ^ k.x, lo_b, hi_a ^ c.w ^ k.y, lo_a);
+  }
+  return c;
+}
+
+// Exp(1) from one 32-bit draw: u = ((x>>8)+1) * 2^-24 in (0, 1], returns (float)(-ln u).
+// ln of the integer n = m * 2^k, m in [sqrt(1/2), sqrt(2)), by the atanh series
+// 2(s + s^3/3 + ... + s^13/13), s = (m-1)/(m+1), in IEEE double with a fixed
+// operation order (the library is compiled with -ffp-contract=off), so the value
+// is bit-identical on every IEEE-754 machine.
+__device__ __forceinline__ float exp_from_u32(uint32_t x) {
+  const uint32_t n = (x >> 8) + 1u;
+  int k = 31 - __clz(n);
+  const double scale = __longlong_as_double((long long)((uint64_t)(1023 - k) << 52));
+  double m = (double)n * scale;
+  if (m >= 0x1.6a09e667f3bcdp+0) { m = m * 0.5; k += 1; }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double p = 0x1.3b13b13b13b14p-3;
+  p = 0x1.745d1745d1746p-3 + z * p;
+  p = 0x1.c71c71c71c71cp-3 + z * p;
+  p = 0x1.2492492492492p-2 + z * p;
+  p = 0x1.999999999999ap-2 + z * p;
+  p = 0x1.5555555555555p-1 + z * p;
+  p = 2.0 + z * p;
+  const double ln_m = s * p;
+  const double r = (double)(24 - k) * 0x1.62e42fefa39efp-1 - ln_m;
+  return (float)r;
+}
+
+// ---------------------------------------------------------------------------
+// Trace hash: per-LP chain over committed (ts bits, src, seq), splitmix64 mixing.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 30; x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 27; x *= 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return x;
+}
+__device__ __forceinline__ uint64_t trace_step(uint64_t h, uint32_t ts, uint32_t src, uint32_t seq) {
+  h = mix64(h ^ (((uint64_t)ts << 32) | src));
+  h = mix64(h ^ ((uint64_t)seq + 0x9E3779B97F4A7C15ull));
+  return h;
+}
+
+// ---------------------------------------------------------------------------
+// wave64 helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)v, off, kWave);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ Event shfl_event(const Event& e, int src_lane) {
+  return Event{(uint32_t)__shfl((int)e.ts, src_lane, kWave), (uint32_t)__shfl((int)e.dst, src_lane, kWave),
+               (uint32_t)__shfl((int)e.src, src_lane, kWave), (uint32_t)__shfl((int)e.seq, src_lane, kWave)};
+}
+
+// Device-resident window state shared by a queue and the model driving it.
+// gmin: minimum pending timestamp (bits); wend: exclusive end of the safe window,
+// min(gmin + lookahead, horizon); done: nonzero once nothing is left below the horizon.
+struct WindowState {
+  uint32_t gmin;
+  uint32_t wend;
+  uint32_t done;
+  uint32_t blocks_arrived;   // last-block-reduction ticket
+  uint64_t windows;          // number of windows computed
+  uint32_t error;            // GPQ_FLAG_* bits
+  uint32_t pad;
+};
+
+enum : uint32_t {
+  kErrStageOverflow = 1u,    // more inserts for one LP in one step than the staging area holds
+  kErrQueueOverflow = 2u,    // an LP's queue exceeded its capacity
+  kErrHeapOverflow = 4u,     // the parallel heap ran out of nodes
+  kErrBadState = 8u,
+};
+
+// Block-wide min of one value per thread, then the last block to finish folds
+// the per-block partials and publishes the window.  Grid-stride safe: `partial`
+// has one slot per block.  blockDim.x must be a multiple of 64 and <= 1024.
+__device__ __forceinline__ void window_reduce_publish(uint32_t v, uint32_t* partial, WindowState* ws,
+                                                      float lookahead, float horizon) {
+  __shared__ uint32_t s_wmin[16];
+  __shared__ bool s_last;
+  v = wave_min_u32(v);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) s_wmin[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = s_wmin[0];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) m = s_wmin[i] < m ? s_wmin[i] : m;
+    partial[blockIdx.x] = m;
+    __threadfence();
+    const uint32_t ticket = atomicAdd(&ws->blocks_arrived, 1u);
+    s_last = (ticket == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  uint32_t m = kInfBits;
+  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+    const uint32_t p = __atomic_load_n(&partial[i], __ATOMIC_RELAXED);
+    m = p < m ? p : m;
+  }
+  m = wave_min_u32(m);
+  if (lane_id() == 0) s_wmin[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t g = s_wmin[0];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) g = s_wmin[i] < g ? s_wmin[i] : g;
+    float wend = bits_to_f(g) + lookahead;            // inf + la stays inf
+    if (horizon < wend) wend = horizon;
+    ws->gmin = g;
+    ws->wend = f_to_bits(wend);
+    ws->done = (g == kInfBits || !(bits_to_f(g) < horizon)) ? 1u : 0u;
+    ws->windows += 1;
+    ws->blocks_arrived = 0;
+  }
+}
+
+}  // namespace gpq

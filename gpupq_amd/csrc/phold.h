@@ -1,0 +1,52 @@
+// phold.h -- host class of the PHOLD model handle and the queue-kind dispatch.
+#pragma once
+
+#include "queues.h"
+
+namespace gpq {
+
+// Calls f(view) with the concrete device view of q's queue kind.
+template <class F>
+gpq_status with_view(QueueBase* q, F&& f) {
+  switch (q->cfg.kind) {
+    case GPQ_LAQ: return f(static_cast<LaqQueue*>(q)->view());
+    default:
+      set_error("queue kind not provided");
+      return GPQ_ERR_UNSUPPORTED;
+  }
+}
+
+gpq_status launch_insert_items(QueueBase* q, const Event* d_items, uint32_t n);
+gpq_status launch_peek_items(QueueBase* q, int set_done, Event* d_out, int32_t* d_index, uint32_t* d_found,
+                             uint32_t max);
+
+class PholdModel {
+ public:
+  static gpq_status create(QueueBase* queue, const gpq_phold_config& c, PholdModel** out);
+  ~PholdModel();
+  gpq_status init();
+  gpq_status launch_step();
+  gpq_status step();
+  gpq_status events(uint64_t* out);
+  gpq_status lp_state(uint32_t* counts, uint64_t* hashes);
+  gpq_status trace_hash(uint64_t* out);
+  gpq_status run(gpq_drive_mode mode, uint32_t batch, gpq_des_report* rep);
+  gpq_status profile(uint32_t batch, gpq_kernel_profile* out);
+
+  gpq_phold_config cfg{};
+  QueueBase* q = nullptr;
+  DeviceArena arena;
+  uint32_t* count = nullptr;
+  uint64_t* hash = nullptr;
+  uint32_t* last_ts = nullptr;
+  uint64_t* sum = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  uint32_t graph_batch = 0;
+};
+
+}  // namespace gpq
+
+struct gpq_phold {
+  gpq::PholdModel* impl;
+};

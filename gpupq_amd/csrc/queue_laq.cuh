@@ -1,0 +1,80 @@
+// queue_laq.cuh -- device API of the local array queue (LAQ): one sorted ring per
+// LP.  Reference: src/local_array_queue.cu (insert :486-513, peek :760-779,
+// set_done :710-728, is_empty/length/clear :795-822, insert_or_update :830-846).
+//
+// HBM layout (LP-major so that a wave reading one LP's run is contiguous):
+//   ring  [L][C]  Event   sorted run of LP lp at slots rd[lp] .. rd[lp]+len[lp]-1 (mod C)
+//   stage [L][S]  Event   this step's inserts for LP lp (unsorted), count ins[lp]
+//   head  [L]     u32     ts bits of the ring head, +inf when empty (read coalesced by peek)
+//   rd, len, ins [L] u32
+// Inserts never touch the ring: they land in the staging slots and are merged by
+// queue_post.  The reference appends after fel_write and insertion-sorts in place
+// (src/local_array_queue.cu:486-513, src/sort.cu:329-356).
+#pragma once
+
+#include "gpq_device.h"
+
+namespace gpq {
+
+struct LaqView {
+  Event* ring;
+  Event* stage;
+  uint32_t* rd;
+  uint32_t* len;
+  uint32_t* ins;
+  uint32_t* head;
+  WindowState* ws;
+  unsigned long long* stats;   // work counters, nullptr when not collected
+  uint32_t num_lps;
+  uint32_t cap_log2;
+  uint32_t stage_cap;
+
+  __device__ __forceinline__ uint32_t slots() const { return num_lps; }
+
+  // queue_insert: staged for the owning LP; false (and a device flag) when the
+  // LP's staging area is full.
+  __device__ __forceinline__ bool insert(const Event& e) const {
+    const uint32_t lp = e.dst;
+    const uint32_t pos = atomicAdd(&ins[lp], 1u);
+    if (pos >= stage_cap) {
+      atomicOr(&ws->error, kErrStageOverflow);
+      return false;
+    }
+    store_event(&stage[(size_t)lp * stage_cap + pos], e);
+    return true;
+  }
+
+  // queue_peek: the head of LP lp if its ts is inside the current safe window.
+  __device__ __forceinline__ int peek(Event** item, int lp) const {
+    const uint32_t h = head[lp];
+    if (!(h < ws->wend)) return -1;
+    const uint32_t idx = ((uint32_t)lp << cap_log2) | rd[lp];
+    *item = ring + idx;
+    return (int)idx;
+  }
+
+  // queue_set_done: removes the head; the new head's ts is refreshed for the window scan.
+  __device__ __forceinline__ void set_done(int index) const {
+    const uint32_t lp = (uint32_t)index >> cap_log2;
+    const uint32_t mask = (1u << cap_log2) - 1u;
+    const uint32_t r = (rd[lp] + 1u) & mask;
+    const uint32_t n = len[lp] - 1u;
+    rd[lp] = r;
+    len[lp] = n;
+    head[lp] = n ? ring[(lp << cap_log2) | r].ts : kInfBits;
+  }
+
+  __device__ __forceinline__ bool is_empty(int lp) const { return len[lp] == 0 && ins[lp] == 0; }
+  __device__ __forceinline__ int length(int lp) const { return (int)len[lp]; }
+  __device__ __forceinline__ void clear(int lp) const {
+    const uint32_t mask = (1u << cap_log2) - 1u;
+    rd[lp] = (rd[lp] + len[lp]) & mask;
+    len[lp] = 0;
+    head[lp] = kInfBits;
+  }
+  // PHOLD events carry no identity for ==, so this is a plain insert (as in the reference
+  // without LOCAL_ARRAY_QUEUE_UPDATE_ELEMENTS, src/local_array_queue.cu:830-846).
+  __device__ __forceinline__ void insert_or_update(const Event& e, int) const { insert(e); }
+};
+
+}  // namespace gpq

@@ -1,0 +1,194 @@
+/*
+ * gpupq.h -- C ABI of the MI355X (gfx950) priority-queue library.
+ *
+ * Plain C types only.  Every entry point returns a gpq_status; nothing exits or
+ * aborts.  Handles own all of their device memory (allocated once at create,
+ * released at destroy); no entry point allocates inside a simulation step.
+ * A handle is not re-entrant; different handles may be used from different
+ * host threads.  Device work of a handle is ordered on that handle's HIP stream.
+ *
+ * The reference binds one queue and one model at compile time through macro
+ * renaming (src/queue.h:33-84, src/queue.cuh:7-40, src/config.h:8-21).  Here the
+ * queue kind and every size are runtime fields of gpq_queue_config, and each
+ * reference entry point maps to one function below (cited per function).
+ * The device-side verbs of src/queue.cuh:46-70 (insert / peek / set_done /
+ * is_empty / length / clear / insert_or_update) are the header-only device API
+ * in gpupq_amd/csrc/queue_*.cuh, compiled into the model kernels.
+ */
+#ifndef GPUPQ_H
+#define GPUPQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GPQ_OK = 0,
+  GPQ_ERR_INVALID = 1,     /* bad argument or configuration */
+  GPQ_ERR_CAPACITY = 2,    /* a queue, staging area or heap overflowed (device flag) */
+  GPQ_ERR_HIP = 3,         /* a HIP runtime call failed */
+  GPQ_ERR_NOMEM = 4,       /* device allocation failed or exceeds the HBM budget */
+  GPQ_ERR_STATE = 5,       /* call out of order, or a runtime invariant was violated */
+  GPQ_ERR_UNSUPPORTED = 6  /* combination not provided (e.g. queue kind x model) */
+} gpq_status;
+
+typedef enum {
+  GPQ_LAQ = 0,   /* per-LP sorted ring ("local array queue"), src/local_array_queue.cu */
+  GPQ_LSQ = 1,   /* per-LP splay tree,                        src/local_splay_queue.cu */
+  GPQ_LHQ = 2,   /* per-LP binary heap,                       src/local_heap_queue.cu */
+  GPQ_HQ  = 3,   /* global parallel heap with r-item nodes,   src/heap_queue.cu */
+  GPQ_GAQ = 4    /* global sorted array,                      src/global_array_queue.cu */
+} gpq_queue_kind;
+
+/* 16-byte PHOLD event: ts holds the float32 bits of a timestamp >= 0.
+ * Order: (ts, src, seq).  Reference: struct event {long ts; int node;} (src/queue_item.h:13-16). */
+typedef struct {
+  uint32_t ts;
+  uint32_t dst;   /* owning LP (the reference's `node`, one node per LP) */
+  uint32_t src;   /* LP that generated the event */
+  uint32_t seq;   /* per-source sequence number */
+} gpq_event;
+
+typedef struct {
+  gpq_queue_kind kind;
+  uint32_t num_lps;          /* NUM_LPS (src/config.h:68) */
+  uint32_t capacity_per_lp;  /* LAQ/LSQ/LHQ: items per LP (LAQ: power of two); FEL_SIZE analogue */
+  uint32_t stage_per_lp;     /* inserts one LP may receive per step (<= 64) */
+  uint32_t node_size;        /* HQ: r = HQ_NODE_SIZE (src/config.h:88); must be >= num_lps */
+  uint32_t heap_nodes;       /* HQ: node count (HQ_HEAP_SIZE); 0 = derive from capacity_per_lp */
+  uint64_t total_capacity;   /* GAQ: items in the global array; 0 = num_lps * capacity_per_lp */
+  float lookahead;           /* LOOKAHEAD (src/config.h:65); 0.1 in the float32 workloads */
+  float horizon;             /* events with ts >= horizon are never dequeued (MAX_TS analogue) */
+  uint32_t block_threads;    /* launch width of the per-LP kernels (multiple of 64, <= 1024; 0 = 256) */
+  uint32_t collect_stats;    /* 1: post kernels count algorithmic work (gpq_queue_stats_get); costs atomics */
+} gpq_queue_config;
+
+typedef struct gpq_queue gpq_queue;
+typedef struct gpq_phold gpq_phold;
+
+/* ---- status ---------------------------------------------------------------------------- */
+const char* gpq_status_string(gpq_status s);
+/* Text of the most recent error on the calling thread ("" if none). */
+const char* gpq_last_error(void);
+/* Library version string. */
+const char* gpq_version(void);
+
+/* ---- host queue API (src/queue.h:86-115) ------------------------------------------------ */
+/* queue_init() (src/queue.h:87): allocates and clears the queue. */
+gpq_status gpq_queue_create(const gpq_queue_config* cfg, gpq_queue** out);
+/* queue_finish() (src/queue.h:103) plus release of all device memory. */
+gpq_status gpq_queue_destroy(gpq_queue* q);
+/* queue_post_init() (src/queue.h:92): bulk maintenance after a chunk of initial inserts. */
+gpq_status gpq_queue_post_init(gpq_queue* q);
+/* queue_pre() (src/queue.h:95): maintenance before a model step (selects the dequeue set for HQ/GAQ). */
+gpq_status gpq_queue_pre(gpq_queue* q);
+/* queue_post() (src/queue.h:97): folds the step's inserts in and publishes the next window. */
+gpq_status gpq_queue_post(gpq_queue* q);
+/* queue_get_min_ts() (src/queue.h:100): smallest pending timestamp (+inf if empty); synchronizes. */
+gpq_status gpq_queue_get_min_ts(gpq_queue* q, float* out);
+/* queue_check_phold() (src/queue.h:114, src/heap_queue.cu:1908-1945): pending item count.
+ * PHOLD conserves events, so pending == num_lps * init_per_lp after every post; this replaces
+ * the per-step get_enqueue_count() == get_dequeue_count() check (src/des.cu:31-43). */
+gpq_status gpq_queue_pending(gpq_queue* q, uint64_t* pending);
+/* Device error flags raised so far (bit 0 staging overflow, 1 queue overflow, 2 heap overflow). */
+gpq_status gpq_queue_error_flags(gpq_queue* q, uint32_t* flags);
+/* Waits for all queued device work of the handle. */
+gpq_status gpq_queue_sync(gpq_queue* q);
+
+/* Work counters of the post kernels, accumulated while collect_stats is set: the inputs
+ * of the algorithmic-byte model (sorted insert of k records displacing d: 16*(2k+2d) bytes). */
+typedef struct {
+  uint64_t post_launches;   /* post / post_init launches */
+  uint64_t inserted;        /* staged records folded in (k summed over LPs) */
+  uint64_t displaced;       /* existing records moved to make room (d summed over LPs) */
+  uint64_t merged_lps;      /* LP merges performed */
+  uint64_t scanned_lps;     /* LP heads folded into the window reductions */
+  uint64_t aux_bytes;       /* kind-specific compulsory bytes not covered above (HQ/GAQ merges) */
+} gpq_queue_stats;
+gpq_status gpq_queue_stats_get(gpq_queue* q, gpq_queue_stats* out);
+
+/* Host-driven use of the device verbs (the reference's test kernels do the same,
+ * src/test_laq.cu:115-145, src/test_gaq.cu:41-61): */
+/* queue_insert() for each of n host events, one device thread per event. */
+gpq_status gpq_queue_insert_host(gpq_queue* q, const gpq_event* items, uint32_t n);
+/* queue_peek() for every peek slot (LP for local queues, dequeue position for HQ/GAQ);
+ * writes up to `max` found items and their indices, *found = count.  With set_done != 0
+ * also calls queue_set_done() on each found item. */
+gpq_status gpq_queue_peek_host(gpq_queue* q, int set_done, gpq_event* items, int32_t* index,
+                               uint32_t max, uint32_t* found);
+/* Contents owned by one LP in key order (local queues; HQ/GAQ: all pending items of that LP). */
+gpq_status gpq_queue_lp_contents(gpq_queue* q, uint32_t lp, gpq_event* out, uint32_t cap, uint32_t* n);
+/* queue_length(lp) / queue_is_empty(lp) / queue_clear(lp) (src/queue.cuh:58-66). */
+gpq_status gpq_queue_length(gpq_queue* q, uint32_t lp, uint32_t* n);
+gpq_status gpq_queue_clear_lp(gpq_queue* q, uint32_t lp);
+
+/* ---- PHOLD model (src/model.h:9-19, src/phold.cu) -------------------------------------- */
+typedef struct {
+  uint32_t num_lps;        /* must equal the queue's num_lps */
+  uint32_t init_per_lp;    /* k initial events per LP */
+  uint32_t seed;           /* Philox key {seed, 0} */
+  float lookahead;         /* ts' = (ts + lookahead) + Exp(1) */
+  float end_time;          /* commit exactly the events with ts < end_time */
+  uint32_t p_self_u32;     /* P(dst == self) * 2^32; 0x80000000 = 0.5 */
+  uint32_t checks;         /* 1: per-LP monotone-timestamp check (src/des.cu:52-64) */
+} gpq_phold_config;
+
+gpq_status gpq_phold_create(gpq_queue* q, const gpq_phold_config* cfg, gpq_phold** out);
+gpq_status gpq_phold_destroy(gpq_phold* m);
+/* model_init() (src/phold.cu:124-163): k initial events per LP, each chunk followed by queue_post_init(). */
+gpq_status gpq_phold_init(gpq_phold* m);
+/* model_handle_next() (src/phold.cu:185-192): one event step over all peek slots (asynchronous). */
+gpq_status gpq_phold_handle_next(gpq_phold* m);
+/* model_get_events() (src/phold.cu:169-182): committed events so far. */
+gpq_status gpq_phold_get_events(gpq_phold* m, uint64_t* events);
+/* model_finish() (src/model.h:19). */
+gpq_status gpq_phold_finish(gpq_phold* m);
+/* Per-LP committed counts and chained trace hashes (arrays of num_lps). */
+gpq_status gpq_phold_lp_state(gpq_phold* m, uint32_t* counts, uint64_t* hashes);
+/* Combined trace hash over the per-LP arrays, LPs in order. */
+gpq_status gpq_phold_trace_hash(gpq_phold* m, uint64_t* hash);
+
+/* ---- DES driver (src/des.h:3-5, src/des.cu:134-221) ------------------------------------ */
+typedef enum {
+  GPQ_DRIVE_PLAIN = 0,   /* per step: get_min_ts (host sync + D2H), pre, model step, post */
+  GPQ_DRIVE_GRAPH = 1    /* `batch` steps captured once in a hipGraph, replayed; done flag polled per replay */
+} gpq_drive_mode;
+
+typedef struct {
+  uint64_t steps;           /* model steps launched (graph mode: rounded up to whole batches) */
+  uint64_t windows;         /* windows published by the queue (non-empty steps + trailing no-ops) */
+  uint64_t events;          /* committed events */
+  double sim_seconds;       /* wall clock of the step loop (src/des.cu:148-215) */
+  double events_per_second; /* events / sim_seconds (src/des.cu:95-96) */
+  float final_min_ts;       /* smallest pending timestamp at the end */
+  uint32_t error_flags;
+} gpq_des_report;
+
+/* des_run(): steps the model until no pending event is below end_time. batch = steps per graph. */
+gpq_status gpq_des_run(gpq_phold* m, gpq_drive_mode mode, uint32_t batch, gpq_des_report* report);
+
+/* Per-kernel timing of a des_run: steps are launched eagerly, `batch` at a time, with HIP
+ * events recorded on the queue's stream around the model step kernel and around the
+ * queue's post kernels; durations are averaged over all profiled steps. */
+typedef struct {
+  uint64_t steps;
+  uint64_t events;
+  double step_kernel_us;    /* mean duration of the model step (handle_next) kernel */
+  double post_kernel_us;    /* mean duration of queue_pre + queue_post kernels */
+  double seconds;           /* wall clock of the profiled loop */
+} gpq_kernel_profile;
+gpq_status gpq_des_profile(gpq_phold* m, uint32_t batch, gpq_kernel_profile* out);
+
+/* ---- self-test hooks: the device random-number path evaluated in bulk ------------------- */
+/* out[i] = device Exp(1) sample of draw ((first + i) << 8), i < count (count <= 2^24). */
+gpq_status gpq_selftest_exp(uint32_t first, uint32_t count, float* out);
+/* out[4*i .. 4*i+3] = device Philox4x32-10 of counter {ctr0 + i, ctr1, ctr2, ctr3}, key {key0, key1}. */
+gpq_status gpq_selftest_philox(uint32_t ctr0, uint32_t ctr1, uint32_t ctr2, uint32_t ctr3, uint32_t key0,
+                               uint32_t key1, uint32_t count, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

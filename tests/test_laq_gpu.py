@@ -1,0 +1,144 @@
+"""Local array queue on the GPU: the reference's LAQ known-answer tests restated
+at the queue API level, random merge rounds against a sorted-list model, and the
+overflow error paths (forced from the host through tiny capacities)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def ev(gpq, items):
+    """items: (ts, dst, src, seq) tuples; ts given as a float."""
+    a = np.zeros(len(items), dtype=gpq.event_dtype)
+    for i, (ts, dst, src, seq) in enumerate(items):
+        a[i] = (gpq.f32_bits(ts), dst, src, seq)
+    return a
+
+
+def key_list(arr):
+    return [(int(e["ts"]), int(e["src"]), int(e["seq"])) for e in arr]
+
+
+def test_insert_ringbuffer_fixture(gpq):
+    # src/test_laq.cu:115-145: inserts (n0,1),(n0,2),(n1,3),(n1,4) -> LP0 holds 1,2 and LP1 holds 3,4
+    with gpq.Queue("laq", num_lps=2, capacity_per_lp=8, stage_per_lp=4, lookahead=10.0, horizon=1e9) as q:
+        q.insert(ev(gpq, [(1, 0, 0, 0), (2, 0, 0, 1), (3, 1, 1, 0), (4, 1, 1, 1)]))
+        q.queue_post()
+        assert [gpq.bits_f32(e["ts"]) for e in q.lp_contents(0)] == [1.0, 2.0]
+        assert [gpq.bits_f32(e["ts"]) for e in q.lp_contents(1)] == [3.0, 4.0]
+        assert q.queue_length(0) == 2 and q.queue_length(1) == 2
+
+
+def test_reduce_fixture(gpq):
+    # src/test_laq.cu:93-113: heads 6 (LP0: 6,10) and 5 (LP1: 5,13) -> global min 5
+    with gpq.Queue("laq", num_lps=2, capacity_per_lp=8, stage_per_lp=4, lookahead=10.0, horizon=1e9) as q:
+        q.insert(ev(gpq, [(10, 0, 0, 0), (6, 0, 0, 1), (13, 1, 1, 0), (5, 1, 1, 1)]))
+        q.queue_post()
+        assert q.queue_get_min_ts() == 5.0
+        # insertion order does not matter: contents are sorted (src/test_laq.cu:54-91)
+        assert [gpq.bits_f32(e["ts"]) for e in q.lp_contents(0)] == [6.0, 10.0]
+        assert [gpq.bits_f32(e["ts"]) for e in q.lp_contents(1)] == [5.0, 13.0]
+
+
+def test_peek_is_one_safe_head_per_lp(gpq):
+    # window [5, 5 + 10): LP0 head 6 and LP1 head 5 are safe, LP2 head 20 is not
+    with gpq.Queue("laq", num_lps=3, capacity_per_lp=8, stage_per_lp=4, lookahead=10.0, horizon=1e9) as q:
+        q.insert(ev(gpq, [(6, 0, 0, 0), (7, 0, 0, 1), (5, 1, 1, 0), (20, 2, 2, 0)]))
+        q.queue_post()
+        items, idx = q.peek(set_done=True)
+        assert sorted(gpq.bits_f32(t) for t in items["ts"]) == [5.0, 6.0]
+        q.queue_post()
+        assert q.queue_get_min_ts() == 7.0
+        assert q.queue_check_phold() == 2
+
+
+def test_ties_keep_total_order(gpq):
+    # equal timestamps are ordered by (src, seq)
+    with gpq.Queue("laq", num_lps=1, capacity_per_lp=16, stage_per_lp=8, lookahead=1.0, horizon=1e9) as q:
+        q.insert(ev(gpq, [(3, 0, 5, 1), (3, 0, 2, 9), (3, 0, 5, 0), (1, 0, 9, 9)]))
+        q.queue_post()
+        q.insert(ev(gpq, [(3, 0, 2, 8), (3, 0, 4, 0)]))
+        q.queue_post()
+        got = [(gpq.bits_f32(e["ts"]), int(e["src"]), int(e["seq"])) for e in q.lp_contents(0)]
+        assert got == [(1.0, 9, 9), (3.0, 2, 8), (3.0, 2, 9), (3.0, 4, 0), (3.0, 5, 0), (3.0, 5, 1)]
+
+
+@pytest.mark.parametrize("cap,stage", [(8, 8), (64, 16), (128, 64)])
+def test_random_merge_rounds_against_sorted_model(gpq, cap, stage):
+    rng = np.random.default_rng(cap)
+    L = 37
+    model = [[] for _ in range(L)]
+    seqs = np.zeros(L, np.int64)
+    with gpq.Queue("laq", num_lps=L, capacity_per_lp=cap, stage_per_lp=stage, lookahead=0.25,
+                   horizon=1e9) as q:
+        for rnd in range(25):
+            batch = []
+            for lp in range(L):
+                room = cap - len(model[lp])
+                k = int(rng.integers(0, min(stage, room) + 1))
+                for _ in range(k):
+                    ts = float(np.float32(rng.integers(0, 40) * 0.125 + rnd))
+                    src = int(rng.integers(0, 5))
+                    batch.append((ts, lp, src, int(seqs[src])))
+                    seqs[src] += 1
+            order = rng.permutation(len(batch))
+            q.insert(ev(gpq, [batch[i] for i in order]))
+            for (ts, lp, src, seq) in batch:
+                model[lp].append((gpq.f32_bits(ts), src, seq))
+            q.queue_post()
+            for lp in range(L):
+                model[lp].sort()
+            # one dequeue round: heads inside [gmin, gmin + 0.25)
+            heads = [m[0] for m in model if m]
+            if heads:
+                gmin = min(h[0] for h in heads)
+                assert q.queue_get_min_ts() == gpq.bits_f32(gmin)
+                wend = gpq.f32_bits(np.float32(gpq.bits_f32(gmin)) + np.float32(0.25))
+                items, _ = q.peek(set_done=True)
+                expect = sorted(m[0] for m in model if m and m[0][0] < wend)
+                assert sorted(key_list(items)) == expect
+                for lp in range(L):
+                    if model[lp] and model[lp][0][0] < wend:
+                        model[lp].pop(0)
+            for lp in (0, 7, L - 1):
+                assert key_list(q.lp_contents(lp)) == model[lp]
+        assert q.queue_check_phold() == sum(len(m) for m in model)
+        assert q.error_flags() == 0
+
+
+def test_stage_overflow_is_reported(gpq):
+    with gpq.Queue("laq", num_lps=2, capacity_per_lp=16, stage_per_lp=2, lookahead=1.0, horizon=1e9) as q:
+        q.insert(ev(gpq, [(1, 0, 0, 0), (2, 0, 0, 1), (3, 0, 0, 2)]))
+        assert q.error_flags() & 1
+        q.queue_post()
+        assert len(q.lp_contents(0)) == 2
+
+
+def test_queue_overflow_is_reported(gpq):
+    with gpq.Queue("laq", num_lps=2, capacity_per_lp=4, stage_per_lp=8, lookahead=1.0, horizon=1e9) as q:
+        q.insert(ev(gpq, [(float(i), 1, 0, i) for i in range(5)]))
+        q.queue_post()
+        assert q.error_flags() & 2
+        assert q.queue_length(1) == 0
+
+
+def test_phold_overflow_surfaces_as_capacity_error(gpq):
+    q = gpq.Queue("laq", num_lps=64, capacity_per_lp=16, stage_per_lp=4, lookahead=0.1, horizon=50.0)
+    m = gpq.Phold(q, init_per_lp=12, seed=42, end_time=50.0)
+    m.model_init()
+    with pytest.raises(gpq.GpqError) as ei:
+        gpq.des_run(m, mode="graph", batch=8)
+    assert ei.value.status == 2
+    m.model_finish()
+    q.queue_finish()
+
+
+def test_clear_and_length(gpq):
+    with gpq.Queue("laq", num_lps=2, capacity_per_lp=8, stage_per_lp=4, lookahead=1.0, horizon=1e9) as q:
+        q.insert(ev(gpq, [(1, 0, 0, 0), (2, 0, 0, 1), (3, 1, 1, 0)]))
+        q.queue_post()
+        q.queue_clear(0)
+        q.sync()
+        assert q.queue_length(0) == 0 and q.queue_length(1) == 1
+        q.queue_post()
+        assert q.queue_get_min_ts() == 3.0

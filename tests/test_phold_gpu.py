@@ -1,0 +1,101 @@
+"""PHOLD on the GPU against the CPU oracle: bit-exact per-LP committed counts and
+trace hashes (small workloads, oracle run live), the committed golden records
+(1024-LP and 2^20-LP workloads), and size-independent invariants (conservation,
+per-LP monotone timestamps, plain == graph driving)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+QUEUE_ARGS = {
+    "laq": dict(capacity_per_lp=128, stage_per_lp=16),
+}
+
+
+def run_gpu(gpq, kind, L, k, T, mode="graph", batch=16, checks=True, **over):
+    args = dict(QUEUE_ARGS[kind])
+    args.update(over)
+    q = gpq.Queue(kind, num_lps=L, lookahead=0.1, horizon=T, **args)
+    m = gpq.Phold(q, init_per_lp=k, seed=42, end_time=T, checks=checks)
+    m.model_init()
+    assert q.queue_check_phold() == L * k
+    rep = gpq.des_run(m, mode=mode, batch=batch)
+    cnt, hsh = m.lp_state()
+    out = dict(rep=rep, cnt=cnt, hsh=hsh, hash=m.trace_hash(), pending=q.queue_check_phold(),
+               flags=q.error_flags())
+    m.model_finish()
+    q.queue_finish()
+    return out
+
+
+def test_device_exp_sampler_bit_exact(gpq, oracle):
+    import ctypes
+    n = 1 << 24
+    dev = np.empty(n, dtype=np.float32)
+    gpq._lib.check(gpq.lib.gpq_selftest_exp(0, n, dev.ctypes.data_as(ctypes.c_void_p)), "selftest_exp")
+    ref = oracle.exp_table(0, n)
+    assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32))
+
+
+def test_device_philox_bit_exact(gpq, oracle):
+    import ctypes
+    for ctr, key in [((0, 0, 0, 0), (0, 0)), ((123, 77, 1, 0), (42, 0)), ((0xFFFFFFF0, 5, 9, 3), (7, 99))]:
+        n = 4096
+        dev = np.empty(4 * n, dtype=np.uint32)
+        gpq._lib.check(gpq.lib.gpq_selftest_philox(*ctr, *key, n, dev.ctypes.data_as(ctypes.c_void_p)), "philox")
+        for i in (0, 1, 17, n - 1):
+            c = [(ctr[0] + i) & 0xFFFFFFFF, ctr[1], ctr[2], ctr[3]]
+            assert list(dev[4 * i:4 * i + 4]) == oracle.philox(c, list(key))
+
+
+@pytest.mark.parametrize("kind", sorted(QUEUE_ARGS))
+@pytest.mark.parametrize("L,k,T", [(1, 1, 20.0), (4, 2, 5.0), (64, 8, 20.0), (97, 3, 7.5), (4096, 16, 20.0)])
+def test_phold_matches_oracle_per_lp(gpq, oracle, kind, L, k, T):
+    res, cnt, hsh = oracle.phold("seq", L, k, T)
+    g = run_gpu(gpq, kind, L, k, T)
+    assert g["flags"] == 0
+    assert g["rep"]["events"] == res.events
+    assert np.array_equal(g["cnt"], cnt)
+    assert np.array_equal(g["hsh"], hsh)
+    assert g["hash"] == res.trace_hash
+    assert g["pending"] == L * k
+
+
+@pytest.mark.parametrize("kind", sorted(QUEUE_ARGS))
+def test_phold_1024_lps_matches_golden(gpq, golden, kind):
+    gd = golden["phold_1024_k16_t100"]
+    g = run_gpu(gpq, kind, 1024, 16, 100.0)
+    assert g["rep"]["events"] == gd["events"] == 1489117
+    assert "%016x" % g["hash"] == gd["trace_hash"]
+    assert [int(x) for x in g["cnt"]] == gd["lp_count"]
+    assert ["%016x" % int(x) for x in g["hsh"]] == gd["lp_hash"]
+
+
+def test_phold_plain_equals_graph(gpq):
+    a = run_gpu(gpq, "laq", 512, 16, 30.0, mode="plain")
+    b = run_gpu(gpq, "laq", 512, 16, 30.0, mode="graph", batch=7)
+    assert a["hash"] == b["hash"] and a["rep"]["events"] == b["rep"]["events"]
+    # one window per post: 16 from populating (one chunk per initial event), then one per step
+    assert a["rep"]["windows"] == a["rep"]["steps"] + 16
+    assert b["rep"]["windows"] == b["rep"]["steps"] + 16 and b["rep"]["steps"] % 7 == 0
+
+
+@pytest.mark.parametrize("kind", sorted(QUEUE_ARGS))
+def test_phold_65536_lps_matches_golden(gpq, golden, kind):
+    gd = golden["mid_l65536_k32_t10"]
+    g = run_gpu(gpq, kind, 65536, 32, 10.0, checks=False)
+    assert g["rep"]["events"] == gd["events"]
+    assert "%016x" % g["hash"] == gd["trace_hash"]
+    assert g["pending"] == 65536 * 32
+
+
+@pytest.mark.parametrize("kind", sorted(QUEUE_ARGS))
+def test_phold_2p20_lps_matches_golden(gpq, golden, kind):
+    gd = golden.get("phold_2p20_k32_t50")
+    if gd is None:
+        pytest.skip("2^20-LP golden record not generated")
+    g = run_gpu(gpq, kind, 1 << 20, 32, 50.0, checks=False, batch=64)
+    assert g["flags"] == 0
+    assert g["rep"]["events"] == gd["events"]
+    assert "%016x" % g["hash"] == gd["trace_hash"]
+    assert g["pending"] == (1 << 20) * 32
