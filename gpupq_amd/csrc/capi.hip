@@ -48,6 +48,51 @@ gpq_status QueueBase::stats_get(gpq_queue_stats* out) {
   return GPQ_OK;
 }
 
+__global__ void window_publish_kernel(const uint32_t* partial, uint32_t count, WindowState* ws, float lookahead,
+                                      float horizon) {
+  __shared__ uint32_t s_w[16];
+  uint32_t m = kInfBits;
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) m = partial[i] < m ? partial[i] : m;
+  m = wave_min_u32(m);
+  if (lane_id() == 0) s_w[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t g = s_w[0];
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) g = s_w[i] < g ? s_w[i] : g;
+    float wend = bits_to_f(g) + lookahead;   // inf + lookahead stays inf
+    if (horizon < wend) wend = horizon;
+    ws->gmin = g;
+    ws->wend = f_to_bits(wend);
+    ws->done = (g == kInfBits || !(bits_to_f(g) < horizon)) ? 1u : 0u;
+    ws->windows += 1;
+  }
+}
+
+__global__ void sum_u32_kernel(const uint32_t* v, uint32_t count, unsigned long long* sum) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t x = i < count ? v[i] : 0u;
+  unsigned long long w = x;
+  for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off, kWave);
+  if (lane_id() == 0 && w) atomicAdd(sum, w);
+}
+
+gpq_status QueueBase::sum_u32(const uint32_t* d_values, uint32_t count, uint64_t* tmp, uint64_t* out) {
+  GPQ_HIP(hipMemsetAsync(tmp, 0, sizeof(uint64_t), stream));
+  hipLaunchKernelGGL(sum_u32_kernel, dim3(ceil_div(count, 256) ? ceil_div(count, 256) : 1), dim3(256), 0, stream,
+                     d_values, count, reinterpret_cast<unsigned long long*>(tmp));
+  GPQ_HIP(hipGetLastError());
+  GPQ_HIP(hipMemcpyAsync(out, tmp, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+  GPQ_HIP(hipStreamSynchronize(stream));
+  return GPQ_OK;
+}
+
+gpq_status QueueBase::publish_window() {
+  hipLaunchKernelGGL(window_publish_kernel, dim3(1), dim3(1024), 0, stream, partial, partial_slots, ws,
+                     cfg.lookahead, cfg.horizon);
+  GPQ_HIP(hipGetLastError());
+  return GPQ_OK;
+}
+
 gpq_status QueueBase::window(WindowState* out) {
   GPQ_HIP(hipMemcpyAsync(host_ws, ws, sizeof(WindowState), hipMemcpyDeviceToHost, stream));
   GPQ_HIP(hipStreamSynchronize(stream));
@@ -65,6 +110,8 @@ gpq_status QueueBase::error_flags(uint32_t* flags) {
 QueueBase* make_queue(const gpq_queue_config& cfg) {
   switch (cfg.kind) {
     case GPQ_LAQ: return new LaqQueue(cfg);
+    case GPQ_LHQ: return new LhqQueue(cfg);
+    case GPQ_LSQ: return new LsqQueue(cfg);
     default: return nullptr;
   }
 }

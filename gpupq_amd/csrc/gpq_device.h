@@ -113,6 +113,18 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
   return v;
 }
 
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, kWave);
+  return v;
+}
+
+// Adds v (summed over the wave) to a work counter; every lane of the wave must call it.
+__device__ __forceinline__ void stats_add(unsigned long long* stats, int slot, uint32_t v) {
+  const uint32_t s = wave_sum_u32(v);
+  if (lane_id() == 0 && s) atomicAdd(&stats[slot], (unsigned long long)s);
+}
+
 __device__ __forceinline__ Event shfl_event(const Event& e, int src_lane) {
   return Event{(uint32_t)__shfl((int)e.ts, src_lane, kWave), (uint32_t)__shfl((int)e.dst, src_lane, kWave),
                (uint32_t)__shfl((int)e.src, src_lane, kWave), (uint32_t)__shfl((int)e.seq, src_lane, kWave)};
@@ -125,7 +137,7 @@ struct WindowState {
   uint32_t gmin;
   uint32_t wend;
   uint32_t done;
-  uint32_t blocks_arrived;   // last-block-reduction ticket
+  uint32_t reserved;
   uint64_t windows;          // number of windows computed
   uint32_t error;            // GPQ_FLAG_* bits
   uint32_t pad;
@@ -138,47 +150,38 @@ enum : uint32_t {
   kErrBadState = 8u,
 };
 
-// Block-wide min of one value per thread, then the last block to finish folds
-// the per-block partials and publishes the window.  Grid-stride safe: `partial`
-// has one slot per block.  blockDim.x must be a multiple of 64 and <= 1024.
-__device__ __forceinline__ void window_reduce_publish(uint32_t v, uint32_t* partial, WindowState* ws,
-                                                      float lookahead, float horizon) {
+// queue_insert for the per-LP queues: the event is staged for its owning LP and folded
+// in by queue_post; false (plus a device flag) when the LP's staging area is full.
+__device__ __forceinline__ bool stage_insert(Event* stage, uint32_t* ins, uint32_t stage_cap, WindowState* ws,
+                                             const Event& e) {
+  const uint32_t pos = atomicAdd(&ins[e.dst], 1u);
+  if (pos >= stage_cap) {
+    atomicOr(&ws->error, kErrStageOverflow);
+    return false;
+  }
+  store_event(&stage[(size_t)e.dst * stage_cap + pos], e);
+  return true;
+}
+
+// Block-wide min of one value per thread, written to partial[blockIdx.x].  No fence:
+// a device-scope fence on gfx950 writes back the XCD's L2 (buffer_wbl2), far too
+// costly per block; the partials are folded by window_publish_kernel, launched
+// after the producing kernel on the same stream.  blockDim.x: multiple of 64, <= 1024.
+__device__ __forceinline__ void block_min_to_partial(uint32_t v, uint32_t* partial) {
   __shared__ uint32_t s_wmin[16];
-  __shared__ bool s_last;
   v = wave_min_u32(v);
-  const int w = threadIdx.x / kWave;
-  if (lane_id() == 0) s_wmin[w] = v;
+  if (lane_id() == 0) s_wmin[threadIdx.x / kWave] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t m = s_wmin[0];
     for (int i = 1; i < (int)(blockDim.x / kWave); ++i) m = s_wmin[i] < m ? s_wmin[i] : m;
     partial[blockIdx.x] = m;
-    __threadfence();
-    const uint32_t ticket = atomicAdd(&ws->blocks_arrived, 1u);
-    s_last = (ticket == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  uint32_t m = kInfBits;
-  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
-    const uint32_t p = __atomic_load_n(&partial[i], __ATOMIC_RELAXED);
-    m = p < m ? p : m;
-  }
-  m = wave_min_u32(m);
-  if (lane_id() == 0) s_wmin[w] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t g = s_wmin[0];
-    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) g = s_wmin[i] < g ? s_wmin[i] : g;
-    float wend = bits_to_f(g) + lookahead;            // inf + la stays inf
-    if (horizon < wend) wend = horizon;
-    ws->gmin = g;
-    ws->wend = f_to_bits(wend);
-    ws->done = (g == kInfBits || !(bits_to_f(g) < horizon)) ? 1u : 0u;
-    ws->windows += 1;
-    ws->blocks_arrived = 0;
   }
 }
+
+// One block: folds the per-block minima and publishes gmin, the window end
+// min(gmin + lookahead, horizon) and the done flag.
+__global__ void window_publish_kernel(const uint32_t* partial, uint32_t count, WindowState* ws, float lookahead,
+                                      float horizon);
 
 }  // namespace gpq

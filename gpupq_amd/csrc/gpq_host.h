@@ -93,9 +93,13 @@ class QueueBase {
   // Number of queue_peek() slots the model iterates over.
   virtual uint32_t peek_slots() const { return cfg.num_lps; }
 
+  // Folds partial[0 .. partial_slots) into the window state (one small kernel on stream).
+  gpq_status publish_window();
   gpq_status window(WindowState* out);   // synchronous copy of the device window state
   gpq_status error_flags(uint32_t* flags);
   gpq_status stats_get(gpq_queue_stats* out);
+  // *out = sum of a device u32 array (synchronous); tmp is one device u64.
+  gpq_status sum_u32(const uint32_t* d_values, uint32_t count, uint64_t* tmp, uint64_t* out);
 
   gpq_queue_config cfg;
   hipStream_t stream = nullptr;

@@ -10,6 +10,8 @@ template <class F>
 gpq_status with_view(QueueBase* q, F&& f) {
   switch (q->cfg.kind) {
     case GPQ_LAQ: return f(static_cast<LaqQueue*>(q)->view());
+    case GPQ_LHQ: return f(static_cast<LhqQueue*>(q)->view());
+    case GPQ_LSQ: return f(static_cast<LsqQueue*>(q)->view());
     default:
       set_error("queue kind not provided");
       return GPQ_ERR_UNSUPPORTED;

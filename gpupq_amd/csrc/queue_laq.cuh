@@ -33,16 +33,7 @@ struct LaqView {
 
   // queue_insert: staged for the owning LP; false (and a device flag) when the
   // LP's staging area is full.
-  __device__ __forceinline__ bool insert(const Event& e) const {
-    const uint32_t lp = e.dst;
-    const uint32_t pos = atomicAdd(&ins[lp], 1u);
-    if (pos >= stage_cap) {
-      atomicOr(&ws->error, kErrStageOverflow);
-      return false;
-    }
-    store_event(&stage[(size_t)lp * stage_cap + pos], e);
-    return true;
-  }
+  __device__ __forceinline__ bool insert(const Event& e) const { return stage_insert(stage, ins, stage_cap, ws, e); }
 
   // queue_peek: the head of LP lp if its ts is inside the current safe window.
   __device__ __forceinline__ int peek(Event** item, int lp) const {

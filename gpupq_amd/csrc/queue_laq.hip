@@ -15,73 +15,93 @@ namespace gpq {
 namespace {
 
 constexpr int kLaqBlock = 256;   // LPs per post block = threads per block (4 waves)
+constexpr int kSlots = 8;        // ring records held per lane: a group of G lanes covers 8G records
 
-// Merges the staged events of LP `lp` into its sorted ring; one wave, lanes in lockstep.
-// Old record j moves to j + c_j where c_j = number of new events with a smaller key;
-// new event m lands at rank_m + (number of old records with a smaller key).  The
-// displaced tail is processed from the back in 64-record chunks and the walk stops
-// at the first chunk whose lowest record does not move, so only displaced records
-// are read and written (plus at most one chunk of non-moving records).
-// Returns the number of existing records moved (d), for the work counters.
-__device__ uint32_t laq_merge_lp(const LaqView& q, uint32_t lp, int lane) {
-  const uint32_t staged = q.ins[lp];
-  const uint32_t k = staged < q.stage_cap ? staged : q.stage_cap;
-  const uint32_t n = q.len[lp];
-  const uint32_t r = q.rd[lp];
+__device__ __forceinline__ uint32_t group_popc(uint64_t ballot, int gbase, int G) {
+  const uint64_t bits = G == kWave ? ballot : (ballot >> gbase) & ((1ull << G) - 1ull);
+  return (uint32_t)__popcll(bits);
+}
+
+// Merges the k staged events of one LP into its sorted run of n records at ring slot r,
+// using a group of G lanes (lane g of the group, group starting at wave lane gbase).
+// All n records and the k staged events are loaded at once (record j in lane j % G,
+// register slot j / G).  Old record j ends at final index j + c_j, c_j = number of new
+// events with a smaller key; new event m at rank_m + (old records with a smaller key).
+// The run may grow at either end: it starts a slots earlier, a in [0, k], so old record
+// j moves iff c_j != a; a is chosen to move the fewest records.  a = 0 is the
+// reference's backward insertion (src/sort.cu:329-356), a = k its forward variant
+// (src/sort.cu:358-379), chosen here per LP and step.  Returns the records moved.
+template <int G>
+__device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, uint32_t k, uint32_t n, uint32_t r, int g,
+                                    int gbase) {
   const uint32_t mask = (1u << q.cap_log2) - 1u;
   Event* ring = q.ring + ((size_t)lp << q.cap_log2);
-  if (n + k > mask + 1u) {
-    if (lane == 0) {
-      atomicOr(&q.ws->error, kErrQueueOverflow);
-      q.ins[lp] = 0;
-    }
-    return 0;
-  }
   Event e{kInfBits, 0u, 0u, 0u};
-  if (lane < (int)k) e = load_event(&q.stage[(size_t)lp * q.stage_cap + lane]);
+  if (g < (int)k) e = load_event(&q.stage[(size_t)lp * q.stage_cap + g]);
+  Event old[kSlots];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    const uint32_t j = (uint32_t)(s * G + g);
+    old[s] = Event{0u, 0u, 0u, 0u};
+    if (j < n) old[s] = load_event(&ring[(r + j) & mask]);
+  }
   uint32_t rank = 0;
   for (uint32_t m = 0; m < k; ++m) {
-    const Event o = shfl_event(e, (int)m);
-    if (lane < (int)k && (key_less(o, e) || (!key_less(e, o) && (int)m < lane))) ++rank;
+    const Event o = shfl_event(e, gbase + (int)m);
+    if (g < (int)k && (key_less(o, e) || (!key_less(e, o) && (int)m < g))) ++rank;
   }
+  uint32_t c[kSlots];
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) c[s] = 0;
   uint32_t greater = 0;   // lane m < k: old records with a key above new event m
-  uint32_t moved = 0;
-  if (n > 0) {
-    for (int base = (int)n - kWave;; base -= kWave) {
-      const int j = base + lane;
-      const bool valid = j >= 0;
-      Event old{0u, 0u, 0u, 0u};
-      if (valid) old = load_event(&ring[(r + (uint32_t)j) & mask]);
-      uint32_t c = 0;
-      for (uint32_t m = 0; m < k; ++m) {
-        const Event nm = shfl_event(e, (int)m);
-        const bool lt = valid && key_less(nm, old);
-        c += lt ? 1u : 0u;
-        const uint64_t b = __ballot(lt);
-        if (lane == (int)m) greater += (uint32_t)__popcll(b);
-      }
-      if (valid && c) store_event(&ring[(r + (uint32_t)j + c) & mask], old);
-      moved += (uint32_t)__popcll(__ballot(valid && c != 0u));
-      const int low = base < 0 ? -base : 0;
-      const uint32_t c_low = (uint32_t)__shfl((int)c, low, kWave);
-      if (base <= 0 || c_low == 0) break;
+  for (uint32_t m = 0; m < k; ++m) {
+    const Event nm = shfl_event(e, gbase + (int)m);
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+      const bool lt = (uint32_t)(s * G + g) < n && key_less(nm, old[s]);
+      c[s] += lt ? 1u : 0u;
+      cnt += group_popc(__ballot(lt), gbase, G);
+    }
+    if (g == (int)m) greater = cnt;
+  }
+  uint32_t best_a = 0, best_stay = 0;
+  for (uint32_t a = 0; a <= k; ++a) {
+    uint32_t stay = 0;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) stay += group_popc(__ballot((uint32_t)(s * G + g) < n && c[s] == a), gbase, G);
+    if (a == 0 || stay > best_stay) {
+      best_stay = stay;
+      best_a = a;
     }
   }
-  if (lane < (int)k) {
-    const uint32_t pos = rank + (n - greater);
-    store_event(&ring[(r + pos) & mask], e);
-    if (pos == 0) q.head[lp] = e.ts;
+  const uint32_t r_new = (r - best_a) & mask;
+#pragma unroll
+  for (int s = 0; s < kSlots; ++s) {
+    const uint32_t j = (uint32_t)(s * G + g);
+    if (j < n && c[s] != best_a) store_event(&ring[(r_new + j + c[s]) & mask], old[s]);
   }
-  if (lane == 0) {
+  if (g < (int)k) {
+    const uint32_t f = rank + (n - greater);
+    store_event(&ring[(r_new + f) & mask], e);
+    if (f == 0) q.head[lp] = e.ts;
+  }
+  if (g == 0) {
+    q.rd[lp] = r_new;
     q.len[lp] = n + k;
     q.ins[lp] = 0;
   }
-  return moved;
+  return n - best_stay;
 }
 
-__global__ __launch_bounds__(kLaqBlock) void laq_post_kernel(LaqView q, uint32_t* partial, float lookahead,
-                                                             float horizon) {
+// One block owns kLaqBlock LPs.  Phase 1 reads their counters coalesced into LDS and
+// lists the LPs that received inserts; phase 2 merges them, one LP per group of G
+// lanes (kLaqBlock / G LPs in flight per block); phase 3 folds the block's heads into
+// the window reduction.
+template <int G>
+__global__ __launch_bounds__(kLaqBlock) void laq_post_kernel(LaqView q, uint32_t* partial) {
   __shared__ uint32_t s_work[kLaqBlock];
+  __shared__ uint32_t s_k[kLaqBlock], s_n[kLaqBlock], s_r[kLaqBlock];
   __shared__ uint32_t s_count;
   __shared__ uint32_t s_inserted;
   __shared__ uint32_t s_moved;
@@ -92,15 +112,36 @@ __global__ __launch_bounds__(kLaqBlock) void laq_post_kernel(LaqView q, uint32_t
     s_moved = 0;
   }
   __syncthreads();
-  const uint32_t staged = lp < q.num_lps ? q.ins[lp] : 0u;
-  if (staged != 0u) s_work[atomicAdd(&s_count, 1u)] = lp;
-  if (q.stats && staged) atomicAdd(&s_inserted, staged < q.stage_cap ? staged : q.stage_cap);
+  uint32_t staged = 0;
+  if (lp < q.num_lps) {
+    staged = q.ins[lp];
+    if (staged) {
+      const uint32_t k = staged < q.stage_cap ? staged : q.stage_cap;
+      const uint32_t n = q.len[lp];
+      if (n + k > (1u << q.cap_log2)) {
+        atomicOr(&q.ws->error, kErrQueueOverflow);
+        q.ins[lp] = 0;
+      } else {
+        const uint32_t slot = atomicAdd(&s_count, 1u);
+        s_work[slot] = threadIdx.x;
+        s_k[threadIdx.x] = k;
+        s_n[threadIdx.x] = n;
+        s_r[threadIdx.x] = q.rd[lp];
+        if (q.stats) atomicAdd(&s_inserted, k);
+      }
+    }
+  }
   __syncthreads();
-  const int wave = (int)threadIdx.x / kWave;
   const int lane = lane_id();
+  const int g = lane % G;
+  const int gbase = lane - g;
+  const int group = (int)threadIdx.x / G;
   uint32_t moved = 0;
-  for (uint32_t i = (uint32_t)wave; i < s_count; i += kLaqBlock / kWave) moved += laq_merge_lp(q, s_work[i], lane);
-  if (q.stats && lane == 0 && moved) atomicAdd(&s_moved, moved);
+  for (uint32_t i = (uint32_t)group; i < s_count; i += kLaqBlock / G) {
+    const uint32_t t = s_work[i];
+    moved += laq_merge_group<G>(q, blockIdx.x * kLaqBlock + t, s_k[t], s_n[t], s_r[t], g, gbase);
+  }
+  if (q.stats && g == 0 && moved) atomicAdd(&s_moved, moved);
   __syncthreads();
   if (q.stats && threadIdx.x == 0) {   // work counters: one device atomic per block and counter
     const uint32_t lps_here = min((uint32_t)kLaqBlock, q.num_lps - blockIdx.x * kLaqBlock);
@@ -110,7 +151,7 @@ __global__ __launch_bounds__(kLaqBlock) void laq_post_kernel(LaqView q, uint32_t
     atomicAdd(&q.stats[3], (unsigned long long)lps_here);
   }
   const uint32_t v = lp < q.num_lps ? q.head[lp] : kInfBits;
-  window_reduce_publish(v, partial, q.ws, lookahead, horizon);
+  block_min_to_partial(v, partial);
 }
 
 __global__ void laq_init_kernel(LaqView q) {
@@ -125,23 +166,9 @@ __global__ void laq_init_kernel(LaqView q) {
     q.ws->gmin = kInfBits;
     q.ws->wend = 0;
     q.ws->done = 0;
-    q.ws->blocks_arrived = 0;
+    q.ws->reserved = 0;
     q.ws->windows = 0;
     q.ws->error = 0;
-  }
-}
-
-__global__ void laq_pending_kernel(LaqView q, unsigned long long* sum) {
-  __shared__ unsigned long long s[kLaqBlock / kWave];
-  const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long v = lp < q.num_lps ? (unsigned long long)q.len[lp] : 0ull;
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  if (lane_id() == 0) s[threadIdx.x / kWave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) t += s[i];
-    atomicAdd(sum, t);
   }
 }
 
@@ -153,12 +180,16 @@ gpq_status LaqQueue::init() {
   GPQ_CHECK_ARG(is_pow2(cfg.capacity_per_lp) && cfg.capacity_per_lp >= 2,
                 "LAQ: capacity_per_lp must be a power of two >= 2");
   GPQ_CHECK_ARG(cfg.stage_per_lp >= 1 && cfg.stage_per_lp <= 64, "LAQ: stage_per_lp must be in [1, 64]");
+  GPQ_CHECK_ARG(cfg.capacity_per_lp <= 8u * 64u, "LAQ: capacity_per_lp must be <= 512");
   GPQ_CHECK_ARG((uint64_t)cfg.num_lps * cfg.capacity_per_lp < (1ull << 31),
                 "LAQ: num_lps * capacity_per_lp must be < 2^31 (int item indices, src/queue.cuh:51)");
   const uint32_t L = cfg.num_lps;
   v_.num_lps = L;
   v_.cap_log2 = log2u(cfg.capacity_per_lp);
   v_.stage_cap = cfg.stage_per_lp;
+  // lanes per LP merge: enough to hold the run (8 records per lane) and one staged event per lane
+  group_ = 16;
+  while (group_ < 64 && (group_ * 8u < cfg.capacity_per_lp || group_ < cfg.stage_per_lp)) group_ *= 2;
   GPQ_TRY(arena.alloc(&v_.ring, (size_t)L << v_.cap_log2));
   GPQ_TRY(arena.alloc(&v_.stage, (size_t)L * v_.stage_cap));
   GPQ_TRY(arena.alloc(&v_.rd, L));
@@ -176,21 +207,18 @@ gpq_status LaqQueue::init() {
 }
 
 gpq_status LaqQueue::post() {
-  hipLaunchKernelGGL(laq_post_kernel, dim3(partial_slots), dim3(kLaqBlock), 0, stream, v_, partial, cfg.lookahead,
-                     cfg.horizon);
+  const dim3 grid(partial_slots), block(kLaqBlock);
+  if (group_ == 16)
+    hipLaunchKernelGGL(laq_post_kernel<16>, grid, block, 0, stream, v_, partial);
+  else if (group_ == 32)
+    hipLaunchKernelGGL(laq_post_kernel<32>, grid, block, 0, stream, v_, partial);
+  else
+    hipLaunchKernelGGL(laq_post_kernel<64>, grid, block, 0, stream, v_, partial);
   GPQ_HIP(hipGetLastError());
-  return GPQ_OK;
+  return publish_window();
 }
 
-gpq_status LaqQueue::pending(uint64_t* n) {
-  GPQ_HIP(hipMemsetAsync(sum_, 0, sizeof(uint64_t), stream));
-  hipLaunchKernelGGL(laq_pending_kernel, dim3(partial_slots), dim3(kLaqBlock), 0, stream, v_,
-                     reinterpret_cast<unsigned long long*>(sum_));
-  GPQ_HIP(hipGetLastError());
-  GPQ_HIP(hipMemcpyAsync(n, sum_, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
-  GPQ_HIP(hipStreamSynchronize(stream));
-  return GPQ_OK;
-}
+gpq_status LaqQueue::pending(uint64_t* n) { return sum_u32(v_.len, cfg.num_lps, sum_, n); }
 
 gpq_status LaqQueue::length(uint32_t lp, uint32_t* n) {
   GPQ_CHECK_ARG(lp < cfg.num_lps, "lp out of range");

@@ -9,6 +9,8 @@ pytestmark = pytest.mark.gpu
 
 QUEUE_ARGS = {
     "laq": dict(capacity_per_lp=128, stage_per_lp=16),
+    "lhq": dict(capacity_per_lp=128, stage_per_lp=16),
+    "lsq": dict(capacity_per_lp=128, stage_per_lp=16),
 }
 
 
