@@ -32,13 +32,19 @@ WORKLOADS = {
 }
 QUEUES = {
     "laq": dict(capacity_per_lp=128, stage_per_lp=16),
+    "lhq": dict(capacity_per_lp=128, stage_per_lp=16),
+    "lsq": dict(capacity_per_lp=128, stage_per_lp=16),
 }
+DRAIN = False
 
 
 def make(gpq, kind, wl, stats=False, checks=False):
+    args = dict(QUEUES[kind])
+    if DRAIN:
+        args["stage_per_lp"] = 32      # a drained window can send ~3 events to one LP on average
     q = gpq.Queue(kind, num_lps=wl["num_lps"], lookahead=0.1, horizon=wl["end_time"], block_threads=256,
-                  collect_stats=stats, **QUEUES[kind])
-    m = gpq.Phold(q, init_per_lp=wl["init_per_lp"], seed=42, end_time=wl["end_time"], checks=checks)
+                  collect_stats=stats, **args)
+    m = gpq.Phold(q, init_per_lp=wl["init_per_lp"], seed=42, end_time=wl["end_time"], checks=checks, drain=DRAIN)
     m.model_init()
     return q, m
 
@@ -64,7 +70,7 @@ def roofline(gpq, kind, wl, log):
     # model step: one head word per LP, and per committed event the head record (16), the
     # per-LP state read+write (count 8, hash 16, ring indices 16, head 4 + next head 4),
     # the staging counter (4) and the staged record (16)
-    events_per_step = rep["events"] / max(prof["steps"], 1)
+    events_per_step = rep["events"] / max(d["post_launches"], 1)
     step_bytes = 4 * L + 88 * events_per_step
     kernels = {
         "queue_post": {"us": prof["post_kernel_us"], "alg_bytes": post_bytes},
@@ -96,6 +102,9 @@ def main():
     ap.add_argument("--queue", default="laq", choices=sorted(QUEUES))
     ap.add_argument("--workload", default="phold_2p20_k32_t50", choices=sorted(WORKLOADS))
     ap.add_argument("--batch", type=int, default=64, help="steps per hipGraph replay")
+    ap.add_argument("--one-event", action="store_true",
+                    help="commit one event per LP per window (the reference's step) instead of draining each "
+                         "LP's safe window; the committed trace is the same")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -108,6 +117,8 @@ def main():
         print("[bench] " + msg, file=sys.stderr, flush=True)
 
     import gpupq_amd as gpq
+    global DRAIN
+    DRAIN = not args.one_event and args.queue in ("laq", "lhq", "lsq")
     wl = WORKLOADS[args.workload]
     golden = None
     gpath = os.path.join(ROOT, "tests", "golden", "phold_golden.json")
@@ -143,6 +154,7 @@ def main():
         "config": {"workload": args.workload, "queue": args.queue, "num_lps": wl["num_lps"],
                    "init_per_lp": wl["init_per_lp"], "end_time": wl["end_time"], "lookahead": 0.1,
                    "drive": "hipgraph x%d" % args.batch, "events_per_step": total_events // args.steps,
+                   "window": "drain each LP's safe window" if DRAIN else "one event per LP per window",
                    "trace_hash_vs_cpu": parity},
     }
     if not args.no_roofline:

@@ -17,7 +17,8 @@ import numpy as np
 from . import _lib
 from ._lib import DesReport, Event, GpqError, PholdConfig, QueueConfig, check
 
-__all__ = ["Queue", "Phold", "des_init", "des_run", "des_finish", "GpqError", "lib", "event_dtype"]
+__all__ = ["Queue", "Phold", "Astar", "AstarBatch", "query_synthetic", "des_init", "des_run", "des_finish",
+           "GpqError", "lib", "event_dtype"]
 
 lib = _lib.load()
 
@@ -117,6 +118,13 @@ class Queue:
         check(lib.gpq_queue_lp_contents(self.handle, lp, _ptr(out), cap, ctypes.byref(n)), "lp_contents")
         return out[:min(n.value, cap)]
 
+    def heap_node(self, node, cap=1 << 20):
+        """Records of one parallel-heap node (GPQ_HQ only), in key order."""
+        out = np.zeros(cap, dtype=event_dtype)
+        n = ctypes.c_uint32()
+        check(lib.gpq_heap_node_contents(self.handle, node, _ptr(out), cap, ctypes.byref(n)), "heap_node")
+        return out[:min(n.value, cap)]
+
     def queue_length(self, lp):
         n = ctypes.c_uint32()
         check(lib.gpq_queue_length(self.handle, lp, ctypes.byref(n)), "queue_length")
@@ -141,12 +149,13 @@ class Queue:
 class Phold:
     """PHOLD model on a queue (src/phold.cu, src/model.h:9-19)."""
 
-    def __init__(self, queue, init_per_lp=16, seed=42, end_time=None, lookahead=None, p_self=0.5, checks=True):
+    def __init__(self, queue, init_per_lp=16, seed=42, end_time=None, lookahead=None, p_self=0.5, checks=True,
+                 drain=False):
         self.queue = queue
         lookahead = queue.cfg.lookahead if lookahead is None else lookahead
         end_time = queue.cfg.horizon if end_time is None else end_time
         self.cfg = PholdConfig(queue.num_lps, init_per_lp, seed, lookahead, end_time,
-                               min(int(round(p_self * 2 ** 32)), 0xFFFFFFFF), 1 if checks else 0)
+                               min(int(round(p_self * 2 ** 32)), 0xFFFFFFFF), 1 if checks else 0, 1 if drain else 0)
         h = ctypes.c_void_p()
         check(lib.gpq_phold_create(queue.handle, ctypes.byref(self.cfg), ctypes.byref(h)), "phold_create")
         self.handle = h
@@ -211,3 +220,120 @@ def des_finish(model):
     q = model.queue
     model.model_finish()
     q.queue_finish()
+
+
+# ---------------------------------------------------------------------------------------
+# A* (src/a_star.cu)
+# ---------------------------------------------------------------------------------------
+class Astar:
+    """One grid query whose open lists are the LPs of a per-LP queue (src/a_star.cu:269-593,
+    model_init :698-842).  The queue needs lookahead = horizon = +inf.  Costs are exact
+    integers: 4-connected unit steps (Manhattan h) or 8-connected 10/14 (octile h)."""
+
+    def __init__(self, queue, width, height, connectivity=4, source=(0, 0), target=None, seed=7, threshold=0.2,
+                 stream_tag=0, query=0, window=0):
+        target = (width - 1, height - 1) if target is None else target
+        thr = threshold if isinstance(threshold, int) else min(int(threshold * 2 ** 32), 0xFFFFFFFF)
+        self.queue = queue
+        self.width, self.height = width, height
+        self.cfg = _lib.AstarConfig(width, height, connectivity, source[0], source[1], target[0], target[1], seed,
+                                    stream_tag, query, thr, min(window, 0xFFFFFFFF))
+        h = ctypes.c_void_p()
+        check(lib.gpq_astar_create(queue.handle, ctypes.byref(self.cfg), ctypes.byref(h)), "astar_create")
+        self.handle = h
+
+    def model_init(self):
+        check(lib.gpq_astar_init(self.handle), "astar model_init")
+
+    def set_grid(self, blocked):
+        b = np.ascontiguousarray(blocked, dtype=np.uint8).reshape(-1)
+        assert b.size == self.width * self.height
+        check(lib.gpq_astar_set_grid(self.handle, _ptr(b)), "astar set_grid")
+
+    def grid(self):
+        out = np.zeros(self.width * self.height, dtype=np.uint8)
+        check(lib.gpq_astar_grid(self.handle, _ptr(out)), "astar grid")
+        return out
+
+    def model_handle_next(self):
+        check(lib.gpq_astar_handle_next(self.handle), "astar model_handle_next")
+
+    def run(self, mode="graph", batch=16):
+        rep = _lib.AstarReport()
+        check(lib.gpq_astar_run(self.handle, _lib.DRIVE_MODES[mode], batch, ctypes.byref(rep)), "astar run")
+        return {k: getattr(rep, k) for k, _ in _lib.AstarReport._fields_}
+
+    def model_finish(self):
+        if self.handle:
+            check(lib.gpq_astar_destroy(self.handle), "astar model_finish")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.model_finish()
+        except Exception:
+            pass
+
+
+def query_synthetic(seed, q):
+    """Query q of the synthetic batched workload (side = round(256 * 2^(3u)), density
+    U[0.1, 0.3], 8-connected, free source and target)."""
+    s = _lib.QuerySpec()
+    check(lib.gpq_query_synthetic(seed, q, ctypes.byref(s)), "query_synthetic")
+    return s
+
+
+class AstarBatch:
+    """Independent grid queries, one workgroup each, pulled from a device work counter by one
+    persistent launch (src/a_star.cu:149-164); mode "plain" searches them one at a time with a
+    host synchronization per expansion step."""
+
+    def __init__(self, specs, threads_per_query=0, heap_capacity=0, stage_capacity=0, max_resident=0):
+        self.specs = (_lib.QuerySpec * len(specs))(*specs)
+        self.n = len(specs)
+        self.tuning = _lib.BatchTuning(threads_per_query, heap_capacity, stage_capacity, max_resident)
+        h = ctypes.c_void_p()
+        check(lib.gpq_astar_batch_create(self.specs, self.n, ctypes.byref(self.tuning), ctypes.byref(h)),
+              "astar_batch_create")
+        self.handle = h
+
+    def run(self, mode="graph"):
+        sec = ctypes.c_double()
+        check(lib.gpq_astar_batch_run(self.handle, _lib.DRIVE_MODES[mode], ctypes.byref(sec)), "astar_batch_run")
+        return sec.value
+
+    def results(self):
+        costs = np.zeros(self.n, dtype=np.int64)
+        exp = np.zeros(self.n, dtype=np.uint64)
+        check(lib.gpq_astar_batch_results(self.handle, _ptr(costs), _ptr(exp)), "astar_batch_results")
+        return costs, exp
+
+    def timing(self):
+        """Per-query (pickup, completion) device times of the last run, microseconds."""
+        st = np.zeros(self.n, dtype=np.float64)
+        fi = np.zeros(self.n, dtype=np.float64)
+        check(lib.gpq_astar_batch_timing(self.handle, _ptr(st), _ptr(fi)), "astar_batch_timing")
+        return st, fi
+
+    def grid(self, q):
+        side = self.specs[q].side
+        out = np.zeros(side * side, dtype=np.uint8)
+        check(lib.gpq_astar_batch_grid(self.handle, q, _ptr(out)), "astar_batch_grid")
+        return out.reshape(side, side)
+
+    def close(self):
+        if self.handle:
+            check(lib.gpq_astar_batch_destroy(self.handle), "astar_batch_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -30,7 +30,7 @@ class QueueConfig(ctypes.Structure):
 
 class PholdConfig(ctypes.Structure):
     _fields_ = [("num_lps", c_u32), ("init_per_lp", c_u32), ("seed", c_u32), ("lookahead", c_f32),
-                ("end_time", c_f32), ("p_self_u32", c_u32), ("checks", c_u32)]
+                ("end_time", c_f32), ("p_self_u32", c_u32), ("checks", c_u32), ("drain", c_u32)]
 
 
 class DesReport(ctypes.Structure):
@@ -46,6 +46,27 @@ class QueueStats(ctypes.Structure):
 class KernelProfile(ctypes.Structure):
     _fields_ = [("steps", c_u64), ("events", c_u64), ("step_kernel_us", ctypes.c_double),
                 ("post_kernel_us", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+class AstarConfig(ctypes.Structure):
+    _fields_ = [("width", c_u32), ("height", c_u32), ("connectivity", c_u32), ("sx", c_u32), ("sy", c_u32),
+                ("tx", c_u32), ("ty", c_u32), ("seed", c_u32), ("stream_tag", c_u32), ("query", c_u32),
+                ("threshold", c_u32), ("window", c_u32)]
+
+
+class AstarReport(ctypes.Structure):
+    _fields_ = [("cost", ctypes.c_int64), ("expanded", c_u64), ("closed", c_u64), ("steps", c_u64),
+                ("seconds", ctypes.c_double), ("error_flags", c_u32)]
+
+
+class QuerySpec(ctypes.Structure):
+    _fields_ = [("side", c_u32), ("threshold", c_u32), ("sx", c_u32), ("sy", c_u32), ("tx", c_u32), ("ty", c_u32),
+                ("connectivity", c_u32), ("seed", c_u32), ("stream_tag", c_u32), ("query", c_u32)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("threads_per_query", c_u32), ("heap_capacity", c_u32), ("stage_capacity", c_u32),
+                ("max_resident", c_u32)]
 
 
 # (name, restype, argtypes) for every entry point of include/gpupq.h
@@ -65,6 +86,7 @@ SIGNATURES = [
     ("gpq_queue_insert_host", ctypes.c_int, [c_void_p, c_void_p, c_u32]),
     ("gpq_queue_peek_host", ctypes.c_int, [c_void_p, ctypes.c_int, c_void_p, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     ("gpq_queue_lp_contents", ctypes.c_int, [c_void_p, c_u32, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
+    ("gpq_heap_node_contents", ctypes.c_int, [c_void_p, c_u32, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     ("gpq_queue_length", ctypes.c_int, [c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     ("gpq_queue_clear_lp", ctypes.c_int, [c_void_p, c_u32]),
     ("gpq_phold_create", ctypes.c_int, [c_void_p, ctypes.POINTER(PholdConfig), ctypes.POINTER(c_void_p)]),
@@ -78,7 +100,25 @@ SIGNATURES = [
     ("gpq_des_run", ctypes.c_int, [c_void_p, ctypes.c_int, c_u32, ctypes.POINTER(DesReport)]),
     ("gpq_queue_stats_get", ctypes.c_int, [c_void_p, ctypes.POINTER(QueueStats)]),
     ("gpq_des_profile", ctypes.c_int, [c_void_p, c_u32, ctypes.POINTER(KernelProfile)]),
+    ("gpq_astar_create", ctypes.c_int, [c_void_p, ctypes.POINTER(AstarConfig), ctypes.POINTER(c_void_p)]),
+    ("gpq_astar_destroy", ctypes.c_int, [c_void_p]),
+    ("gpq_astar_init", ctypes.c_int, [c_void_p]),
+    ("gpq_astar_set_grid", ctypes.c_int, [c_void_p, c_void_p]),
+    ("gpq_astar_handle_next", ctypes.c_int, [c_void_p]),
+    ("gpq_astar_run", ctypes.c_int, [c_void_p, ctypes.c_int, c_u32, ctypes.POINTER(AstarReport)]),
+    ("gpq_astar_grid", ctypes.c_int, [c_void_p, c_void_p]),
+    ("gpq_query_synthetic", ctypes.c_int, [c_u32, c_u32, ctypes.POINTER(QuerySpec)]),
+    ("gpq_astar_batch_create", ctypes.c_int, [c_void_p, c_u32, ctypes.POINTER(BatchTuning), ctypes.POINTER(c_void_p)]),
+    ("gpq_astar_batch_destroy", ctypes.c_int, [c_void_p]),
+    ("gpq_astar_batch_run", ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    ("gpq_astar_batch_results", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    ("gpq_astar_batch_timing", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    ("gpq_astar_batch_grid", ctypes.c_int, [c_void_p, c_u32, c_void_p]),
     ("gpq_selftest_exp", ctypes.c_int, [c_u32, c_u32, c_void_p]),
+    ("gpq_selftest_sort", ctypes.c_int, [c_void_p, c_u32, c_void_p]),
+    ("gpq_selftest_merge", ctypes.c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p]),
+    ("gpq_selftest_select", ctypes.c_int, [c_void_p, c_u32, c_u32, c_f32, c_void_p, c_void_p, ctypes.POINTER(c_u32),
+                                           c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     ("gpq_selftest_philox", ctypes.c_int, [c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_void_p]),
 ]
 

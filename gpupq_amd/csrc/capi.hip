@@ -112,6 +112,8 @@ QueueBase* make_queue(const gpq_queue_config& cfg) {
     case GPQ_LAQ: return new LaqQueue(cfg);
     case GPQ_LHQ: return new LhqQueue(cfg);
     case GPQ_LSQ: return new LsqQueue(cfg);
+    case GPQ_GAQ: return new GaqQueue(cfg);
+    case GPQ_HQ: return new HqQueue(cfg);
     default: return nullptr;
   }
 }
@@ -261,6 +263,13 @@ gpq_status gpq_queue_lp_contents(gpq_queue* q, uint32_t lp, gpq_event* out, uint
   GPQ_Q(q);
   if (!n || (cap && !out)) { set_error("null out"); return GPQ_ERR_INVALID; }
   return q->impl->lp_contents(lp, reinterpret_cast<gpq::Event*>(out), cap, n);
+}
+
+gpq_status gpq_heap_node_contents(gpq_queue* q, uint32_t node, gpq_event* out, uint32_t cap, uint32_t* n) {
+  GPQ_Q(q);
+  if (q->impl->cfg.kind != GPQ_HQ) { set_error("not a heap queue"); return GPQ_ERR_UNSUPPORTED; }
+  if (!n || (cap && !out)) { set_error("null out"); return GPQ_ERR_INVALID; }
+  return static_cast<gpq::HqQueue*>(q->impl)->node_contents(node, reinterpret_cast<gpq::Event*>(out), cap, n);
 }
 
 gpq_status gpq_queue_length(gpq_queue* q, uint32_t lp, uint32_t* n) {

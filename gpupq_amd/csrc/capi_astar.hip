@@ -1,0 +1,108 @@
+// capi_astar.hip -- the extern "C" boundary of the A* models (include/gpupq.h).
+#include <string>
+
+#include "astar.h"
+
+using gpq::set_error;
+
+#define GPQ_NONNULL(p)                                      \
+  do {                                                      \
+    if (!(p)) { set_error("null argument"); return GPQ_ERR_INVALID; } \
+  } while (0)
+
+extern "C" {
+
+gpq_status gpq_astar_create(gpq_queue* open_lists, const gpq_astar_config* cfg, gpq_astar** out) {
+  GPQ_NONNULL(open_lists);
+  GPQ_NONNULL(open_lists->impl);
+  GPQ_NONNULL(cfg);
+  GPQ_NONNULL(out);
+  gpq::AstarModel* m = nullptr;
+  GPQ_TRY(gpq::AstarModel::create(open_lists->impl, *cfg, &m));
+  *out = new gpq_astar{m};
+  return GPQ_OK;
+}
+
+gpq_status gpq_astar_destroy(gpq_astar* a) {
+  if (!a) return GPQ_OK;
+  if (a->impl && a->impl->q) (void)hipStreamSynchronize(a->impl->q->stream);
+  delete a->impl;
+  delete a;
+  return GPQ_OK;
+}
+
+gpq_status gpq_astar_init(gpq_astar* a) {
+  GPQ_NONNULL(a);
+  return a->impl->init();
+}
+
+gpq_status gpq_astar_set_grid(gpq_astar* a, const uint8_t* blocked) {
+  GPQ_NONNULL(a);
+  GPQ_NONNULL(blocked);
+  return a->impl->set_grid(blocked);
+}
+
+gpq_status gpq_astar_handle_next(gpq_astar* a) {
+  GPQ_NONNULL(a);
+  return a->impl->step();
+}
+
+gpq_status gpq_astar_run(gpq_astar* a, gpq_drive_mode mode, uint32_t batch, gpq_astar_report* rep) {
+  GPQ_NONNULL(a);
+  if (mode != GPQ_DRIVE_PLAIN && mode != GPQ_DRIVE_GRAPH) { set_error("bad drive mode"); return GPQ_ERR_INVALID; }
+  return a->impl->run(mode, batch, rep);
+}
+
+gpq_status gpq_astar_grid(gpq_astar* a, uint8_t* blocked) {
+  GPQ_NONNULL(a);
+  GPQ_NONNULL(blocked);
+  return a->impl->grid_bytes(blocked);
+}
+
+gpq_status gpq_query_synthetic(uint32_t seed, uint32_t q, gpq_query_spec* out) {
+  GPQ_NONNULL(out);
+  return gpq::query_synthetic(seed, q, out);
+}
+
+gpq_status gpq_astar_batch_create(const gpq_query_spec* specs, uint32_t count, const gpq_batch_tuning* tuning,
+                                  gpq_astar_batch** out) {
+  GPQ_NONNULL(specs);
+  GPQ_NONNULL(out);
+  gpq_batch_tuning t{};
+  if (tuning) t = *tuning;
+  gpq::AstarBatch* b = nullptr;
+  GPQ_TRY(gpq::AstarBatch::create(specs, count, t, &b));
+  *out = new gpq_astar_batch{b};
+  return GPQ_OK;
+}
+
+gpq_status gpq_astar_batch_destroy(gpq_astar_batch* b) {
+  if (!b) return GPQ_OK;
+  delete b->impl;
+  delete b;
+  return GPQ_OK;
+}
+
+gpq_status gpq_astar_batch_run(gpq_astar_batch* b, gpq_drive_mode mode, double* seconds) {
+  GPQ_NONNULL(b);
+  if (mode != GPQ_DRIVE_PLAIN && mode != GPQ_DRIVE_GRAPH) { set_error("bad drive mode"); return GPQ_ERR_INVALID; }
+  return b->impl->run(mode, seconds);
+}
+
+gpq_status gpq_astar_batch_results(gpq_astar_batch* b, int64_t* costs, uint64_t* expanded) {
+  GPQ_NONNULL(b);
+  return b->impl->results(costs, expanded);
+}
+
+gpq_status gpq_astar_batch_timing(gpq_astar_batch* b, double* start_us, double* finish_us) {
+  GPQ_NONNULL(b);
+  return b->impl->timing(start_us, finish_us);
+}
+
+gpq_status gpq_astar_batch_grid(gpq_astar_batch* b, uint32_t q, uint8_t* blocked) {
+  GPQ_NONNULL(b);
+  GPQ_NONNULL(blocked);
+  return b->impl->grid_bytes(q, blocked);
+}
+
+}  // extern "C"

@@ -46,14 +46,14 @@ __device__ __forceinline__ uint32_t f_to_bits(float f) { return __float_as_uint(
 // Philox4x32-10 (Salmon et al., SC'11): 10 rounds of two 32x32->64 products,
 // Weyl key bumps between rounds.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
+__host__ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
 #pragma unroll
   for (int round = 0; round < 10; ++round) {
     if (round) { k.x += 0x9E3779B9u; k.y += 0xBB67AE85u; }
-    const uint32_t lo_a = 0xD2511F53u * c.x;
-    const uint32_t hi_a = __umulhi(0xD2511F53u, c.x);
-    const uint32_t lo_b = 0xCD9E8D57u * c.z;
-    const uint32_t hi_b = __umulhi(0xCD9E8D57u, c.z);
+    const uint64_t pa = (uint64_t)0xD2511F53u * c.x;
+    const uint64_t pb = (uint64_t)0xCD9E8D57u * c.z;
+    const uint32_t lo_a = (uint32_t)pa, hi_a = (uint32_t)(pa >> 32);
+    const uint32_t lo_b = (uint32_t)pb, hi_b = (uint32_t)(pb >> 32);
     c = make_uint4(hi_b ^ c.y ^ k.x, lo_b, hi_a ^ c.w ^ k.y, lo_a);
   }
   return c;

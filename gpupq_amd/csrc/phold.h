@@ -12,6 +12,8 @@ gpq_status with_view(QueueBase* q, F&& f) {
     case GPQ_LAQ: return f(static_cast<LaqQueue*>(q)->view());
     case GPQ_LHQ: return f(static_cast<LhqQueue*>(q)->view());
     case GPQ_LSQ: return f(static_cast<LsqQueue*>(q)->view());
+    case GPQ_GAQ: return f(static_cast<GaqQueue*>(q)->view());
+    case GPQ_HQ: return f(static_cast<HqQueue*>(q)->view());
     default:
       set_error("queue kind not provided");
       return GPQ_ERR_UNSUPPORTED;

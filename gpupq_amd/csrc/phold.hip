@@ -28,33 +28,47 @@ struct PholdDev {
   uint32_t init_per_lp;
   uint32_t seed;
   uint32_t p_self;
+  uint32_t drain;
   float lookahead;
 };
 
 // handle_next_event (src/phold.cu:81-117): every peek slot whose item is inside the
 // window commits it, counts it, extends the LP trace hash and schedules one new event.
+// In drain mode the slot's owner keeps committing its LP's next events while they are
+// inside the window.
+template <class Q>
+__device__ __forceinline__ bool phold_commit_one(const Q& q, const PholdDev& m, uint32_t slot) {
+  Event* ev;
+  const int idx = q.peek(&ev, (int)slot);
+  if (idx < 0) return false;
+  const Event e = load_event(ev);
+  const uint32_t lp = e.dst;
+  const uint32_t n = m.count[lp];
+  m.count[lp] = n + 1u;
+  m.hash[lp] = trace_step(m.hash[lp], e.ts, e.src, e.seq);
+  if (m.last_ts) {                                   // src/des.cu:52-64
+    if (e.ts < m.last_ts[lp]) atomicOr(&m.ws->error, kErrBadState);
+    m.last_ts[lp] = e.ts;
+  }
+  const uint4 r = philox4x32_10(make_uint4(n, lp, 0u, 0u), make_uint2(m.seed, 0u));
+  const float ex = exp_from_u32(r.x);
+  const uint32_t dst = r.y < m.p_self ? lp : (uint32_t)(((uint64_t)r.z * m.num_lps) >> 32);
+  const float t_new = (bits_to_f(e.ts) + m.lookahead) + ex;
+  q.set_done(idx);
+  q.insert(Event{f_to_bits(t_new), dst, lp, m.init_per_lp + n});
+  return true;
+}
+
 template <class Q>
 __global__ __launch_bounds__(kStepBlock) void phold_step_kernel(Q q, PholdDev m) {
   const uint32_t slots = q.slots();
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += gridDim.x * blockDim.x) {
-    Event* ev;
-    const int idx = q.peek(&ev, (int)s);
-    if (idx < 0) continue;
-    const Event e = load_event(ev);
-    const uint32_t lp = e.dst;
-    const uint32_t n = m.count[lp];
-    m.count[lp] = n + 1u;
-    m.hash[lp] = trace_step(m.hash[lp], e.ts, e.src, e.seq);
-    if (m.last_ts) {                                   // src/des.cu:52-64
-      if (e.ts < m.last_ts[lp]) atomicOr(&m.ws->error, kErrBadState);
-      m.last_ts[lp] = e.ts;
+    if (m.drain) {
+      while (phold_commit_one(q, m, s)) {
+      }
+    } else {
+      (void)phold_commit_one(q, m, s);
     }
-    const uint4 r = philox4x32_10(make_uint4(n, lp, 0u, 0u), make_uint2(m.seed, 0u));
-    const float ex = exp_from_u32(r.x);
-    const uint32_t dst = r.y < m.p_self ? lp : (uint32_t)(((uint64_t)r.z * m.num_lps) >> 32);
-    const float t_new = (bits_to_f(e.ts) + m.lookahead) + ex;
-    q.set_done(idx);
-    q.insert(Event{f_to_bits(t_new), dst, lp, m.init_per_lp + n});
   }
 }
 
@@ -114,7 +128,7 @@ __global__ void phold_sum_kernel(const uint32_t* count, uint32_t n, unsigned lon
 
 PholdDev dev_of(const PholdModel* m) {
   return PholdDev{m->count, m->hash, m->last_ts, m->q->ws, m->cfg.num_lps, m->cfg.init_per_lp, m->cfg.seed,
-                  m->cfg.p_self_u32, m->cfg.lookahead};
+                  m->cfg.p_self_u32, m->cfg.drain, m->cfg.lookahead};
 }
 
 uint32_t grid_for(uint32_t items, uint32_t block) {
@@ -159,6 +173,8 @@ gpq_status PholdModel::create(QueueBase* queue, const gpq_phold_config& c, Phold
   GPQ_CHECK_ARG(c.lookahead > 0.0f, "phold: lookahead must be > 0 (conservative window)");
   GPQ_CHECK_ARG(c.lookahead == queue->cfg.lookahead, "phold lookahead must equal the queue's lookahead");
   GPQ_CHECK_ARG(c.end_time == queue->cfg.horizon, "phold end_time must equal the queue's horizon");
+  GPQ_CHECK_ARG(!c.drain || queue->cfg.kind == GPQ_LAQ || queue->cfg.kind == GPQ_LHQ || queue->cfg.kind == GPQ_LSQ,
+                "phold drain mode needs a per-LP queue (peek by LP)");
   PholdModel* m = new PholdModel();
   m->cfg = c;
   m->q = queue;

@@ -1,0 +1,65 @@
+// queue_hq.cuh -- device API of the global parallel heap (HQ): a binary heap whose nodes
+// hold up to r sorted records, every record of a node <= every record of its children.
+// Reference: src/heap_queue.cu (insert :1809-1826, peek :1860-1896, set_done :1898-1904,
+// post :1609-1719, partition :478-547, delete-update :631-784, insert-update :817-960,
+// copy_last :975-1094).
+//
+// HBM layout: nodes[v * r + i] (node v, position i), counts[v]; the step's inserts are
+// appended to enq; queue_post leaves the next step's safe events in deq (peek slot = position).
+#pragma once
+
+#include "gpq_device.h"
+#include "sortprim.h"
+
+namespace gpq {
+
+constexpr int kHqMaxLevels = 20;
+constexpr int kHqMaxChunks = 8;
+
+struct HqState {
+  uint32_t last;           // highest node holding records
+  uint32_t m;              // inserts appended this step
+  uint32_t mcount;         // records in the merged buffer
+  uint32_t prefix, deq_n, rest_n;
+  uint32_t bound_valid;
+  uint32_t carry_n;        // records that travel down as insert chunks
+  uint32_t select;         // 1: queue_post selects the next step's events
+  uint32_t pad[3];
+  Event bound;             // the root's largest record while nodes exist below the root
+  const Event* mptr;       // = merged buffer (pointer slot for the selection)
+  // insert-update chunks: target node, size, current data pointer, next data pointer
+  uint32_t chunk_target[kHqMaxChunks];
+  uint32_t chunk_n[kHqMaxChunks];
+  const Event* chunk_ptr[kHqMaxChunks];
+  Event* chunk_alt[kHqMaxChunks];
+};
+
+struct HqView {
+  Event* enq;
+  Event* deq;
+  HqState* st;
+  WindowState* ws;
+  uint32_t enq_cap;
+  uint32_t num_lps;
+
+  __device__ __forceinline__ uint32_t slots() const { return num_lps; }
+  __device__ __forceinline__ bool insert(const Event& e) const {
+    const uint32_t pos = wave_append_slot(&st->m);
+    if (pos >= enq_cap) {
+      atomicOr(&ws->error, kErrStageOverflow);
+      return false;
+    }
+    store_event(enq + pos, e);
+    return true;
+  }
+  // heap_queue_peek(item, pos): dequeue-buffer slot pos (src/heap_queue.cu:1860-1896).
+  __device__ __forceinline__ int peek(Event** item, int pos) const {
+    if ((uint32_t)pos >= st->deq_n) return -1;
+    *item = deq + pos;
+    return pos;
+  }
+  __device__ __forceinline__ void set_done(int) const {}   // src/heap_queue.cu:1898-1904 (PHOLD)
+  __device__ __forceinline__ void insert_or_update(const Event& e, int) const { insert(e); }
+};
+
+}  // namespace gpq

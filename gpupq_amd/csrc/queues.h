@@ -6,6 +6,8 @@
 #include "queue_laq.cuh"
 #include "queue_lhq.cuh"
 #include "queue_lsq.cuh"
+#include "queue_gaq.cuh"
+#include "queue_hq.cuh"
 
 namespace gpq {
 
@@ -47,5 +49,63 @@ class LaqQueue final : public QueueBase {
 
 GPQ_LOCAL_QUEUE_CLASS(LhqQueue, LhqView)
 GPQ_LOCAL_QUEUE_CLASS(LsqQueue, LsqView)
+
+class GaqQueue final : public QueueBase {
+ public:
+  explicit GaqQueue(const gpq_queue_config& c) : QueueBase(c) {}
+  gpq_status init() override;
+  gpq_status pre() override;
+  gpq_status post() override;
+  gpq_status lp_contents(uint32_t lp, Event* out, uint32_t cap, uint32_t* n) override;
+  gpq_status pending(uint64_t* n) override;
+  gpq_status length(uint32_t lp, uint32_t* n) override;
+  gpq_status clear_lp(uint32_t lp) override;
+  GaqView view() const { return v_; }
+
+ private:
+  gpq_status snapshot(std::vector<Event>* out);
+  GaqView v_{};
+  Event* fel_[2] = {nullptr, nullptr};
+  Event* rest_ = nullptr;
+  Event* sort_[2] = {nullptr, nullptr};
+  MergeTask* task_ = nullptr;
+  uint32_t* first_ = nullptr;
+  uint32_t* scratch_ = nullptr;
+  uint32_t cap_ = 0;
+};
+
+class HqQueue final : public QueueBase {
+ public:
+  explicit HqQueue(const gpq_queue_config& c) : QueueBase(c) {}
+  gpq_status init() override;
+  gpq_status post() override;
+  gpq_status post_init() override;
+  gpq_status lp_contents(uint32_t lp, Event* out, uint32_t cap, uint32_t* n) override;
+  gpq_status pending(uint64_t* n) override;
+  gpq_status length(uint32_t lp, uint32_t* n) override;
+  gpq_status clear_lp(uint32_t lp) override;
+  gpq_status node_contents(uint32_t v, Event* out, uint32_t cap, uint32_t* n);
+  HqView view() const { return v_; }
+
+ private:
+  gpq_status step(bool select);
+  gpq_status snapshot(std::vector<Event>* out, std::vector<uint32_t>* counts);
+  HqView v_{};
+  uint32_t r_ = 0, nodes_ = 0, levels_ = 0, slots_ = 1, depth_ = 0;
+  Event* nodes_buf_ = nullptr;
+  uint32_t* counts_ = nullptr;
+  uint8_t* dirty_ = nullptr;
+  Event* sort_[2] = {nullptr, nullptr};
+  Event* merged_ = nullptr;
+  Event* rest_ = nullptr;
+  Event* carry_ = nullptr;
+  Event* chunk_buf_ = nullptr;
+  Event* tmp2_ = nullptr;
+  Event* tmp1_ = nullptr;
+  uint32_t* first_ = nullptr;
+  uint32_t* scratch_ = nullptr;
+  MergeTask* tasks_ = nullptr;
+  uint64_t* sum_ = nullptr;
+};
 
 }  // namespace gpq

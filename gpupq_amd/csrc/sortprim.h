@@ -1,0 +1,71 @@
+// sortprim.h -- record sort / merge / select primitives for the global queues (GAQ, HQ),
+// replacing the reference's moderngpu mergesort (lib/moderngpu/src/moderngpu/
+// kernel_mergesort.hxx:27-148) and cub DeviceReduce::Min.  All sizes that depend on the
+// simulation live in device memory, so the launch sequence is fixed and graph-capturable;
+// grids are sized from capacities and surplus blocks exit.
+#pragma once
+
+#include "gpq_host.h"
+
+namespace gpq {
+
+constexpr uint32_t kMergeTile = 2048;   // outputs per merge block (256 threads x 8)
+constexpr uint32_t kSortTile = 2048;    // records per tile-sort block
+
+// Stable merge of A = a1[0:n_a1) ++ a2[0:n_a2) (both parts sorted, a1 before a2) with
+// sorted B (ties take A first).  Output rank o goes to out[s][o - start_s], where segment
+// s covers ranks [end[s-1], end[s]) (end[-1] = 0, unused segments have end = total).
+struct MergeTask {
+  const Event* a1;
+  const Event* a2;
+  const Event* b;
+  Event* out[3];
+  uint32_t n_a1;
+  uint32_t n_a2;
+  uint32_t n_b;
+  uint32_t end[3];
+};
+
+// Sorts *d_n (<= cap) records of `in` (not modified) into buf0/buf1; returns the buffer
+// holding the sorted run.  Passes: one tile sort, then ceil(log2(cap / kSortTile)) merge passes.
+gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_t* d_n, uint32_t cap,
+                        hipStream_t stream, Event** sorted);
+
+// Runs `ntasks` merge tasks (device array) in one launch; max_total bounds every task's
+// n_a1 + n_a2 + n_b.
+gpq_status merge_tasks(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_total, hipStream_t stream);
+
+// One safe event per LP over the sorted array s[0:*d_n), s = *d_s: the prefix of records with
+// ts < *d_wend and key <= *d_bound (when *d_bound_valid) is scanned, the first record of
+// each LP in it is moved to `deq` (in order, count -> *d_deq_n) and the others to `rest`
+// (in order, count -> *d_rest_n); *d_prefix receives the prefix length.  `first` (one
+// u32 per LP, all 0xFFFFFFFF between calls) and `scratch` (>= 2 * ceil(cap / 1024) + 2
+// u32) are workspace.  Reference: src/global_array_queue.cu:147-179 (serial scan) and
+// src/heap_queue.cu:478-547 (atomicMin per LP).
+struct SelectBufs {
+  uint32_t* first;
+  uint32_t* scratch;
+  uint32_t* d_prefix;
+  uint32_t* d_deq_n;
+  uint32_t* d_rest_n;
+};
+gpq_status select_safe(const Event* const* d_s, const uint32_t* d_n, uint32_t cap, const uint32_t* d_wend,
+                       const Event* d_bound, const uint32_t* d_bound_valid, const SelectBufs& b, Event* deq,
+                       Event* rest, hipStream_t stream);
+
+// Copies records: dst[i] = src[i] for i < *d_n (bounded by cap).
+gpq_status copy_records(const Event* src, Event* dst, const uint32_t* d_n, uint32_t cap, hipStream_t stream);
+
+// Reserves one slot of a shared append counter per calling lane with one atomic per wave
+// (every active lane of the wave that reaches the call takes part).
+__device__ __forceinline__ uint32_t wave_append_slot(uint32_t* counter) {
+  const uint64_t mask = __ballot(1);
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = (uint32_t)__shfl((int)base, leader, kWave);
+  return base + rank;
+}
+
+}  // namespace gpq

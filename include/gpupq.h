@@ -120,6 +120,9 @@ gpq_status gpq_queue_peek_host(gpq_queue* q, int set_done, gpq_event* items, int
                                uint32_t max, uint32_t* found);
 /* Contents owned by one LP in key order (local queues; HQ/GAQ: all pending items of that LP). */
 gpq_status gpq_queue_lp_contents(gpq_queue* q, uint32_t lp, gpq_event* out, uint32_t cap, uint32_t* n);
+/* Records of one node of the parallel heap (GPQ_HQ), in key order; node 0 is the root
+ * (the reference's tests read heap_queue_data.fel per node, src/test_hq.cu:580-917). */
+gpq_status gpq_heap_node_contents(gpq_queue* q, uint32_t node, gpq_event* out, uint32_t cap, uint32_t* n);
 /* queue_length(lp) / queue_is_empty(lp) / queue_clear(lp) (src/queue.cuh:58-66). */
 gpq_status gpq_queue_length(gpq_queue* q, uint32_t lp, uint32_t* n);
 gpq_status gpq_queue_clear_lp(gpq_queue* q, uint32_t lp);
@@ -133,6 +136,10 @@ typedef struct {
   float end_time;          /* commit exactly the events with ts < end_time */
   uint32_t p_self_u32;     /* P(dst == self) * 2^32; 0x80000000 = 0.5 */
   uint32_t checks;         /* 1: per-LP monotone-timestamp check (src/des.cu:52-64) */
+  uint32_t drain;          /* 0: one safe event per LP per step, as src/phold.cu:81-117 does;
+                              1: each LP commits all of its events below the window end in key
+                              order in one step (per-LP queues only).  Events created in a step
+                              lie at or beyond the window end, so the committed trace is the same. */
 } gpq_phold_config;
 
 gpq_status gpq_phold_create(gpq_queue* q, const gpq_phold_config* cfg, gpq_phold** out);
@@ -181,12 +188,103 @@ typedef struct {
 } gpq_kernel_profile;
 gpq_status gpq_des_profile(gpq_phold* m, uint32_t batch, gpq_kernel_profile* out);
 
+/* ---- A* on grids (src/a_star.cu, src/model.h:9-19) ----------------------------------------
+ * Grid cell c = y * width + x is an obstacle iff word (c & 3) of Philox4x32-10 of counter
+ * {c >> 2, query, stream_tag, 2} under key {seed, 0} is below `threshold` (density * 2^32);
+ * source and target are forced free.  connectivity 4: unit costs, Manhattan heuristic;
+ * 8: costs 10 / 14, octile heuristic 10 * max + 4 * min (diagonal moves may cut corners, as
+ * src/a_star.cu:393-459 allows).  Costs and f are exact uint32. */
+typedef struct {
+  uint32_t width, height;
+  uint32_t connectivity;     /* 4 or 8 */
+  uint32_t sx, sy, tx, ty;
+  uint32_t seed, stream_tag, query;
+  uint32_t threshold;        /* obstacle density * 2^32 */
+  uint32_t window;           /* a list pops its best entry only if f <= (smallest f of all lists) + window;
+                                0xFFFFFFFF: every non-empty list pops each step */
+} gpq_astar_config;
+
+typedef struct {
+  int64_t cost;              /* optimal path cost, -1 if the target is unreachable */
+  uint64_t expanded;         /* expansions (a cell may be expanded again after a cheaper path) */
+  uint64_t closed;           /* distinct cells in the closed set */
+  uint64_t steps;            /* model steps launched */
+  double seconds;            /* wall clock of the step loop */
+  uint32_t error_flags;
+} gpq_astar_report;
+
+typedef struct gpq_astar gpq_astar;
+
+/* One query searched by many open lists at once: the LPs of `open_lists` (a per-LP queue:
+ * GPQ_LAQ, GPQ_LHQ or GPQ_LSQ; its lookahead and horizon must be +inf) are the open lists.
+ * model_init (src/a_star.cu:698-842) generates the grid and pushes the source. */
+gpq_status gpq_astar_create(gpq_queue* open_lists, const gpq_astar_config* cfg, gpq_astar** out);
+gpq_status gpq_astar_destroy(gpq_astar* a);
+gpq_status gpq_astar_init(gpq_astar* a);
+/* Replaces the generated obstacle map (w*h bytes, nonzero = obstacle) and restarts the search. */
+gpq_status gpq_astar_set_grid(gpq_astar* a, const uint8_t* blocked);
+/* model_handle_next (src/a_star.cu:854-872): every open list pops its best entry, expands it
+ * and sends improved neighbours to their lists; queue_post folds them in. */
+gpq_status gpq_astar_handle_next(gpq_astar* a);
+/* des_run for A*: steps until the best goal cost is <= the smallest f of any open list. */
+gpq_status gpq_astar_run(gpq_astar* a, gpq_drive_mode mode, uint32_t batch, gpq_astar_report* rep);
+/* The grid the search ran on (w*h bytes). */
+gpq_status gpq_astar_grid(gpq_astar* a, uint8_t* blocked);
+
+/* Batched A*: many independent queries, each searched by one workgroup with per-thread
+ * local heaps; one persistent launch pulls queries from a device work counter (the reference's
+ * get_new_agent, src/a_star.cu:149-164).  Query q of a batch uses the shape and endpoints of
+ * the synthetic distribution (side = round(256 * 2^(3u)), density U[0.1, 0.3], 8-connected)
+ * unless explicit specs are given. */
+typedef struct {
+  uint32_t side;             /* square grid side */
+  uint32_t threshold;        /* obstacle density * 2^32 */
+  uint32_t sx, sy, tx, ty;
+  uint32_t connectivity;     /* 4 or 8 */
+  uint32_t seed, stream_tag, query;   /* obstacle generator (see gpq_astar_config) */
+} gpq_query_spec;
+
+typedef struct {
+  uint32_t threads_per_query;   /* workgroup size = open lists per query (64..1024, multiple of 64) */
+  uint32_t heap_capacity;       /* entries per open list */
+  uint32_t stage_capacity;      /* inserts one open list may receive per iteration */
+  uint32_t max_resident;        /* queries searched at once (0 = fill the device) */
+} gpq_batch_tuning;
+
+typedef struct gpq_astar_batch gpq_astar_batch;
+
+/* Synthetic query q of the batched workload (seed = master seed). */
+gpq_status gpq_query_synthetic(uint32_t seed, uint32_t q, gpq_query_spec* out);
+gpq_status gpq_astar_batch_create(const gpq_query_spec* specs, uint32_t count, const gpq_batch_tuning* tuning,
+                                  gpq_astar_batch** out);
+gpq_status gpq_astar_batch_destroy(gpq_astar_batch* b);
+/* Runs every query: GPQ_DRIVE_GRAPH = one persistent launch (captured once in a hipGraph);
+ * GPQ_DRIVE_PLAIN = the one-query-per-launch path: each query alone, one launch per expansion
+ * step with a host synchronization per step (the reference's des_run pattern). */
+gpq_status gpq_astar_batch_run(gpq_astar_batch* b, gpq_drive_mode mode, double* seconds);
+/* Per-query optimal cost (-1 unreachable) and expansions. */
+gpq_status gpq_astar_batch_results(gpq_astar_batch* b, int64_t* costs, uint64_t* expanded);
+/* Device wall-clock times of the last run, per query, in microseconds from the earliest
+ * pickup: when a workgroup picked the query up and when its search completed. */
+gpq_status gpq_astar_batch_timing(gpq_astar_batch* b, double* start_us, double* finish_us);
+/* Obstacle map of query q (side*side bytes). */
+gpq_status gpq_astar_batch_grid(gpq_astar_batch* b, uint32_t q, uint8_t* blocked);
+
 /* ---- self-test hooks: the device random-number path evaluated in bulk ------------------- */
 /* out[i] = device Exp(1) sample of draw ((first + i) << 8), i < count (count <= 2^24). */
 gpq_status gpq_selftest_exp(uint32_t first, uint32_t count, float* out);
 /* out[4*i .. 4*i+3] = device Philox4x32-10 of counter {ctr0 + i, ctr1, ctr2, ctr3}, key {key0, key1}. */
 gpq_status gpq_selftest_philox(uint32_t ctr0, uint32_t ctr1, uint32_t ctr2, uint32_t ctr3, uint32_t key0,
                                uint32_t key1, uint32_t count, uint32_t* out);
+/* Record primitives of the global queues on host arrays: sort by (ts, src, seq); stable
+ * merge of two sorted arrays; one-safe-event-per-LP selection over a sorted array (prefix
+ * ts < wend and key <= *bound when bound != NULL; first record per LP -> deq, others of the
+ * prefix -> rest, in order). */
+gpq_status gpq_selftest_sort(const gpq_event* in, uint32_t n, gpq_event* out);
+gpq_status gpq_selftest_merge(const gpq_event* a, uint32_t na, const gpq_event* b, uint32_t nb, gpq_event* out);
+gpq_status gpq_selftest_select(const gpq_event* sorted, uint32_t n, uint32_t num_lps, float wend,
+                               const gpq_event* bound, gpq_event* deq, uint32_t* ndeq, gpq_event* rest,
+                               uint32_t* nrest, uint32_t* prefix);
 
 #ifdef __cplusplus
 }

@@ -49,4 +49,4 @@ def test_struct_layouts_match_header():
     from gpupq_amd import _lib
     assert ctypes.sizeof(_lib.Event) == 16
     assert ctypes.sizeof(_lib.QueueConfig) == 48
-    assert ctypes.sizeof(_lib.PholdConfig) == 28
+    assert ctypes.sizeof(_lib.PholdConfig) == 32

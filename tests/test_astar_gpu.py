@@ -1,0 +1,181 @@
+"""A* on the GPU against the oracle (exact integer costs).
+
+Single query (open lists = LPs of a per-LP queue) and batched queries (one workgroup per
+query).  The oracle's A*, Dijkstra and BFS are themselves pinned in test_oracle.py; here
+the checks are: the device obstacle generator equals the oracle's bit for bit; costs equal
+the oracle's on random grids; closed forms on obstacle-free grids; source == target gives
+0; an enclosed target gives -1; plain and persistent drives give the same costs."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+INF = float("inf")
+
+
+def open_lists(gpq, kind, lists=64, cap=64, stage=32):
+    return gpq.Queue(kind, num_lps=lists, capacity_per_lp=cap, stage_per_lp=stage, lookahead=INF, horizon=INF)
+
+
+def thr(p):
+    return min(int(p * 2 ** 32), 0xFFFFFFFF)
+
+
+def oracle_grid(oracle, w, h, seed, tag, q, t, forced=()):
+    g = oracle.grid(w, h, seed, tag, q, t)
+    for (x, y) in forced:
+        g[y * w + x] = 0
+    return g
+
+
+@pytest.mark.parametrize("kind", ["laq", "lhq", "lsq"])
+@pytest.mark.parametrize("conn", [4, 8])
+def test_single_query_matches_oracle(gpq, oracle, kind, conn):
+    w, h = 97, 61
+    for seed in (1, 2, 3):
+        s, t = (3, 5), (90, 57)
+        with open_lists(gpq, kind) as q:
+            a = gpq.Astar(q, w, h, connectivity=conn, source=s, target=t, seed=seed, threshold=0.25)
+            a.model_init()
+            grid = a.grid()
+            ref_grid = oracle_grid(oracle, w, h, seed, 0, 0, thr(0.25), forced=(s, t))
+            assert np.array_equal(grid, ref_grid)
+            rep = a.run("graph", 8)
+            cost, _ = oracle.astar(ref_grid, w, h, conn, s, t)
+            assert rep["cost"] == cost == oracle.dijkstra(ref_grid, w, h, conn, s, t)
+            assert rep["error_flags"] == 0 and rep["expanded"] >= 1
+            a.model_finish()
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_single_query_closed_forms(gpq, conn):
+    w, h = 200, 130
+    with open_lists(gpq, "lhq", lists=128) as q:
+        for (s, t) in [((0, 0), (199, 129)), ((150, 3), (7, 100)), ((5, 5), (5, 5))]:
+            a = gpq.Astar(q, w, h, connectivity=conn, source=s, target=t, threshold=0)
+            a.model_init()
+            rep = a.run("plain")
+            dx, dy = abs(s[0] - t[0]), abs(s[1] - t[1])
+            expect = dx + dy if conn == 4 else 10 * max(dx, dy) + 4 * min(dx, dy)
+            assert rep["cost"] == expect
+            a.model_finish()
+
+
+def test_single_query_enclosed_target(gpq, oracle):
+    w, h = 40, 30
+    grid = np.zeros((h, w), dtype=np.uint8)
+    grid[9:14, 19] = grid[9:14, 23] = 1
+    grid[9, 19:24] = grid[13, 19:24] = 1        # a closed box around (21, 11)
+    with open_lists(gpq, "laq") as q:
+        a = gpq.Astar(q, w, h, connectivity=8, source=(0, 0), target=(21, 11), threshold=0)
+        a.model_init()
+        a.set_grid(grid)
+        assert a.run("graph", 4)["cost"] == -1 == oracle.dijkstra(grid.reshape(-1), w, h, 8, (0, 0), (21, 11))
+        a.model_finish()
+
+
+def test_single_query_plain_equals_graph(gpq, oracle):
+    w, h = 128, 128
+    with open_lists(gpq, "laq", lists=256) as q:
+        a = gpq.Astar(q, w, h, connectivity=4, seed=9, threshold=0.2)
+        a.model_init()
+        r1 = a.run("plain")
+        a.model_init()
+        r2 = a.run("graph", 16)
+        assert r1["cost"] == r2["cost"]
+        ref = oracle_grid(oracle, w, h, 9, 0, 0, thr(0.2), forced=((0, 0), (w - 1, h - 1)))
+        assert r1["cost"] == oracle.bfs4(ref, w, h, (0, 0), (w - 1, h - 1))
+        a.model_finish()
+
+
+def test_single_query_4096_grid(gpq, oracle):
+    """The 4096 x 4096, 4-connected, Bernoulli(0.2) seed 7 query, (0,0) -> (4095,4095)."""
+    w = h = 4096
+    with gpq.Queue("lhq", num_lps=1 << 16, capacity_per_lp=64, stage_per_lp=16, lookahead=INF, horizon=INF) as q:
+        a = gpq.Astar(q, w, h, connectivity=4, seed=7, threshold=0.2)
+        a.model_init()
+        rep = a.run("graph", 32)
+        grid = a.grid()
+        ref = oracle_grid(oracle, w, h, 7, 0, 0, thr(0.2), forced=((0, 0), (w - 1, h - 1)))
+        assert np.array_equal(grid, ref)
+        assert rep["cost"] == oracle.bfs4(ref, w, h, (0, 0), (w - 1, h - 1))
+        assert rep["error_flags"] == 0
+        a.model_finish()
+
+
+# ---------------------------------------------------------------------------------------
+# batched
+# ---------------------------------------------------------------------------------------
+
+def synthetic(gpq, seed, n):
+    return [gpq.query_synthetic(seed, q) for q in range(n)]
+
+
+def test_synthetic_specs_match_oracle(gpq, oracle):
+    for q in range(12):
+        s = gpq.query_synthetic(11, q)
+        side, t = oracle.query_shape(11, q)
+        assert (s.side, s.threshold, s.connectivity) == (side, t, 8)
+        ref = oracle.query_spec(11, q, oracle.grid(side, side, 11, 1, q, t))
+        assert (s.sx, s.sy, s.tx, s.ty) == (ref.sx, ref.sy, ref.tx, ref.ty)
+
+
+@pytest.mark.parametrize("k", [64, 256])
+def test_batch_matches_oracle(gpq, oracle, k):
+    specs = synthetic(gpq, 11, 24)
+    with gpq.AstarBatch(specs, threads_per_query=k) as b:
+        b.run("graph")
+        costs, expanded = b.results()
+        for q, s in enumerate(specs):
+            grid = oracle.grid(s.side, s.side, 11, 1, q, s.threshold)
+            if q < 3:
+                assert np.array_equal(b.grid(q).reshape(-1), grid)
+            c, _ = oracle.astar(grid, s.side, s.side, 8, (s.sx, s.sy), (s.tx, s.ty))
+            assert costs[q] == c, "query %d" % q
+            assert expanded[q] >= 1 or s.sx == s.tx and s.sy == s.ty
+
+
+def test_batch_plain_equals_graph_and_reruns(gpq):
+    specs = synthetic(gpq, 5, 6)
+    with gpq.AstarBatch(specs, threads_per_query=64, max_resident=2) as b:
+        b.run("plain")
+        c1, _ = b.results()
+        for _ in range(3):                       # generation tags: reruns in the same slots
+            b.run("graph")
+            c2, _ = b.results()
+            assert np.array_equal(c1, c2)
+
+
+def test_batch_edge_cases(gpq, oracle):
+    S = gpq._lib.QuerySpec
+    specs = [
+        S(64, 0, 10, 10, 10, 10, 8, 1, 1, 0),            # source == target
+        S(300, 0, 0, 0, 299, 299, 8, 1, 1, 1),           # obstacle-free: closed form
+        S(300, 0, 7, 250, 280, 3, 4, 1, 1, 2),           # obstacle-free 4-connected
+        S(256, thr(0.35), 0, 0, 255, 255, 4, 3, 1, 3),   # dense 4-connected
+        S(1, 0, 0, 0, 0, 0, 8, 1, 1, 4),                 # 1 x 1 grid
+        S(2048, thr(0.1), 0, 0, 2047, 2047, 8, 3, 1, 5), # the largest side
+    ]
+    with gpq.AstarBatch(specs, threads_per_query=128) as b:
+        b.run("graph")
+        costs, _ = b.results()
+    assert costs[0] == 0 and costs[4] == 0
+    assert costs[1] == 14 * 299
+    assert costs[2] == 273 + 247
+    for i in (3, 5):
+        s = specs[i]
+        grid = oracle_grid(oracle, s.side, s.side, s.seed, 1, s.query, s.threshold,
+                           forced=((s.sx, s.sy), (s.tx, s.ty)))
+        assert costs[i] == oracle.dijkstra(grid, s.side, s.side, s.connectivity, (s.sx, s.sy), (s.tx, s.ty))
+
+
+def test_batch_rejects_bad_specs(gpq):
+    S = gpq._lib.QuerySpec
+    with pytest.raises(gpq.GpqError):
+        gpq.AstarBatch([S(4096, 0, 0, 0, 1, 1, 8, 1, 1, 0)])
+    with pytest.raises(gpq.GpqError):
+        gpq.AstarBatch([S(64, 0, 0, 0, 64, 1, 8, 1, 1, 0)])
+    with pytest.raises(gpq.GpqError):
+        gpq.AstarBatch([S(64, 0, 0, 0, 1, 1, 8, 1, 1, 0)], threads_per_query=96)

@@ -11,14 +11,24 @@ QUEUE_ARGS = {
     "laq": dict(capacity_per_lp=128, stage_per_lp=16),
     "lhq": dict(capacity_per_lp=128, stage_per_lp=16),
     "lsq": dict(capacity_per_lp=128, stage_per_lp=16),
+    "gaq": dict(capacity_per_lp=64, stage_per_lp=2),
+    "hq": dict(capacity_per_lp=40, stage_per_lp=1),
 }
 
 
-def run_gpu(gpq, kind, L, k, T, mode="graph", batch=16, checks=True, **over):
+def _over(kind, L):
+    # the parallel heap's node size r: one node holds one safe event per LP (src/heap_queue.cu:1308)
+    return {"node_size": max(L, 32)} if kind == "hq" else {}
+
+
+def run_gpu(gpq, kind, L, k, T, mode="graph", batch=16, checks=True, drain=False, **over):
     args = dict(QUEUE_ARGS[kind])
+    args.update(_over(kind, L))
+    if drain:
+        args["stage_per_lp"] = 32
     args.update(over)
     q = gpq.Queue(kind, num_lps=L, lookahead=0.1, horizon=T, **args)
-    m = gpq.Phold(q, init_per_lp=k, seed=42, end_time=T, checks=checks)
+    m = gpq.Phold(q, init_per_lp=k, seed=42, end_time=T, checks=checks, drain=drain)
     m.model_init()
     assert q.queue_check_phold() == L * k
     rep = gpq.des_run(m, mode=mode, batch=batch)
@@ -50,11 +60,17 @@ def test_device_philox_bit_exact(gpq, oracle):
             assert list(dev[4 * i:4 * i + 4]) == oracle.philox(c, list(key))
 
 
+LOCAL = ["laq", "lhq", "lsq"]
+
+
+@pytest.mark.parametrize("drain", [False, True])
 @pytest.mark.parametrize("kind", sorted(QUEUE_ARGS))
 @pytest.mark.parametrize("L,k,T", [(1, 1, 20.0), (4, 2, 5.0), (64, 8, 20.0), (97, 3, 7.5), (4096, 16, 20.0)])
-def test_phold_matches_oracle_per_lp(gpq, oracle, kind, L, k, T):
+def test_phold_matches_oracle_per_lp(gpq, oracle, kind, L, k, T, drain):
+    if drain and kind not in LOCAL:
+        pytest.skip("drain mode needs a per-LP queue")
     res, cnt, hsh = oracle.phold("seq", L, k, T)
-    g = run_gpu(gpq, kind, L, k, T)
+    g = run_gpu(gpq, kind, L, k, T, drain=drain)
     assert g["flags"] == 0
     assert g["rep"]["events"] == res.events
     assert np.array_equal(g["cnt"], cnt)
@@ -71,6 +87,15 @@ def test_phold_1024_lps_matches_golden(gpq, golden, kind):
     assert "%016x" % g["hash"] == gd["trace_hash"]
     assert [int(x) for x in g["cnt"]] == gd["lp_count"]
     assert ["%016x" % int(x) for x in g["hsh"]] == gd["lp_hash"]
+
+
+def test_phold_drain_takes_fewer_windows(gpq):
+    a = run_gpu(gpq, "laq", 4096, 16, 20.0, mode="plain")
+    b = run_gpu(gpq, "laq", 4096, 16, 20.0, drain=True, mode="plain")
+    assert a["hash"] == b["hash"]
+    assert b["rep"]["windows"] * 4 < a["rep"]["windows"]
+    # every window advances the minimum by at least the lookahead
+    assert b["rep"]["windows"] - 16 <= 20.0 / 0.1 + 2
 
 
 def test_phold_plain_equals_graph(gpq):
@@ -91,12 +116,15 @@ def test_phold_65536_lps_matches_golden(gpq, golden, kind):
     assert g["pending"] == 65536 * 32
 
 
+@pytest.mark.parametrize("drain", [False, True])
 @pytest.mark.parametrize("kind", sorted(QUEUE_ARGS))
-def test_phold_2p20_lps_matches_golden(gpq, golden, kind):
+def test_phold_2p20_lps_matches_golden(gpq, golden, kind, drain):
+    if drain and kind not in LOCAL:
+        pytest.skip("drain mode needs a per-LP queue")
     gd = golden.get("phold_2p20_k32_t50")
     if gd is None:
         pytest.skip("2^20-LP golden record not generated")
-    g = run_gpu(gpq, kind, 1 << 20, 32, 50.0, checks=False, batch=64)
+    g = run_gpu(gpq, kind, 1 << 20, 32, 50.0, checks=False, batch=64, drain=drain)
     assert g["flags"] == 0
     assert g["rep"]["events"] == gd["events"]
     assert "%016x" % g["hash"] == gd["trace_hash"]
