@@ -17,7 +17,8 @@ import numpy as np
 from . import _lib
 from ._lib import DesReport, Event, GpqError, PholdConfig, QueueConfig, check
 
-__all__ = ["Queue", "Phold", "Astar", "AstarBatch", "query_synthetic", "des_init", "des_run", "des_finish",
+__all__ = ["Queue", "Phold", "Astar", "AstarBatch", "query_synthetic", "serve_trace", "serve_run", "des_init",
+           "des_run", "des_finish",
            "GpqError", "lib", "event_dtype"]
 
 lib = _lib.load()
@@ -337,3 +338,34 @@ class AstarBatch:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------------------
+# serving
+# ---------------------------------------------------------------------------------------
+def _serve_config(requests, seed=1, rate=0.0, phold_fraction=0.5, hbm_budget=0, phold_log2=(12, 18),
+                  phold_init_per_lp=16, phold_end_time=20.0, phold_batch=16, astar_lanes=4):
+    return _lib.ServeConfig(requests, seed, rate, phold_fraction, hbm_budget, phold_log2[0], phold_log2[1],
+                            phold_init_per_lp, phold_end_time, phold_batch, astar_lanes)
+
+
+def _requests_dict(arr):
+    return [{k: getattr(r, k) for k, _ in _lib.Request._fields_} for r in arr]
+
+
+def serve_trace(requests, **kw):
+    """The synthetic request trace: kind (0 PHOLD, 1 A*), size and arrival time of each request."""
+    cfg = _serve_config(requests, **kw)
+    out = (_lib.Request * requests)()
+    check(lib.gpq_serve_trace(ctypes.byref(cfg), out), "serve_trace")
+    return _requests_dict(out)
+
+
+def serve_run(requests, mode="graph", **kw):
+    """Serves the trace; mode "graph" = continuous batching, "plain" = one request at a time.
+    Returns (report dict, list of per-request dicts)."""
+    cfg = _serve_config(requests, **kw)
+    rep = _lib.ServeReport()
+    out = (_lib.Request * requests)()
+    check(lib.gpq_serve_run(ctypes.byref(cfg), _lib.DRIVE_MODES[mode], ctypes.byref(rep), out), "serve_run")
+    return {k: getattr(rep, k) for k, _ in _lib.ServeReport._fields_}, _requests_dict(out)

@@ -69,6 +69,25 @@ class BatchTuning(ctypes.Structure):
                 ("max_resident", c_u32)]
 
 
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("requests", c_u32), ("seed", c_u32), ("rate", ctypes.c_double), ("phold_fraction", ctypes.c_double),
+                ("hbm_budget", c_u64), ("phold_min_log2", c_u32), ("phold_max_log2", c_u32),
+                ("phold_init_per_lp", c_u32), ("phold_end_time", c_f32), ("phold_batch", c_u32),
+                ("astar_lanes", c_u32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", c_u32), ("size", c_u32), ("arrival_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("finish_ms", ctypes.c_double), ("result", c_u64), ("work", c_u64), ("bytes", c_u64)]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [("completed", c_u32), ("peak_inflight", c_u32), ("seconds", ctypes.c_double),
+                ("requests_per_second", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("phold_p99_ms", ctypes.c_double), ("astar_p99_ms", ctypes.c_double),
+                ("peak_bytes", c_u64)]
+
+
 # (name, restype, argtypes) for every entry point of include/gpupq.h
 SIGNATURES = [
     ("gpq_status_string", ctypes.c_char_p, [ctypes.c_int]),
@@ -114,6 +133,9 @@ SIGNATURES = [
     ("gpq_astar_batch_results", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     ("gpq_astar_batch_timing", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     ("gpq_astar_batch_grid", ctypes.c_int, [c_void_p, c_u32, c_void_p]),
+    ("gpq_serve_trace", ctypes.c_int, [ctypes.POINTER(ServeConfig), c_void_p]),
+    ("gpq_serve_run", ctypes.c_int, [ctypes.POINTER(ServeConfig), ctypes.c_int, ctypes.POINTER(ServeReport),
+                                     c_void_p]),
     ("gpq_selftest_exp", ctypes.c_int, [c_u32, c_u32, c_void_p]),
     ("gpq_selftest_sort", ctypes.c_int, [c_void_p, c_u32, c_void_p]),
     ("gpq_selftest_merge", ctypes.c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p]),

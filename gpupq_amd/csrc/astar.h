@@ -86,8 +86,15 @@ class AstarBatch {
   gpq_status timing(double* start_us, double* finish_us);
   gpq_status grid_bytes(uint32_t q, uint8_t* out);
 
-  // Searches the queries d_ids[0..n) (device array) in one persistent launch on stream s.
-  gpq_status run_subset(const uint32_t* d_ids, uint32_t n, hipStream_t s);
+  // Searches the queries d_ids[0..n) (device array) in one persistent launch on stream s,
+  // workgroups in slots [slot_base, slot_base + slots), work counter d_counter.
+  gpq_status launch(const uint32_t* d_ids, uint32_t n, uint32_t slot_base, uint32_t slots, uint32_t* d_counter,
+                    hipStream_t s);
+  // One query (device id d_id, spec spec_index) alone, one launch + host sync per iteration.
+  gpq_status plain_one(const uint32_t* d_id, uint32_t spec_index, uint32_t slot, uint32_t* d_done, uint32_t* h_done,
+                       hipStream_t s);
+  uint32_t resident_slots() const { return resident; }
+  const uint32_t* device_ids() const { return d_ids_all; }
 
   std::vector<gpq_query_spec> specs;
   gpq_batch_tuning tuning{};
@@ -96,7 +103,6 @@ class AstarBatch {
   struct Dev;                    // device-side layout (astar_batch.hip)
 
  private:
-  gpq_status launch(const uint32_t* d_ids, uint32_t n, uint32_t resident, hipStream_t s);
   DeviceArena arena;
   void* dev = nullptr;           // Dev*
   uint32_t* d_ids_all = nullptr;

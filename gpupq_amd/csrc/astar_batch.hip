@@ -67,6 +67,7 @@ struct BatchDev {
   const uint32_t* ids;                  // queries of this launch
   uint32_t n;
   uint32_t* counter;                    // work counter of the persistent launch
+  uint32_t slot_base;                   // workgroup b of the launch uses slot slot_base + b
   uint64_t g_stride;
   uint32_t heap_cap, stage_cap, log2k;
 };
@@ -279,17 +280,17 @@ __global__ __launch_bounds__(1024) void astar_batch_kernel(BatchDev d) {
     const uint32_t qi = sh.qi;
     __syncthreads();
     if (qi >= d.n) break;
-    search_query(d, sh, cnt, stage, d.ids[qi], blockIdx.x, 0, false);
+    search_query(d, sh, cnt, stage, d.ids[qi], d.slot_base + blockIdx.x, 0, false);
   }
 }
 
-// Plain drive: one iteration of query ids[0] in slot 0 per launch.
+// Plain drive: one iteration of query ids[0] in slot slot_base per launch.
 __global__ __launch_bounds__(1024) void astar_batch_step_kernel(BatchDev d, uint32_t resume, uint32_t* done) {
   extern __shared__ uint64_t lds[];
   __shared__ Shared sh;
   uint64_t* stage = lds;
   uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + (size_t)blockDim.x * d.stage_cap);
-  const bool fin = search_query(d, sh, cnt, stage, d.ids[0], 0, 1, resume != 0);
+  const bool fin = search_query(d, sh, cnt, stage, d.ids[0], d.slot_base, 1, resume != 0);
   if (threadIdx.x == 0) *done = fin ? 1u : 0u;
 }
 
@@ -447,21 +448,47 @@ AstarBatch::~AstarBatch() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
-gpq_status AstarBatch::launch(const uint32_t* d_ids, uint32_t n, uint32_t res, hipStream_t s) {
+gpq_status AstarBatch::launch(const uint32_t* d_ids, uint32_t n, uint32_t slot_base, uint32_t slots,
+                              uint32_t* d_counter, hipStream_t s) {
+  GPQ_CHECK_ARG(slots >= 1 && slot_base + slots <= resident, "batch: slot range outside the resident slots");
   Dev d = *static_cast<Dev*>(dev);
   d.ids = d_ids;
   d.n = n;
+  d.counter = d_counter;
+  d.slot_base = slot_base;
   const uint32_t K = tuning.threads_per_query;
   const size_t lds = (size_t)K * (tuning.stage_capacity * 8 + 4);
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, counter);
-  hipLaunchKernelGGL(astar_batch_kernel, dim3(std::max<uint32_t>(1, std::min(res, n))), dim3(K), lds, s,
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, d_counter);
+  hipLaunchKernelGGL(astar_batch_kernel, dim3(std::max<uint32_t>(1, std::min(slots, n))), dim3(K), lds, s,
                      static_cast<BatchDev>(d));
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }
 
-gpq_status AstarBatch::run_subset(const uint32_t* d_ids, uint32_t n, hipStream_t s) {
-  return launch(d_ids, n, resident, s);
+// The one-query-per-launch path: query ids[0] (device) alone in `slot`, one launch per
+// iteration, each followed by a host synchronization that reads the done flag.
+gpq_status AstarBatch::plain_one(const uint32_t* d_id, uint32_t spec_index, uint32_t slot, uint32_t* d_done,
+                                 uint32_t* h_done, hipStream_t s) {
+  GPQ_CHECK_ARG(slot < resident, "batch: slot outside the resident slots");
+  Dev d = *static_cast<Dev*>(dev);
+  d.ids = d_id;
+  d.n = 1;
+  d.slot_base = slot;
+  const uint32_t K = tuning.threads_per_query;
+  const size_t lds = (size_t)K * (tuning.stage_capacity * 8 + 4);
+  const uint64_t step_cap = 64ull * specs[spec_index].side * specs[spec_index].side + 1024;
+  for (uint64_t step = 0;; ++step) {
+    if (step > step_cap) {
+      set_error("batch plain: step bound exceeded");
+      return GPQ_ERR_STATE;
+    }
+    hipLaunchKernelGGL(astar_batch_step_kernel, dim3(1), dim3(K), lds, s, static_cast<BatchDev>(d), step ? 1u : 0u,
+                       d_done);
+    GPQ_HIP(hipGetLastError());
+    GPQ_HIP(hipMemcpyAsync(h_done, d_done, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    GPQ_HIP(hipStreamSynchronize(s));
+    if (*h_done) return GPQ_OK;
+  }
 }
 
 gpq_status AstarBatch::run(gpq_drive_mode mode, double* seconds) {
@@ -471,7 +498,7 @@ gpq_status AstarBatch::run(gpq_drive_mode mode, double* seconds) {
   if (mode == GPQ_DRIVE_GRAPH) {
     if (!graph_exec) {
       GPQ_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-      const gpq_status st = launch(d_ids_all, n, resident, stream);
+      const gpq_status st = launch(d_ids_all, n, 0, resident, counter, stream);
       hipGraph_t gph = nullptr;
       const hipError_t e = hipStreamEndCapture(stream, &gph);
       if (st != GPQ_OK) return st;
@@ -482,35 +509,10 @@ gpq_status AstarBatch::run(gpq_drive_mode mode, double* seconds) {
     GPQ_HIP(hipGraphLaunch(graph_exec, stream));
     GPQ_HIP(hipStreamSynchronize(stream));
   } else {
-    Dev d = *static_cast<Dev*>(dev);
-    const uint32_t K = tuning.threads_per_query;
-    const size_t lds = (size_t)K * (tuning.stage_capacity * 8 + 4);
     uint32_t* h_done = nullptr;
     GPQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_done), sizeof(uint32_t), hipHostMallocDefault));
     gpq_status st = GPQ_OK;
-    for (uint32_t qi = 0; qi < n && st == GPQ_OK; ++qi) {
-      d.ids = d_ids_all + qi;
-      d.n = 1;
-      const uint64_t step_cap = 64ull * specs[qi].side * specs[qi].side + 1024;
-      for (uint64_t step = 0;; ++step) {
-        if (step > step_cap) {
-          set_error("batch plain: step bound exceeded");
-          st = GPQ_ERR_STATE;
-          break;
-        }
-        hipLaunchKernelGGL(astar_batch_step_kernel, dim3(1), dim3(K), lds, stream, static_cast<BatchDev>(d),
-                           step ? 1u : 0u, counter + 1);
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(h_done, counter + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) {
-          set_error(std::string("batch plain step: ") + hipGetErrorString(e));
-          st = GPQ_ERR_HIP;
-          break;
-        }
-        if (*h_done) break;
-      }
-    }
+    for (uint32_t qi = 0; qi < n && st == GPQ_OK; ++qi) st = plain_one(d_ids_all + qi, qi, 0, counter + 1, h_done, stream);
     (void)hipHostFree(h_done);
     if (st != GPQ_OK) return st;
   }

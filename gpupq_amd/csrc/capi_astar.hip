@@ -1,7 +1,7 @@
 // capi_astar.hip -- the extern "C" boundary of the A* models (include/gpupq.h).
 #include <string>
 
-#include "astar.h"
+#include "serve.h"
 
 using gpq::set_error;
 
@@ -103,6 +103,18 @@ gpq_status gpq_astar_batch_grid(gpq_astar_batch* b, uint32_t q, uint8_t* blocked
   GPQ_NONNULL(b);
   GPQ_NONNULL(blocked);
   return b->impl->grid_bytes(q, blocked);
+}
+
+gpq_status gpq_serve_trace(const gpq_serve_config* cfg, gpq_request* out) {
+  GPQ_NONNULL(cfg);
+  GPQ_NONNULL(out);
+  return gpq::serve_trace(*cfg, out, nullptr);
+}
+
+gpq_status gpq_serve_run(const gpq_serve_config* cfg, gpq_drive_mode mode, gpq_serve_report* rep, gpq_request* out) {
+  GPQ_NONNULL(cfg);
+  if (mode != GPQ_DRIVE_PLAIN && mode != GPQ_DRIVE_GRAPH) { set_error("bad drive mode"); return GPQ_ERR_INVALID; }
+  return gpq::serve_run(*cfg, mode, rep, out);
 }
 
 }  // extern "C"

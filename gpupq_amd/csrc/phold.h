@@ -35,6 +35,8 @@ class PholdModel {
   gpq_status lp_state(uint32_t* counts, uint64_t* hashes);
   gpq_status trace_hash(uint64_t* out);
   gpq_status run(gpq_drive_mode mode, uint32_t batch, gpq_des_report* rep);
+  gpq_status prepare_graph(uint32_t batch);
+  gpq_status launch_async(hipEvent_t done);
   gpq_status profile(uint32_t batch, gpq_kernel_profile* out);
 
   gpq_phold_config cfg{};

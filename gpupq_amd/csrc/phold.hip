@@ -276,6 +276,35 @@ gpq_status PholdModel::trace_hash(uint64_t* out) {
   return GPQ_OK;
 }
 
+// Captures `batch` steps (queue_pre, model step, queue_post) into a graph, once per batch size.
+gpq_status PholdModel::prepare_graph(uint32_t batch) {
+  GPQ_CHECK_ARG(batch >= 1, "batch must be >= 1");
+  if (graph_exec && graph_batch == batch) return GPQ_OK;
+  if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
+  if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+  GPQ_HIP(hipStreamBeginCapture(q->stream, hipStreamCaptureModeThreadLocal));
+  gpq_status s = GPQ_OK;
+  for (uint32_t i = 0; i < batch && s == GPQ_OK; ++i) s = step();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(q->stream, &g);
+  if (s != GPQ_OK) return s;
+  GPQ_HIP(e);
+  graph = g;
+  GPQ_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+  graph_batch = batch;
+  return GPQ_OK;
+}
+
+// Non-blocking drive for callers that advance many models at once: one graph replay, the
+// window state copied to the queue's pinned mirror, and `done` recorded behind both.
+gpq_status PholdModel::launch_async(hipEvent_t done) {
+  GPQ_CHECK_ARG(graph_exec, "launch_async before prepare_graph");
+  GPQ_HIP(hipGraphLaunch(graph_exec, q->stream));
+  GPQ_HIP(hipMemcpyAsync(q->host_ws, q->ws, sizeof(WindowState), hipMemcpyDeviceToHost, q->stream));
+  GPQ_HIP(hipEventRecord(done, q->stream));
+  return GPQ_OK;
+}
+
 // des_run (src/des.cu:134-221).  Plain: the reference's pattern, one host round trip
 // per step for the minimum timestamp.  Graph: `batch` steps captured once and replayed;
 // after the last window, remaining steps of a batch find nothing to commit.
@@ -293,20 +322,7 @@ gpq_status PholdModel::run(gpq_drive_mode mode, uint32_t batch, gpq_des_report* 
       ++steps;
     }
   } else {
-    if (!graph_exec || graph_batch != batch) {
-      if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
-      if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
-      GPQ_HIP(hipStreamBeginCapture(q->stream, hipStreamCaptureModeThreadLocal));
-      gpq_status s = GPQ_OK;
-      for (uint32_t i = 0; i < batch && s == GPQ_OK; ++i) s = step();
-      hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(q->stream, &g);
-      if (s != GPQ_OK) return s;
-      GPQ_HIP(e);
-      graph = g;
-      GPQ_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-      graph_batch = batch;
-    }
+    GPQ_TRY(prepare_graph(batch));
     for (;;) {
       GPQ_TRY(q->window(&w));
       if (w.done || w.error) break;

@@ -270,6 +270,52 @@ gpq_status gpq_astar_batch_timing(gpq_astar_batch* b, double* start_us, double* 
 /* Obstacle map of query q (side*side bytes). */
 gpq_status gpq_astar_batch_grid(gpq_astar_batch* b, uint32_t q, uint8_t* blocked);
 
+/* ---- Serving: a stream of mixed requests (PHOLD simulations and A* queries) --------------
+ * Request i of the synthetic trace: Philox4x32-10 of counter {i, 0, 6, 0} under key {seed, 0}
+ * gives the gap to the previous arrival (Exp with mean 1/rate; rate 0: all arrive at t = 0),
+ * the kind (word 1 < phold_fraction * 2^32: PHOLD), the PHOLD size (LPs = round(2^U[min, max]),
+ * init_per_lp events per LP, end time, lookahead 0.1, p_self 0.5) and the PHOLD seed (word 3).
+ * An A* request i is gpq_query_synthetic(seed, i). */
+typedef struct {
+  uint32_t requests;
+  uint32_t seed;
+  double rate;                       /* arrivals per second */
+  double phold_fraction;             /* share of PHOLD requests */
+  uint64_t hbm_budget;               /* bytes admitted PHOLD requests may hold at once (0: 32 GiB) */
+  uint32_t phold_min_log2, phold_max_log2;   /* 0: 12 and 18 */
+  uint32_t phold_init_per_lp;        /* 0: 16 */
+  float phold_end_time;              /* 0: 20 */
+  uint32_t phold_batch;              /* steps per graph replay of a served PHOLD request (0: 16) */
+  uint32_t astar_lanes;              /* concurrent A* launches when served (0: 4) */
+} gpq_serve_config;
+
+typedef struct {
+  uint32_t kind;                     /* 0 PHOLD, 1 A* */
+  uint32_t size;                     /* PHOLD: LPs; A*: grid side */
+  double arrival_ms, start_ms, finish_ms;   /* from the start of the run */
+  uint64_t result;                   /* PHOLD: trace hash; A*: optimal cost (UINT64_MAX: unreachable) */
+  uint64_t work;                     /* PHOLD: committed events; A*: expansions */
+  uint64_t bytes;                    /* HBM held while admitted */
+} gpq_request;
+
+typedef struct {
+  uint32_t completed;
+  uint32_t peak_inflight;
+  double seconds;                    /* first arrival to last completion */
+  double requests_per_second;
+  double p50_ms, p99_ms, mean_ms;    /* latency = completion - arrival */
+  double phold_p99_ms, astar_p99_ms;
+  uint64_t peak_bytes;               /* most HBM held by admitted requests at once */
+} gpq_serve_report;
+
+/* The trace alone (kind, size, arrival_ms of each request). */
+gpq_status gpq_serve_trace(const gpq_serve_config* cfg, gpq_request* out);
+/* Serves the trace.  GPQ_DRIVE_GRAPH: continuous batching -- admission in arrival order under
+ * the HBM budget, each PHOLD request on its own stream by graph replays, A* queries gathered
+ * into persistent-kernel launches on astar_lanes streams.  GPQ_DRIVE_PLAIN: one request at a
+ * time, a host synchronization per step.  out (may be NULL) receives every request. */
+gpq_status gpq_serve_run(const gpq_serve_config* cfg, gpq_drive_mode mode, gpq_serve_report* rep, gpq_request* out);
+
 /* ---- self-test hooks: the device random-number path evaluated in bulk ------------------- */
 /* out[i] = device Exp(1) sample of draw ((first + i) << 8), i < count (count <= 2^24). */
 gpq_status gpq_selftest_exp(uint32_t first, uint32_t count, float* out);
