@@ -9,9 +9,18 @@ of a step is the whole step loop, driven by hipGraph replays.
 
 Beside the JSON line's value:
   roofline      the dominant kernel's algorithmic bytes per launch / its mean
-                launch duration (HIP events on the queue's stream), against 8 TB/s
+                launch duration (HIP events on the queue's stream), against 8 TB/s;
+                traffic = HBM bytes per launch from the rocprofv3 counter summary
+                committed in profiles/ (null when absent)
   cpu_baseline  the sequential std::priority_queue oracle on a bounded sample of
                 the same workload (end time 0.5), one host core
+  astar_batch   512 synthetic grid queries (side 2^U[8,11], density U[0.1,0.3],
+                8-connected) in one persistent launch: queries/s, p50/p99 completion
+                time, against the plain one-query-per-launch path (host sync per
+                expansion step) on the first queries, and the oracle's A* on one core
+  serving       a Poisson trace of mixed PHOLD / A* requests served by continuous
+                batching: requests/s, p50/p99 latency, against serving them one at a
+                time with a host sync per step
 """
 import argparse
 import json
@@ -34,6 +43,8 @@ QUEUES = {
     "laq": dict(capacity_per_lp=128, stage_per_lp=16),
     "lhq": dict(capacity_per_lp=128, stage_per_lp=16),
     "lsq": dict(capacity_per_lp=128, stage_per_lp=16),
+    "gaq": dict(capacity_per_lp=64, stage_per_lp=2),
+    "hq": dict(capacity_per_lp=40, stage_per_lp=1),
 }
 DRAIN = False
 
@@ -42,6 +53,8 @@ def make(gpq, kind, wl, stats=False, checks=False):
     args = dict(QUEUES[kind])
     if DRAIN:
         args["stage_per_lp"] = 32      # a drained window can send ~3 events to one LP on average
+    if kind == "hq":
+        args["node_size"] = max(wl["num_lps"], 32)   # one node holds one safe event per LP
     q = gpq.Queue(kind, num_lps=wl["num_lps"], lookahead=0.1, horizon=wl["end_time"], block_threads=256,
                   collect_stats=stats, **args)
     m = gpq.Phold(q, init_per_lp=wl["init_per_lp"], seed=42, end_time=wl["end_time"], checks=checks, drain=DRAIN)
@@ -64,9 +77,10 @@ def roofline(gpq, kind, wl, log):
     d = {k: after[k] - before[k] for k in after}
     launches = max(d["post_launches"], 1)
     L = wl["num_lps"]
-    # queue post: sorted insert of k records displacing d (16*(2k+2d)) + one head word per LP
-    post_bytes = (RECORD * (2 * d["inserted"] + 2 * d["displaced"]) + 4 * d["scanned_lps"] + d["aux_bytes"]
-                  + 4 * launches) / launches
+    # queue post kernel: sorted insert of k records displacing d (16*(2k+2d)), one head word
+    # per LP for the window scan, one partial minimum per 256-LP block
+    post_bytes = ((RECORD * (2 * d["inserted"] + 2 * d["displaced"]) + 4 * d["scanned_lps"] + d["aux_bytes"])
+                  / launches + 4 * ((L + 255) // 256))
     # model step: one head word per LP, and per committed event the head record (16), the
     # per-LP state read+write (count 8, hash 16, ring indices 16, head 4 + next head 4),
     # the staging counter (4) and the staged record (16)
@@ -75,6 +89,7 @@ def roofline(gpq, kind, wl, log):
     kernels = {
         "queue_post": {"us": prof["post_kernel_us"], "alg_bytes": post_bytes},
         "model_step": {"us": prof["step_kernel_us"], "alg_bytes": step_bytes},
+        "window_publish": {"us": prof["window_kernel_us"], "alg_bytes": 4 * ((L + 255) // 256) + 24},
     }
     for k in kernels.values():
         k["gbps"] = k["alg_bytes"] / (k["us"] * 1e-6) / 1e9 if k["us"] > 0 else 0.0
@@ -94,6 +109,97 @@ def cpu_baseline(wl, log):
                       "(%d events), event loop only" % (wl["num_lps"], wl["init_per_lp"], sample_T, res.events)}
 
 
+ASTAR_SEED = 2024
+ASTAR_QUERIES = 512
+
+
+def pct(v, p):
+    v = sorted(v)
+    k = max(0, min(len(v) - 1, int(-(-p * len(v) // 100)) - 1))
+    return v[k]
+
+
+def astar_bench(gpq, log, plain_n=16, threads=256, cpu=True):
+    specs = [gpq.query_synthetic(ASTAR_SEED, q) for q in range(ASTAR_QUERIES)]
+    with gpq.AstarBatch(specs, threads_per_query=threads) as b:
+        b.run("graph")                                  # capture + warm up
+        t = b.run("graph")
+        start, finish = b.timing()
+        costs, expanded = b.results()
+    with gpq.AstarBatch(specs[:plain_n], threads_per_query=threads) as b2:
+        tp = b2.run("plain")
+        cp, _ = b2.results()
+        b2.run("graph")
+        tb = b2.run("graph")
+    if list(cp) != list(costs[:plain_n]):
+        raise RuntimeError("A*: plain and batched costs differ")
+    out = {"queries": ASTAR_QUERIES, "seed": ASTAR_SEED, "threads_per_query": threads,
+           "queries_per_s": ASTAR_QUERIES / t, "seconds": t,
+           "expanded_per_s": float(expanded.sum()) / t,
+           "p50_completion_ms": pct(finish, 50) / 1000.0, "p99_completion_ms": pct(finish, 99) / 1000.0,
+           "plain_subset": plain_n, "plain_queries_per_s": plain_n / tp, "batched_subset_queries_per_s": plain_n / tb,
+           "batched_vs_plain": (ASTAR_QUERIES / t) / (plain_n / tp), "unreachable": int((costs < 0).sum())}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import Oracle
+        o = Oracle()
+        n_cpu, t_cpu = 0, 0.0
+        for q in range(ASTAR_QUERIES):
+            s = specs[q]
+            grid = o.grid(s.side, s.side, ASTAR_SEED, 1, q, s.threshold)
+            t0 = time.perf_counter()
+            c, _ = o.astar(grid, s.side, s.side, 8, (s.sx, s.sy), (s.tx, s.ty))
+            t_cpu += time.perf_counter() - t0
+            n_cpu += 1
+            if c != costs[q]:
+                raise RuntimeError("A*: query %d cost %d, oracle %d" % (q, costs[q], c))
+            if t_cpu > 20.0:
+                break
+        out["cpu_queries_per_s"] = n_cpu / t_cpu
+        out["cpu_sample"] = "oracle A* (binary heap), 1 core, first %d queries, search only" % n_cpu
+        out["costs_vs_cpu"] = "match (%d queries)" % n_cpu
+    log("astar_batch: %s" % json.dumps(out))
+    return out
+
+
+def serving_bench(gpq, log, requests=64, rate=200.0):
+    kw = dict(seed=7, rate=rate, phold_fraction=0.5)
+    rep_s, served = gpq.serve_run(requests, mode="graph", **kw)
+    rep_p, plain = gpq.serve_run(requests, mode="plain", **kw)
+    if [r["result"] for r in served] != [r["result"] for r in plain]:
+        raise RuntimeError("serving: served and plain results differ")
+    out = {"requests": requests, "rate_per_s": rate, "phold_lps": "2^U[12,18] x 16, end time 20",
+           "astar": "side 2^U[8,11], 8-connected",
+           "served": {k: rep_s[k] for k in ("requests_per_second", "p50_ms", "p99_ms", "phold_p99_ms",
+                                            "astar_p99_ms", "peak_inflight", "peak_bytes")},
+           "plain": {k: rep_p[k] for k in ("requests_per_second", "p50_ms", "p99_ms")},
+           "results_served_vs_plain": "match"}
+    log("serving: %s" % json.dumps(out))
+    return out
+
+
+def traffic_from_profiles(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    e = d.get(workload, {}).get(kernel)
+    return None if e is None else e.get("bytes_per_launch")
+
+
+def copy_calibration():
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 1 << 28                                         # 1 GiB of float32
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +213,16 @@ def main():
                          "LP's safe window; the committed trace is the same")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-astar", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--skip-phold", action="store_true", help="profiling aid: A* and serving parts only")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy five times (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not provided "
               "(the multi-GPU sharding is a design note only)" % args.gpus, file=sys.stderr)
@@ -125,6 +240,14 @@ def main():
     if os.path.exists(gpath):
         golden = json.load(open(gpath)).get(args.workload)
 
+    if args.skip_phold:
+        line = {"metric": METRIC, "value": None, "unit": "events/s", "n_gpus": 1, "note": "PHOLD part skipped"}
+        if not args.no_astar:
+            line["astar_batch"] = astar_bench(gpq, log, cpu=not args.no_cpu)
+        if not args.no_serving:
+            line["serving"] = serving_bench(gpq, log, rate=args.serve_rate)
+        print(json.dumps(line), flush=True)
+        return
     total_events, total_s, hashes = 0, 0.0, set()
     for i in range(args.warmup + args.steps):
         q, m = make(gpq, args.queue, wl)
@@ -157,16 +280,22 @@ def main():
                    "window": "drain each LP's safe window" if DRAIN else "one event per LP per window",
                    "trace_hash_vs_cpu": parity},
     }
-    if not args.no_roofline:
+    if not args.no_roofline and args.queue in ("laq", "lhq", "lsq"):
         dom, kernels, counters = roofline(gpq, args.queue, wl, log)
         k = kernels[dom]
+        kname = "laq_post_kernel" if args.queue == "laq" and dom == "queue_post" else dom
         line["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS,
-                            "unit": "GB/s", "frac": k["gbps"] / HBM_PEAK_GBPS, "traffic": None,
+                            "unit": "GB/s", "frac": k["gbps"] / HBM_PEAK_GBPS,
+                            "traffic": traffic_from_profiles(kname, args.workload),
                             "alg_bytes_per_launch": k["alg_bytes"], "mean_launch_us": k["us"],
                             "other_kernels": {n: {"achieved": v["gbps"], "mean_launch_us": v["us"]}
                                               for n, v in kernels.items() if n != dom}}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(wl, log)
+    if not args.no_astar:
+        line["astar_batch"] = astar_bench(gpq, log, cpu=not args.no_cpu)
+    if not args.no_serving:
+        line["serving"] = serving_bench(gpq, log, rate=args.serve_rate)
     print(json.dumps(line), flush=True)
 
 

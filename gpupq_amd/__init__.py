@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib
 from ._lib import DesReport, Event, GpqError, PholdConfig, QueueConfig, check
 
-__all__ = ["Queue", "Phold", "Astar", "AstarBatch", "query_synthetic", "serve_trace", "serve_run", "des_init",
+__all__ = ["Queue", "Phold", "Astar", "AstarBatch", "autotune_batch", "query_synthetic", "serve_trace", "serve_run", "des_init",
            "des_run", "des_finish",
            "GpqError", "lib", "event_dtype"]
 
@@ -282,6 +282,17 @@ def query_synthetic(seed, q):
     s = _lib.QuerySpec()
     check(lib.gpq_query_synthetic(seed, q, ctypes.byref(s)), "query_synthetic")
     return s
+
+
+def autotune_batch(specs, sample=64, heap_capacity=0, max_resident=0):
+    """Fastest (threads_per_query, stage_capacity) for these queries, timed on the first
+    `sample`; returns (kwargs for AstarBatch, seconds of the best candidate)."""
+    arr = (_lib.QuerySpec * len(specs))(*specs)
+    t = _lib.BatchTuning(0, heap_capacity, 0, max_resident)
+    sec = ctypes.c_double()
+    check(lib.gpq_astar_batch_autotune(arr, len(specs), sample, ctypes.byref(t), ctypes.byref(sec)), "autotune")
+    return dict(threads_per_query=t.threads_per_query, heap_capacity=t.heap_capacity,
+                stage_capacity=t.stage_capacity, max_resident=t.max_resident), sec.value
 
 
 class AstarBatch:

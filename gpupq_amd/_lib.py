@@ -45,7 +45,8 @@ class QueueStats(ctypes.Structure):
 
 class KernelProfile(ctypes.Structure):
     _fields_ = [("steps", c_u64), ("events", c_u64), ("step_kernel_us", ctypes.c_double),
-                ("post_kernel_us", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("post_kernel_us", ctypes.c_double), ("window_kernel_us", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
 
 
 class AstarConfig(ctypes.Structure):
@@ -129,6 +130,8 @@ SIGNATURES = [
     ("gpq_query_synthetic", ctypes.c_int, [c_u32, c_u32, ctypes.POINTER(QuerySpec)]),
     ("gpq_astar_batch_create", ctypes.c_int, [c_void_p, c_u32, ctypes.POINTER(BatchTuning), ctypes.POINTER(c_void_p)]),
     ("gpq_astar_batch_destroy", ctypes.c_int, [c_void_p]),
+    ("gpq_astar_batch_autotune", ctypes.c_int, [c_void_p, c_u32, c_u32, ctypes.POINTER(BatchTuning),
+                                                ctypes.POINTER(ctypes.c_double)]),
     ("gpq_astar_batch_run", ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     ("gpq_astar_batch_results", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     ("gpq_astar_batch_timing", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),

@@ -76,6 +76,8 @@ class AstarModel {
 };
 
 gpq_status query_synthetic(uint32_t seed, uint32_t q, gpq_query_spec* out);
+gpq_status batch_autotune(const gpq_query_spec* specs, uint32_t count, uint32_t sample, const gpq_batch_tuning& base,
+                          gpq_batch_tuning* out, double* best_seconds);
 
 class AstarBatch {
  public:

@@ -521,6 +521,46 @@ gpq_status AstarBatch::run(gpq_drive_mode mode, double* seconds) {
   return GPQ_OK;
 }
 
+// Times every candidate (threads per query x stage capacity) on the first `sample` queries
+// (two persistent runs each, the second timed) and returns the fastest; heap_capacity and
+// max_resident are kept from `base`.  Candidates whose open lists overflow are skipped.
+gpq_status batch_autotune(const gpq_query_spec* sp, uint32_t count, uint32_t sample, const gpq_batch_tuning& base,
+                          gpq_batch_tuning* out, double* best_seconds) {
+  GPQ_CHECK_ARG(sp && count >= 1, "autotune: no queries");
+  const uint32_t n = sample ? std::min(sample, count) : count;
+  static const uint32_t kThreads[] = {64, 128, 256, 512, 1024};
+  static const uint32_t kStage[] = {8, 16, 32};
+  double best = 1e300;
+  gpq_batch_tuning pick{};
+  for (uint32_t k : kThreads)
+    for (uint32_t st : kStage) {
+      if ((size_t)k * (st * 8 + 4) > 64 * 1024) continue;
+      gpq_batch_tuning t = base;
+      t.threads_per_query = k;
+      t.stage_capacity = st;
+      AstarBatch* b = nullptr;
+      if (AstarBatch::create(sp, n, t, &b) != GPQ_OK) continue;
+      double sec = 0.0;
+      gpq_status s = b->run(GPQ_DRIVE_GRAPH, &sec);
+      if (s == GPQ_OK) s = b->run(GPQ_DRIVE_GRAPH, &sec);
+      if (s == GPQ_OK) s = b->results(nullptr, nullptr);
+      const gpq_batch_tuning used = b->tuning;
+      delete b;
+      if (s == GPQ_OK && sec < best) {
+        best = sec;
+        pick = used;
+        pick.max_resident = base.max_resident;
+      }
+    }
+  if (best == 1e300) {
+    set_error("autotune: no candidate completed");
+    return GPQ_ERR_CAPACITY;
+  }
+  *out = pick;
+  if (best_seconds) *best_seconds = best;
+  return GPQ_OK;
+}
+
 gpq_status AstarBatch::results(int64_t* costs, uint64_t* expanded) {
   const Dev* d = static_cast<const Dev*>(dev);
   const size_t n = specs.size();

@@ -87,6 +87,7 @@ gpq_status QueueBase::sum_u32(const uint32_t* d_values, uint32_t count, uint64_t
 }
 
 gpq_status QueueBase::publish_window() {
+  if (profile_mark) GPQ_HIP(hipEventRecord(profile_mark, stream));
   hipLaunchKernelGGL(window_publish_kernel, dim3(1), dim3(1024), 0, stream, partial, partial_slots, ws,
                      cfg.lookahead, cfg.horizon);
   GPQ_HIP(hipGetLastError());

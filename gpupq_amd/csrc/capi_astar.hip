@@ -76,6 +76,14 @@ gpq_status gpq_astar_batch_create(const gpq_query_spec* specs, uint32_t count, c
   return GPQ_OK;
 }
 
+gpq_status gpq_astar_batch_autotune(const gpq_query_spec* specs, uint32_t count, uint32_t sample,
+                                    gpq_batch_tuning* tuning, double* best_seconds) {
+  GPQ_NONNULL(specs);
+  GPQ_NONNULL(tuning);
+  gpq_batch_tuning base = *tuning;
+  return gpq::batch_autotune(specs, count, sample, base, tuning, best_seconds);
+}
+
 gpq_status gpq_astar_batch_destroy(gpq_astar_batch* b) {
   if (!b) return GPQ_OK;
   delete b->impl;

@@ -106,6 +106,7 @@ class QueueBase {
   DeviceArena arena;
   WindowState* ws = nullptr;          // device
   WindowState* host_ws = nullptr;     // pinned mirror for polling
+  hipEvent_t profile_mark = nullptr;  // when set, recorded just before the window publication
   uint32_t* partial = nullptr;        // device, one slot per reduction block
   uint32_t partial_slots = 0;
   uint32_t threads = 256;

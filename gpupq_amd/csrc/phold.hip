@@ -59,13 +59,57 @@ __device__ __forceinline__ bool phold_commit_one(const Q& q, const PholdDev& m, 
   return true;
 }
 
+// Drain mode, any queue: commit the slot's events while its head is inside the window.
+template <class Q>
+__device__ __forceinline__ void phold_drain(const Q& q, const PholdDev& m, uint32_t slot) {
+  while (phold_commit_one(q, m, slot)) {
+  }
+}
+
+// Drain mode on the local array queue: the same commits in the same order as repeated
+// peek / set_done, with the LP's ring cursor, count and hash held in registers and the
+// next record loaded while the current one is processed; written back once.
+__device__ __forceinline__ void phold_drain(const LaqView& q, const PholdDev& m, uint32_t lp) {
+  const uint32_t wend = q.ws->wend;
+  if (!(q.head[lp] < wend)) return;
+  const uint32_t mask = (1u << q.cap_log2) - 1u;
+  const Event* ring = q.ring + ((size_t)lp << q.cap_log2);
+  uint32_t rd = q.rd[lp], len = q.len[lp];
+  uint32_t n = m.count[lp];
+  uint64_t h = m.hash[lp];
+  uint32_t last = m.last_ts ? m.last_ts[lp] : 0u;
+  Event cur = load_event(&ring[rd]);
+  while (len && cur.ts < wend) {
+    const Event nxt = len > 1 ? load_event(&ring[(rd + 1) & mask]) : Event{kInfBits, 0u, 0u, 0u};
+    h = trace_step(h, cur.ts, cur.src, cur.seq);
+    if (m.last_ts) {
+      if (cur.ts < last) atomicOr(&m.ws->error, kErrBadState);
+      last = cur.ts;
+    }
+    const uint4 r = philox4x32_10(make_uint4(n, lp, 0u, 0u), make_uint2(m.seed, 0u));
+    const float ex = exp_from_u32(r.x);
+    const uint32_t dst = r.y < m.p_self ? lp : (uint32_t)(((uint64_t)r.z * m.num_lps) >> 32);
+    const float t_new = (bits_to_f(cur.ts) + m.lookahead) + ex;
+    q.insert(Event{f_to_bits(t_new), dst, lp, m.init_per_lp + n});
+    ++n;
+    rd = (rd + 1) & mask;
+    --len;
+    cur = nxt;
+  }
+  q.rd[lp] = rd;
+  q.len[lp] = len;
+  q.head[lp] = len ? cur.ts : kInfBits;
+  m.count[lp] = n;
+  m.hash[lp] = h;
+  if (m.last_ts) m.last_ts[lp] = last;
+}
+
 template <class Q>
 __global__ __launch_bounds__(kStepBlock) void phold_step_kernel(Q q, PholdDev m) {
   const uint32_t slots = q.slots();
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += gridDim.x * blockDim.x) {
     if (m.drain) {
-      while (phold_commit_one(q, m, s)) {
-      }
+      phold_drain(q, m, s);
     } else {
       (void)phold_commit_one(q, m, s);
     }
@@ -349,18 +393,24 @@ gpq_status PholdModel::run(gpq_drive_mode mode, uint32_t batch, gpq_des_report* 
   return GPQ_OK;
 }
 
-// Per-kernel timing: steps launched eagerly in groups of `batch`, four events per step
-// (before queue_pre, before the model step kernel, after it, after queue_post).
+// Per-kernel timing: steps launched eagerly in groups of `batch`, five events per step
+// (before queue_pre, before the model step kernel, after it, before the window publication
+// of a per-LP queue's post, after queue_post).
 gpq_status PholdModel::profile(uint32_t batch, gpq_kernel_profile* out) {
   GPQ_CHECK_ARG(batch >= 1 && batch <= 4096, "profile batch must be in [1, 4096]");
-  std::vector<hipEvent_t> ev(4 * (size_t)batch, nullptr);
+  std::vector<hipEvent_t> ev(5 * (size_t)batch, nullptr);
   for (auto& e : ev) GPQ_HIP(hipEventCreate(&e));
   struct Release {
     std::vector<hipEvent_t>& v;
-    ~Release() { for (auto e : v) if (e) (void)hipEventDestroy(e); }
-  } release{ev};
+    QueueBase* q;
+    ~Release() {
+      q->profile_mark = nullptr;
+      for (auto e : v) if (e) (void)hipEventDestroy(e);
+    }
+  } release{ev, q};
+  const bool marked = q->cfg.kind == GPQ_LAQ || q->cfg.kind == GPQ_LHQ || q->cfg.kind == GPQ_LSQ;
   GPQ_HIP(hipStreamSynchronize(q->stream));
-  double step_ms = 0.0, queue_ms = 0.0;
+  double step_ms = 0.0, queue_ms = 0.0, window_ms = 0.0;
   uint64_t steps = 0;
   WindowState w{};
   const auto t0 = std::chrono::steady_clock::now();
@@ -368,22 +418,32 @@ gpq_status PholdModel::profile(uint32_t batch, gpq_kernel_profile* out) {
     GPQ_TRY(q->window(&w));
     if (w.done || w.error) break;
     for (uint32_t i = 0; i < batch; ++i) {
-      GPQ_HIP(hipEventRecord(ev[4 * i + 0], q->stream));
+      hipEvent_t* e = &ev[5 * (size_t)i];
+      GPQ_HIP(hipEventRecord(e[0], q->stream));
       GPQ_TRY(q->pre());
-      GPQ_HIP(hipEventRecord(ev[4 * i + 1], q->stream));
+      GPQ_HIP(hipEventRecord(e[1], q->stream));
       GPQ_TRY(launch_step());
-      GPQ_HIP(hipEventRecord(ev[4 * i + 2], q->stream));
+      GPQ_HIP(hipEventRecord(e[2], q->stream));
+      q->profile_mark = marked ? e[3] : nullptr;
       GPQ_TRY(q->post());
-      GPQ_HIP(hipEventRecord(ev[4 * i + 3], q->stream));
+      q->profile_mark = nullptr;
+      GPQ_HIP(hipEventRecord(e[4], q->stream));
     }
-    GPQ_HIP(hipEventSynchronize(ev[4 * (batch - 1) + 3]));
+    GPQ_HIP(hipEventSynchronize(ev[5 * (size_t)(batch - 1) + 4]));
     for (uint32_t i = 0; i < batch; ++i) {
-      float a = 0, b = 0, c = 0;
-      GPQ_HIP(hipEventElapsedTime(&a, ev[4 * i + 0], ev[4 * i + 1]));
-      GPQ_HIP(hipEventElapsedTime(&b, ev[4 * i + 1], ev[4 * i + 2]));
-      GPQ_HIP(hipEventElapsedTime(&c, ev[4 * i + 2], ev[4 * i + 3]));
+      hipEvent_t* e = &ev[5 * (size_t)i];
+      float a = 0, b = 0, c = 0, d = 0;
+      GPQ_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+      GPQ_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+      if (marked) {
+        GPQ_HIP(hipEventElapsedTime(&c, e[2], e[3]));
+        GPQ_HIP(hipEventElapsedTime(&d, e[3], e[4]));
+      } else {
+        GPQ_HIP(hipEventElapsedTime(&c, e[2], e[4]));
+      }
       step_ms += b;
       queue_ms += a + c;
+      window_ms += d;
     }
     steps += batch;
   }
@@ -393,6 +453,7 @@ gpq_status PholdModel::profile(uint32_t batch, gpq_kernel_profile* out) {
   out->seconds = std::chrono::duration<double>(t1 - t0).count();
   out->step_kernel_us = steps ? 1000.0 * step_ms / (double)steps : 0.0;
   out->post_kernel_us = steps ? 1000.0 * queue_ms / (double)steps : 0.0;
+  out->window_kernel_us = steps ? 1000.0 * window_ms / (double)steps : 0.0;
   GPQ_TRY(events(&out->events));
   if (w.error) {
     set_error("device error flags " + std::to_string(w.error));

@@ -17,81 +17,120 @@ namespace {
 constexpr int kLaqBlock = 256;   // LPs per post block = threads per block (4 waves)
 constexpr int kSlots = 8;        // ring records held per lane: a group of G lanes covers 8G records
 
-__device__ __forceinline__ uint32_t group_popc(uint64_t ballot, int gbase, int G) {
-  const uint64_t bits = G == kWave ? ballot : (ballot >> gbase) & ((1ull << G) - 1ull);
-  return (uint32_t)__popcll(bits);
+struct Key3 {
+  uint32_t ts, src, seq;
+};
+
+__device__ __forceinline__ Key3 shfl_key(const Event& e, int src_lane) {
+  return Key3{(uint32_t)__shfl((int)e.ts, src_lane, kWave), (uint32_t)__shfl((int)e.src, src_lane, kWave),
+              (uint32_t)__shfl((int)e.seq, src_lane, kWave)};
+}
+
+__device__ __forceinline__ bool key_less3(const Key3& a, const Event& b) {
+  if (a.ts != b.ts) return a.ts < b.ts;
+  if (a.src != b.src) return a.src < b.src;
+  return a.seq < b.seq;
+}
+
+__device__ __forceinline__ bool key_less3(const Key3& a, const Key3& b) {
+  if (a.ts != b.ts) return a.ts < b.ts;
+  if (a.src != b.src) return a.src < b.src;
+  return a.seq < b.seq;
+}
+
+// Orders the LDS histogram updates of the lanes of one wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Merges the k staged events of one LP into its sorted run of n records at ring slot r,
 // using a group of G lanes (lane g of the group, group starting at wave lane gbase).
-// All n records and the k staged events are loaded at once (record j in lane j % G,
-// register slot j / G).  Old record j ends at final index j + c_j, c_j = number of new
-// events with a smaller key; new event m at rank_m + (old records with a smaller key).
-// The run may grow at either end: it starts a slots earlier, a in [0, k], so old record
-// j moves iff c_j != a; a is chosen to move the fewest records.  a = 0 is the
-// reference's backward insertion (src/sort.cu:329-356), a = k its forward variant
+// The run is loaded at once (record j in lane j % G, register slot j / G); staged events
+// are taken G - 1 at a time (one per lane).  Old record j ends at final index j + c_j,
+// c_j = number of new events with a smaller key, and c is nondecreasing in j; the new
+// event of rank a lands at a + #{j : c_j <= a}.  A histogram of c over [0, k] (LDS, one
+// bin per lane) gives both: its inclusive prefix places the new events, and bin a counts
+// the old records that stay put when the run grows by a slots at the front and k - a at
+// the back.  The a with the largest bin is taken, so the fewest records move: a = 0 is
+// the reference's backward insertion (src/sort.cu:329-356), a = k its forward variant
 // (src/sort.cu:358-379), chosen here per LP and step.  Returns the records moved.
 template <int G>
-__device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, uint32_t k, uint32_t n, uint32_t r, int g,
-                                    int gbase) {
+__device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, uint32_t k_total, uint32_t n, uint32_t r, int g,
+                                    int gbase, uint32_t* hist) {
   const uint32_t mask = (1u << q.cap_log2) - 1u;
   Event* ring = q.ring + ((size_t)lp << q.cap_log2);
-  Event e{kInfBits, 0u, 0u, 0u};
-  if (g < (int)k) e = load_event(&q.stage[(size_t)lp * q.stage_cap + g]);
-  Event old[kSlots];
-#pragma unroll
-  for (int s = 0; s < kSlots; ++s) {
-    const uint32_t j = (uint32_t)(s * G + g);
-    old[s] = Event{0u, 0u, 0u, 0u};
-    if (j < n) old[s] = load_event(&ring[(r + j) & mask]);
-  }
-  uint32_t rank = 0;
-  for (uint32_t m = 0; m < k; ++m) {
-    const Event o = shfl_event(e, gbase + (int)m);
-    if (g < (int)k && (key_less(o, e) || (!key_less(e, o) && (int)m < g))) ++rank;
-  }
-  uint32_t c[kSlots];
-#pragma unroll
-  for (int s = 0; s < kSlots; ++s) c[s] = 0;
-  uint32_t greater = 0;   // lane m < k: old records with a key above new event m
-  for (uint32_t m = 0; m < k; ++m) {
-    const Event nm = shfl_event(e, gbase + (int)m);
-    uint32_t cnt = 0;
+  const Event* stg = q.stage + (size_t)lp * q.stage_cap;
+  uint32_t moved = 0;
+  for (uint32_t done = 0; done < k_total;) {
+    if (done) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");   // previous pass's ring writes
+    const uint32_t k = min(k_total - done, (uint32_t)G - 1u);
+    Event e{kInfBits, 0u, 0u, 0u};
+    if (g < (int)k) e = load_event(&stg[done + g]);
+    const uint32_t ns = (n + G - 1) / G;
+    Event old[kSlots];
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
-      const bool lt = (uint32_t)(s * G + g) < n && key_less(nm, old[s]);
-      c[s] += lt ? 1u : 0u;
-      cnt += group_popc(__ballot(lt), gbase, G);
+      const uint32_t j = (uint32_t)(s * G + g);
+      old[s] = Event{0u, 0u, 0u, 0u};
+      if ((uint32_t)s < ns && j < n) old[s] = load_event(&ring[(r + j) & mask]);
     }
-    if (g == (int)m) greater = cnt;
-  }
-  uint32_t best_a = 0, best_stay = 0;
-  for (uint32_t a = 0; a <= k; ++a) {
-    uint32_t stay = 0;
+    // rank of each new event among the new events (keys are unique)
+    uint32_t rank = 0;
+    uint32_t c[kSlots];
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) stay += group_popc(__ballot((uint32_t)(s * G + g) < n && c[s] == a), gbase, G);
-    if (a == 0 || stay > best_stay) {
-      best_stay = stay;
-      best_a = a;
+    for (int s = 0; s < kSlots; ++s) c[s] = 0;
+    for (uint32_t m = 0; m < k; ++m) {
+      const Key3 nm = shfl_key(e, gbase + (int)m);
+      rank += (g < (int)k && key_less3(nm, Key3{e.ts, e.src, e.seq})) ? 1u : 0u;
+#pragma unroll
+      for (int s = 0; s < kSlots; ++s)
+        if ((uint32_t)s < ns) c[s] += ((uint32_t)(s * G + g) < n && key_less3(nm, old[s])) ? 1u : 0u;
     }
-  }
-  const uint32_t r_new = (r - best_a) & mask;
+    // histogram of c over [0, k], k < G: bin b in lane b
+    hist[g] = 0;
+    wave_lds_sync();
 #pragma unroll
-  for (int s = 0; s < kSlots; ++s) {
-    const uint32_t j = (uint32_t)(s * G + g);
-    if (j < n && c[s] != best_a) store_event(&ring[(r_new + j + c[s]) & mask], old[s]);
-  }
-  if (g < (int)k) {
-    const uint32_t f = rank + (n - greater);
-    store_event(&ring[(r_new + f) & mask], e);
-    if (f == 0) q.head[lp] = e.ts;
+    for (int s = 0; s < kSlots; ++s)
+      if ((uint32_t)s < ns && (uint32_t)(s * G + g) < n) atomicAdd(&hist[c[s]], 1u);
+    wave_lds_sync();
+    const uint32_t h = g <= (int)k ? hist[g] : 0u;
+    wave_lds_sync();
+    // a = argmax over bins (smallest a on ties); inclusive prefix of the bins
+    uint32_t best = g <= (int)k ? ((h << 8) | (255u - (uint32_t)g)) : 0u;
+    uint32_t pre = h;
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+      const uint32_t o = (uint32_t)__shfl_xor((int)best, off, kWave);
+      best = o > best ? o : best;
+      const uint32_t t = (uint32_t)__shfl((int)pre, gbase + (g >= off ? g - off : g), kWave);
+      if (g >= off) pre += t;
+    }
+    const uint32_t best_a = 255u - (best & 255u), best_stay = best >> 8;
+    const uint32_t below = (uint32_t)__shfl((int)pre, gbase + (int)(rank < k ? rank : 0u), kWave);
+    const uint32_t r_new = (r - best_a) & mask;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+      const uint32_t j = (uint32_t)(s * G + g);
+      if ((uint32_t)s < ns && j < n && c[s] != best_a) store_event(&ring[(r_new + j + c[s]) & mask], old[s]);
+    }
+    if (g < (int)k) {
+      const uint32_t f = rank + below;
+      store_event(&ring[(r_new + f) & mask], e);
+      if (f == 0) q.head[lp] = e.ts;
+    }
+    moved += n - best_stay;
+    n += k;
+    r = r_new;
+    done += k;
   }
   if (g == 0) {
-    q.rd[lp] = r_new;
-    q.len[lp] = n + k;
+    q.rd[lp] = r;
+    q.len[lp] = n;
     q.ins[lp] = 0;
   }
-  return n - best_stay;
+  return moved;
 }
 
 // One block owns kLaqBlock LPs.  Phase 1 reads their counters coalesced into LDS and
@@ -99,9 +138,10 @@ __device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, uint32_t k, u
 // lanes (kLaqBlock / G LPs in flight per block); phase 3 folds the block's heads into
 // the window reduction.
 template <int G>
-__global__ __launch_bounds__(kLaqBlock) void laq_post_kernel(LaqView q, uint32_t* partial) {
+__global__ __launch_bounds__(kLaqBlock, 5) void laq_post_kernel(LaqView q, uint32_t* partial) {
   __shared__ uint32_t s_work[kLaqBlock];
   __shared__ uint32_t s_k[kLaqBlock], s_n[kLaqBlock], s_r[kLaqBlock];
+  __shared__ uint32_t s_hist[kLaqBlock];
   __shared__ uint32_t s_count;
   __shared__ uint32_t s_inserted;
   __shared__ uint32_t s_moved;
@@ -139,7 +179,7 @@ __global__ __launch_bounds__(kLaqBlock) void laq_post_kernel(LaqView q, uint32_t
   uint32_t moved = 0;
   for (uint32_t i = (uint32_t)group; i < s_count; i += kLaqBlock / G) {
     const uint32_t t = s_work[i];
-    moved += laq_merge_group<G>(q, blockIdx.x * kLaqBlock + t, s_k[t], s_n[t], s_r[t], g, gbase);
+    moved += laq_merge_group<G>(q, blockIdx.x * kLaqBlock + t, s_k[t], s_n[t], s_r[t], g, gbase, s_hist + group * G);
   }
   if (q.stats && g == 0 && moved) atomicAdd(&s_moved, moved);
   __syncthreads();
@@ -187,9 +227,10 @@ gpq_status LaqQueue::init() {
   v_.num_lps = L;
   v_.cap_log2 = log2u(cfg.capacity_per_lp);
   v_.stage_cap = cfg.stage_per_lp;
-  // lanes per LP merge: enough to hold the run (8 records per lane) and one staged event per lane
+  // lanes per LP merge: enough to hold the run (kSlots records per lane); staged events are
+  // merged G - 1 at a time
   group_ = 16;
-  while (group_ < 64 && (group_ * 8u < cfg.capacity_per_lp || group_ < cfg.stage_per_lp)) group_ *= 2;
+  while (group_ < 64 && group_ * (uint32_t)kSlots < cfg.capacity_per_lp) group_ *= 2;
   GPQ_TRY(arena.alloc(&v_.ring, (size_t)L << v_.cap_log2));
   GPQ_TRY(arena.alloc(&v_.stage, (size_t)L * v_.stage_cap));
   GPQ_TRY(arena.alloc(&v_.rd, L));

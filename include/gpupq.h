@@ -183,7 +183,9 @@ typedef struct {
   uint64_t steps;
   uint64_t events;
   double step_kernel_us;    /* mean duration of the model step (handle_next) kernel */
-  double post_kernel_us;    /* mean duration of queue_pre + queue_post kernels */
+  double post_kernel_us;    /* mean duration of queue_pre + queue_post kernels, without the window
+                               publication of the per-LP queues (their post is one kernel) */
+  double window_kernel_us;  /* mean duration of the window publication kernel (per-LP queues; 0 otherwise) */
   double seconds;           /* wall clock of the profiled loop */
 } gpq_kernel_profile;
 gpq_status gpq_des_profile(gpq_phold* m, uint32_t batch, gpq_kernel_profile* out);
@@ -258,6 +260,12 @@ gpq_status gpq_query_synthetic(uint32_t seed, uint32_t q, gpq_query_spec* out);
 gpq_status gpq_astar_batch_create(const gpq_query_spec* specs, uint32_t count, const gpq_batch_tuning* tuning,
                                   gpq_astar_batch** out);
 gpq_status gpq_astar_batch_destroy(gpq_astar_batch* b);
+/* Autotuner: times threads_per_query in {64..1024} x stage_capacity in {8, 16, 32} on the
+ * first `sample` queries (0 = all) and writes the fastest into *tuning (its heap_capacity
+ * and max_resident are kept as given; 0 = defaults).  best_seconds (may be NULL) receives
+ * that candidate's time. */
+gpq_status gpq_astar_batch_autotune(const gpq_query_spec* specs, uint32_t count, uint32_t sample,
+                                    gpq_batch_tuning* tuning, double* best_seconds);
 /* Runs every query: GPQ_DRIVE_GRAPH = one persistent launch (captured once in a hipGraph);
  * GPQ_DRIVE_PLAIN = the one-query-per-launch path: each query alone, one launch per expansion
  * step with a host synchronization per step (the reference's des_run pattern). */

@@ -179,3 +179,17 @@ def test_batch_rejects_bad_specs(gpq):
         gpq.AstarBatch([S(64, 0, 0, 0, 64, 1, 8, 1, 1, 0)])
     with pytest.raises(gpq.GpqError):
         gpq.AstarBatch([S(64, 0, 0, 0, 1, 1, 8, 1, 1, 0)], threads_per_query=96)
+
+
+def test_batch_autotune_keeps_results(gpq):
+    specs = synthetic(gpq, 13, 16)
+    kw, sec = gpq.autotune_batch(specs, sample=8)
+    assert kw["threads_per_query"] in (64, 128, 256, 512, 1024) and kw["stage_capacity"] in (8, 16, 32)
+    assert sec > 0
+    with gpq.AstarBatch(specs, **kw) as b:
+        b.run("graph")
+        c1, _ = b.results()
+    with gpq.AstarBatch(specs, threads_per_query=64) as b:
+        b.run("graph")
+        c2, _ = b.results()
+    assert np.array_equal(c1, c2)
