@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib
 from ._lib import DesReport, Event, GpqError, PholdConfig, QueueConfig, check
 
-__all__ = ["Queue", "Phold", "Astar", "AstarBatch", "autotune_batch", "query_synthetic", "serve_trace", "serve_run", "des_init",
+__all__ = ["Queue", "Phold", "PholdBatch", "Astar", "AstarBatch", "autotune_batch", "query_synthetic", "serve_trace", "serve_run", "des_init",
            "des_run", "des_finish",
            "GpqError", "lib", "event_dtype"]
 
@@ -336,6 +336,53 @@ class AstarBatch:
     def close(self):
         if self.handle:
             check(lib.gpq_astar_batch_destroy(self.handle), "astar_batch_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------------------
+# batched PHOLD
+# ---------------------------------------------------------------------------------------
+class PholdBatch:
+    """Many PHOLD instances of uneven size advanced together in one local array queue;
+    instances = [(num_lps, init_per_lp, seed, end_time), ...] (lookahead 0.1, p_self 0.5)."""
+
+    def __init__(self, instances, capacity_per_lp=128, stage_per_lp=32, lookahead=0.1, p_self=0.5):
+        ps = min(int(round(p_self * 2 ** 32)), 0xFFFFFFFF)
+        self.n = len(instances)
+        self.inst = (_lib.PholdInstance * self.n)(*[_lib.PholdInstance(l, k, s, t, lookahead, ps)
+                                                     for (l, k, s, t) in instances])
+        h = ctypes.c_void_p()
+        check(lib.gpq_phold_batch_create(self.inst, self.n, capacity_per_lp, stage_per_lp, ctypes.byref(h)),
+              "phold_batch_create")
+        self.handle = h
+
+    def run(self, mode="graph", batch=16):
+        sec = ctypes.c_double()
+        check(lib.gpq_phold_batch_run(self.handle, _lib.DRIVE_MODES[mode], batch, ctypes.byref(sec)),
+              "phold_batch_run")
+        return sec.value
+
+    def results(self):
+        h = np.zeros(self.n, dtype=np.uint64)
+        e = np.zeros(self.n, dtype=np.uint64)
+        check(lib.gpq_phold_batch_results(self.handle, _ptr(h), _ptr(e)), "phold_batch_results")
+        return h, e
+
+    def close(self):
+        if self.handle:
+            check(lib.gpq_phold_batch_destroy(self.handle), "phold_batch_destroy")
             self.handle = None
 
     def __enter__(self):

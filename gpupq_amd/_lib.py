@@ -70,6 +70,11 @@ class BatchTuning(ctypes.Structure):
                 ("max_resident", c_u32)]
 
 
+class PholdInstance(ctypes.Structure):
+    _fields_ = [("num_lps", c_u32), ("init_per_lp", c_u32), ("seed", c_u32), ("end_time", c_f32),
+                ("lookahead", c_f32), ("p_self_u32", c_u32)]
+
+
 class ServeConfig(ctypes.Structure):
     _fields_ = [("requests", c_u32), ("seed", c_u32), ("rate", ctypes.c_double), ("phold_fraction", ctypes.c_double),
                 ("hbm_budget", c_u64), ("phold_min_log2", c_u32), ("phold_max_log2", c_u32),
@@ -136,6 +141,10 @@ SIGNATURES = [
     ("gpq_astar_batch_results", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     ("gpq_astar_batch_timing", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     ("gpq_astar_batch_grid", ctypes.c_int, [c_void_p, c_u32, c_void_p]),
+    ("gpq_phold_batch_create", ctypes.c_int, [c_void_p, c_u32, c_u32, c_u32, ctypes.POINTER(c_void_p)]),
+    ("gpq_phold_batch_destroy", ctypes.c_int, [c_void_p]),
+    ("gpq_phold_batch_run", ctypes.c_int, [c_void_p, ctypes.c_int, c_u32, ctypes.POINTER(ctypes.c_double)]),
+    ("gpq_phold_batch_results", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     ("gpq_serve_trace", ctypes.c_int, [ctypes.POINTER(ServeConfig), c_void_p]),
     ("gpq_serve_run", ctypes.c_int, [ctypes.POINTER(ServeConfig), ctypes.c_int, ctypes.POINTER(ServeReport),
                                      c_void_p]),

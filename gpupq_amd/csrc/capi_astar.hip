@@ -1,6 +1,7 @@
 // capi_astar.hip -- the extern "C" boundary of the A* models (include/gpupq.h).
 #include <string>
 
+#include "phold_batch.h"
 #include "serve.h"
 
 using gpq::set_error;
@@ -111,6 +112,34 @@ gpq_status gpq_astar_batch_grid(gpq_astar_batch* b, uint32_t q, uint8_t* blocked
   GPQ_NONNULL(b);
   GPQ_NONNULL(blocked);
   return b->impl->grid_bytes(q, blocked);
+}
+
+gpq_status gpq_phold_batch_create(const gpq_phold_instance* inst, uint32_t count, uint32_t capacity_per_lp,
+                                  uint32_t stage_per_lp, gpq_phold_batch** out) {
+  GPQ_NONNULL(inst);
+  GPQ_NONNULL(out);
+  gpq::PholdBatch* b = nullptr;
+  GPQ_TRY(gpq::PholdBatch::create(inst, count, capacity_per_lp, stage_per_lp, &b));
+  *out = new gpq_phold_batch{b};
+  return GPQ_OK;
+}
+
+gpq_status gpq_phold_batch_destroy(gpq_phold_batch* b) {
+  if (!b) return GPQ_OK;
+  delete b->impl;
+  delete b;
+  return GPQ_OK;
+}
+
+gpq_status gpq_phold_batch_run(gpq_phold_batch* b, gpq_drive_mode mode, uint32_t batch, double* seconds) {
+  GPQ_NONNULL(b);
+  if (mode != GPQ_DRIVE_PLAIN && mode != GPQ_DRIVE_GRAPH) { set_error("bad drive mode"); return GPQ_ERR_INVALID; }
+  return b->impl->run(mode, batch, seconds);
+}
+
+gpq_status gpq_phold_batch_results(gpq_phold_batch* b, uint64_t* trace_hashes, uint64_t* events) {
+  GPQ_NONNULL(b);
+  return b->impl->results(trace_hashes, events);
 }
 
 gpq_status gpq_serve_trace(const gpq_serve_config* cfg, gpq_request* out) {

@@ -1,0 +1,460 @@
+// phold_batch.hip -- the PHOLD pool: many instances in one LAQ, advanced together.
+//
+// Per step, over the whole pool:
+//   pool_step_kernel     every LP of an active instance drains its safe window (the
+//                        instance's own window end), exactly as one instance alone would:
+//                        Philox counters, destinations and the (ts, src, seq) identity use
+//                        the LP's index inside its instance, so each instance's trace hash
+//                        equals the single-instance run's;
+//   LaqQueue::merge_only the LAQ merge of the staged inserts, one partial minimum per block;
+//   pool_publish_kernel  one workgroup per instance folds the partials of its blocks into
+//                        its window (gmin, window end, done).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+
+#include "phold_batch.h"
+
+namespace gpq {
+
+namespace {
+
+constexpr uint32_t kNoInst = 0xFFFFFFFFu;
+constexpr uint32_t kBlk = LaqQueue::kBlockLps;
+
+uint64_t host_mix64_b(uint64_t x) {
+  x ^= x >> 30; x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 27; x *= 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return x;
+}
+
+__global__ __launch_bounds__(256) void pool_step_kernel(LaqView q, const PoolInst* inst, const WindowState* ws,
+                                                        const uint32_t* lp_inst, uint32_t* count, uint64_t* hash,
+                                                        uint32_t capacity, uint32_t* error) {
+  const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lp >= capacity) return;
+  const uint32_t i = lp_inst[lp];
+  if (i == kNoInst) return;
+  const WindowState w = ws[i];
+  if (w.done) return;
+  const uint32_t wend = w.wend;
+  if (!(q.head[lp] < wend)) return;
+  const PoolInst in = inst[i];
+  const uint32_t local = lp - in.base;
+  const uint32_t mask = (1u << q.cap_log2) - 1u;
+  const Event* ring = q.ring + ((size_t)lp << q.cap_log2);
+  uint32_t rd = q.rd[lp], len = q.len[lp];
+  uint32_t n = count[lp];
+  uint64_t h = hash[lp];
+  Event cur = load_event(&ring[rd]);
+  while (len && cur.ts < wend) {
+    const Event nxt = len > 1 ? load_event(&ring[(rd + 1) & mask]) : Event{kInfBits, 0u, 0u, 0u};
+    h = trace_step(h, cur.ts, cur.src, cur.seq);
+    const uint4 r = philox4x32_10(make_uint4(n, local, 0u, 0u), make_uint2(in.seed, 0u));
+    const float ex = exp_from_u32(r.x);
+    const uint32_t dst_local = r.y < in.p_self ? local : (uint32_t)(((uint64_t)r.z * in.lps) >> 32);
+    const float t_new = (bits_to_f(cur.ts) + in.lookahead) + ex;
+    q.insert(Event{f_to_bits(t_new), in.base + dst_local, local, in.init_per_lp + n});
+    ++n;
+    rd = (rd + 1) & mask;
+    --len;
+    cur = nxt;
+  }
+  q.rd[lp] = rd;
+  q.len[lp] = len;
+  q.head[lp] = len ? cur.ts : kInfBits;
+  count[lp] = n;
+  hash[lp] = h;
+  (void)error;
+}
+
+// One workgroup per instance slot: the minimum over its post blocks' partials.
+__global__ void pool_publish_kernel(const uint32_t* partial, const PoolInst* inst, WindowState* ws, uint32_t slots) {
+  const uint32_t i = blockIdx.x;
+  if (i >= slots) return;
+  const PoolInst in = inst[i];
+  if (!in.active || ws[i].done) return;
+  __shared__ uint32_t s_w[4];
+  uint32_t m = kInfBits;
+  for (uint32_t b = in.block_begin + threadIdx.x; b < in.block_end; b += blockDim.x) m = partial[b] < m ? partial[b] : m;
+  m = wave_min_u32(m);
+  if (lane_id() == 0) s_w[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t g = s_w[0];
+    for (uint32_t k = 1; k < blockDim.x / kWave; ++k) g = s_w[k] < g ? s_w[k] : g;
+    float wend = bits_to_f(g) + in.lookahead;
+    if (in.end_time < wend) wend = in.end_time;
+    WindowState w = ws[i];
+    w.gmin = g;
+    w.wend = f_to_bits(wend);
+    w.done = (g == kInfBits || !(bits_to_f(g) < in.end_time)) ? 1u : 0u;
+    w.windows += 1;
+    ws[i] = w;
+  }
+}
+
+// Admission: the instance's LPs get their initial events directly in sorted order (the same
+// ring contents as init_per_lp rounds of populate + queue_post_init), padding LPs of its
+// last block are emptied, and the counters and hashes start at zero.
+__global__ void pool_populate_kernel(LaqView q, PoolInst in, uint32_t slot, uint32_t* lp_inst, uint32_t* count,
+                                     uint64_t* hash, uint32_t padded) {
+  const uint32_t local = blockIdx.x * blockDim.x + threadIdx.x;
+  if (local >= padded) return;
+  const uint32_t lp = in.base + local;
+  Event* ring = q.ring + ((size_t)lp << q.cap_log2);
+  q.rd[lp] = 0;
+  q.ins[lp] = 0;
+  count[lp] = 0;
+  hash[lp] = 0;
+  if (local >= in.lps) {
+    lp_inst[lp] = kNoInst;
+    q.len[lp] = 0;
+    q.head[lp] = kInfBits;
+    return;
+  }
+  lp_inst[lp] = slot;
+  const uint32_t k = in.init_per_lp;
+  for (uint32_t j = 0; j < k; ++j) {
+    const uint4 r = philox4x32_10(make_uint4(j, local, 1u, 0u), make_uint2(in.seed, 0u));
+    Event e{f_to_bits(exp_from_u32(r.x)), lp, local, j};
+    uint32_t p = j;                        // insertion into the sorted prefix
+    while (p > 0) {
+      const Event prev = load_event(&ring[p - 1]);
+      if (!key_less(e, prev)) break;
+      store_event(&ring[p], prev);
+      --p;
+    }
+    store_event(&ring[p], e);
+  }
+  q.len[lp] = k;
+  q.head[lp] = k ? ring[0].ts : kInfBits;
+}
+
+// The first window of a freshly admitted instance: the minimum head over its LPs.
+__global__ void pool_first_window_kernel(const LaqView q, PoolInst in, WindowState* ws, uint32_t slot) {
+  __shared__ uint32_t s_w[16];
+  uint32_t m = kInfBits;
+  for (uint32_t l = threadIdx.x; l < in.lps; l += blockDim.x) {
+    const uint32_t h = q.head[in.base + l];
+    m = h < m ? h : m;
+  }
+  m = wave_min_u32(m);
+  if (lane_id() == 0) s_w[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t g = s_w[0];
+    for (uint32_t k = 1; k < blockDim.x / kWave; ++k) g = s_w[k] < g ? s_w[k] : g;
+    float wend = bits_to_f(g) + in.lookahead;
+    if (in.end_time < wend) wend = in.end_time;
+    ws[slot] = WindowState{g, f_to_bits(wend), (g == kInfBits || !(bits_to_f(g) < in.end_time)) ? 1u : 0u, 0u, 1ull,
+                           0u, 0u};
+  }
+}
+
+__global__ void pool_release_kernel(uint32_t* lp_inst, uint32_t base, uint32_t n, LaqView q) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < n) {
+    lp_inst[base + l] = kNoInst;
+    q.len[base + l] = 0;
+    q.ins[base + l] = 0;
+    q.head[base + l] = kInfBits;
+  }
+}
+
+__global__ void pool_init_kernel(uint32_t* lp_inst, uint32_t n, WindowState* ws, uint32_t slots) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) lp_inst[i] = kNoInst;
+  if (i < slots) ws[i] = WindowState{kInfBits, 0u, 1u, 0u, 0ull, 0u, 0u};
+}
+
+}  // namespace
+
+uint64_t PholdPool::bytes_per_lp(uint32_t capacity_per_lp, uint32_t stage_per_lp) {
+  return (uint64_t)(capacity_per_lp + stage_per_lp) * sizeof(Event) + 4ull * 4 + 4 + 4 + 8 + 4;
+}
+
+gpq_status PholdPool::create(uint32_t capacity_lps, uint32_t capacity_per_lp, uint32_t stage_per_lp,
+                             uint32_t max_instances, PholdPool** out) {
+  GPQ_CHECK_ARG(capacity_lps >= 1 && max_instances >= 1, "pool: empty");
+  const uint32_t cap = (uint32_t)ceil_div(capacity_lps, kBlk) * kBlk;
+  gpq_queue_config qc{};
+  qc.kind = GPQ_LAQ;
+  qc.num_lps = cap;
+  qc.capacity_per_lp = capacity_per_lp;
+  qc.stage_per_lp = stage_per_lp;
+  qc.lookahead = 0.1f;
+  qc.horizon = 1.0f;
+  qc.block_threads = 256;
+  auto p = std::make_unique<PholdPool>();
+  p->q_ = new LaqQueue(qc);
+  GPQ_TRY(p->q_->init_common());
+  GPQ_TRY(p->q_->init());
+  p->capacity_ = cap;
+  p->max_inst_ = max_instances;
+  GPQ_TRY(p->arena_.alloc(&p->d_inst_, max_instances));
+  GPQ_TRY(p->arena_.alloc(&p->d_ws_, max_instances));
+  GPQ_TRY(p->arena_.alloc(&p->d_lp_inst_, cap));
+  GPQ_TRY(p->arena_.alloc(&p->d_count_, cap));
+  GPQ_TRY(p->arena_.alloc(&p->d_hash_, cap));
+  GPQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_ws_), sizeof(WindowState) * max_instances,
+                        hipHostMallocDefault));
+  GPQ_HIP(hipEventCreateWithFlags(&p->done_, hipEventDisableTiming));
+  p->inst_.assign(max_instances, PoolInst{});
+  p->block_used_.assign(cap / kBlk, 0);
+  GPQ_HIP(hipMemcpyAsync(p->d_inst_, p->inst_.data(), sizeof(PoolInst) * max_instances, hipMemcpyHostToDevice,
+                         p->q_->stream));
+  const uint32_t n = std::max(cap, max_instances);
+  hipLaunchKernelGGL(pool_init_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, p->q_->stream, p->d_lp_inst_, cap,
+                     p->d_ws_, max_instances);
+  GPQ_HIP(hipGetLastError());
+  GPQ_HIP(hipMemcpyAsync(p->h_ws_, p->d_ws_, sizeof(WindowState) * max_instances, hipMemcpyDeviceToHost,
+                         p->q_->stream));
+  GPQ_HIP(hipStreamSynchronize(p->q_->stream));
+  *out = p.release();
+  return GPQ_OK;
+}
+
+PholdPool::~PholdPool() {
+  if (q_) (void)hipStreamSynchronize(q_->stream);
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  if (done_) (void)hipEventDestroy(done_);
+  if (h_ws_) (void)hipHostFree(h_ws_);
+  arena_.release();
+  delete q_;
+}
+
+uint32_t PholdPool::free_lps() const {
+  uint32_t f = 0;
+  for (uint8_t u : block_used_) f += u ? 0u : kBlk;
+  return f;
+}
+
+gpq_status PholdPool::admit(const gpq_phold_instance& c, uint32_t* slot, bool* ok) {
+  *ok = false;
+  GPQ_CHECK_ARG(c.num_lps >= 1, "pool: instance without LPs");
+  GPQ_CHECK_ARG(c.init_per_lp <= (1u << q_->view().cap_log2), "pool: init_per_lp exceeds the LP capacity");
+  GPQ_CHECK_ARG(c.lookahead > 0.0f && c.end_time > 0.0f, "pool: lookahead and end_time must be > 0");
+  GPQ_CHECK_ARG(!pending_, "pool: admit while a replay is in flight");
+  uint32_t s = 0;
+  while (s < max_inst_ && inst_[s].active) ++s;
+  if (s == max_inst_) return GPQ_OK;
+  const uint32_t need = (uint32_t)ceil_div(c.num_lps, kBlk);
+  const uint32_t nb = (uint32_t)block_used_.size();
+  uint32_t run = 0, start = 0;
+  bool found = false;
+  for (uint32_t b = 0; b < nb; ++b) {
+    if (block_used_[b]) { run = 0; continue; }
+    if (run == 0) start = b;
+    if (++run == need) { found = true; break; }
+  }
+  if (!found) return GPQ_OK;
+  for (uint32_t b = start; b < start + need; ++b) block_used_[b] = 1;
+  PoolInst in{};
+  in.base = start * kBlk;
+  in.lps = c.num_lps;
+  in.init_per_lp = c.init_per_lp;
+  in.seed = c.seed;
+  in.end_time = c.end_time;
+  in.lookahead = c.lookahead;
+  in.p_self = c.p_self_u32;
+  in.active = 1;
+  in.block_begin = start;
+  in.block_end = start + need;
+  inst_[s] = in;
+  const LaqView v = q_->view();
+  const uint32_t padded = need * kBlk;
+  hipLaunchKernelGGL(pool_populate_kernel, dim3(ceil_div(padded, 256)), dim3(256), 0, q_->stream, v, in, s,
+                     d_lp_inst_, d_count_, d_hash_, padded);
+  hipLaunchKernelGGL(pool_first_window_kernel, dim3(1), dim3(1024), 0, q_->stream, v, in, d_ws_, s);
+  GPQ_HIP(hipGetLastError());
+  GPQ_HIP(hipMemcpyAsync(d_inst_ + s, &inst_[s], sizeof(PoolInst), hipMemcpyHostToDevice, q_->stream));
+  GPQ_HIP(hipMemcpyAsync(h_ws_ + s, d_ws_ + s, sizeof(WindowState), hipMemcpyDeviceToHost, q_->stream));
+  GPQ_HIP(hipStreamSynchronize(q_->stream));
+  *slot = s;
+  *ok = true;
+  return GPQ_OK;
+}
+
+gpq_status PholdPool::prepare(uint32_t batch) {
+  GPQ_CHECK_ARG(batch >= 1, "pool: batch must be >= 1");
+  if (graph_exec_ && batch_ == batch) return GPQ_OK;
+  if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+  if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+  const LaqView v = q_->view();
+  GPQ_HIP(hipStreamBeginCapture(q_->stream, hipStreamCaptureModeThreadLocal));
+  gpq_status st = GPQ_OK;
+  for (uint32_t i = 0; i < batch && st == GPQ_OK; ++i) {
+    hipLaunchKernelGGL(pool_step_kernel, dim3(ceil_div(capacity_, 256)), dim3(256), 0, q_->stream, v, d_inst_, d_ws_,
+                       d_lp_inst_, d_count_, d_hash_, capacity_, &q_->ws->error);
+    st = q_->merge_only();
+    if (st == GPQ_OK)
+      hipLaunchKernelGGL(pool_publish_kernel, dim3(max_inst_), dim3(256), 0, q_->stream, q_->partial, d_inst_, d_ws_,
+                         max_inst_);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(q_->stream, &g);
+  if (st != GPQ_OK) return st;
+  GPQ_HIP(e);
+  graph_ = g;
+  GPQ_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+  batch_ = batch;
+  return GPQ_OK;
+}
+
+gpq_status PholdPool::launch_async() {
+  GPQ_CHECK_ARG(graph_exec_, "pool: launch before prepare");
+  GPQ_HIP(hipGraphLaunch(graph_exec_, q_->stream));
+  GPQ_HIP(hipMemcpyAsync(h_ws_, d_ws_, sizeof(WindowState) * max_inst_, hipMemcpyDeviceToHost, q_->stream));
+  GPQ_HIP(hipMemcpyAsync(q_->host_ws, q_->ws, sizeof(WindowState), hipMemcpyDeviceToHost, q_->stream));
+  GPQ_HIP(hipEventRecord(done_, q_->stream));
+  pending_ = true;
+  return GPQ_OK;
+}
+
+bool PholdPool::replay_done() {
+  if (!pending_) return true;
+  if (hipEventQuery(done_) != hipSuccess) return false;
+  pending_ = false;
+  return true;
+}
+
+void PholdPool::finished(std::vector<uint32_t>* slots) const {
+  slots->clear();
+  for (uint32_t s = 0; s < max_inst_; ++s)
+    if (inst_[s].active && h_ws_[s].done) slots->push_back(s);
+}
+
+bool PholdPool::any_active() const {
+  for (const auto& in : inst_)
+    if (in.active) return true;
+  return false;
+}
+
+uint32_t PholdPool::error_flags_host() const { return q_->host_ws->error; }
+
+gpq_status PholdPool::errors(uint32_t* flags) { return q_->error_flags(flags); }
+
+gpq_status PholdPool::retire(uint32_t s, uint64_t* trace_hash, uint64_t* events) {
+  GPQ_CHECK_ARG(s < max_inst_ && inst_[s].active, "pool: retire of an inactive slot");
+  GPQ_CHECK_ARG(!pending_, "pool: retire while a replay is in flight");
+  const PoolInst in = inst_[s];
+  std::vector<uint32_t> c(in.lps);
+  std::vector<uint64_t> h(in.lps);
+  GPQ_HIP(hipMemcpyAsync(c.data(), d_count_ + in.base, sizeof(uint32_t) * in.lps, hipMemcpyDeviceToHost, q_->stream));
+  GPQ_HIP(hipMemcpyAsync(h.data(), d_hash_ + in.base, sizeof(uint64_t) * in.lps, hipMemcpyDeviceToHost, q_->stream));
+  inst_[s].active = 0;
+  GPQ_HIP(hipMemcpyAsync(d_inst_ + s, &inst_[s], sizeof(PoolInst), hipMemcpyHostToDevice, q_->stream));
+  const uint32_t padded = (in.block_end - in.block_begin) * kBlk;
+  hipLaunchKernelGGL(pool_release_kernel, dim3(ceil_div(padded, 256)), dim3(256), 0, q_->stream, d_lp_inst_, in.base,
+                     padded, q_->view());
+  GPQ_HIP(hipGetLastError());
+  GPQ_HIP(hipStreamSynchronize(q_->stream));
+  for (uint32_t b = in.block_begin; b < in.block_end; ++b) block_used_[b] = 0;
+  uint64_t x = 0x6A09E667F3BCC908ull, ev = 0;
+  for (uint32_t i = 0; i < in.lps; ++i) {
+    x = host_mix64_b(x ^ h[i]);
+    x = host_mix64_b(x ^ (((uint64_t)c[i] << 32) | i));
+    ev += c[i];
+  }
+  if (trace_hash) *trace_hash = x;
+  if (events) *events = ev;
+  return GPQ_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// PholdBatch
+// ---------------------------------------------------------------------------------------
+gpq_status PholdBatch::create(const gpq_phold_instance* in, uint32_t count, uint32_t cap, uint32_t stage,
+                              PholdBatch** out) {
+  GPQ_CHECK_ARG(in && count >= 1, "phold batch: no instances");
+  for (uint32_t i = 0; i < count; ++i) {
+    GPQ_CHECK_ARG(in[i].num_lps >= 1 && in[i].init_per_lp >= 1, "phold batch: empty instance");
+    GPQ_CHECK_ARG(in[i].init_per_lp <= cap, "phold batch: init_per_lp exceeds capacity_per_lp");
+    GPQ_CHECK_ARG(in[i].lookahead > 0.0f && in[i].end_time > 0.0f, "phold batch: lookahead/end_time must be > 0");
+  }
+  auto b = std::make_unique<PholdBatch>();
+  b->inst.assign(in, in + count);
+  b->capacity_per_lp = cap ? cap : 128;
+  b->stage_per_lp = stage ? stage : 32;
+  b->hash.assign(count, 0);
+  b->events.assign(count, 0);
+  *out = b.release();
+  return GPQ_OK;
+}
+
+gpq_status PholdBatch::run(gpq_drive_mode mode, uint32_t batch, double* seconds) {
+  const uint32_t n = (uint32_t)inst.size();
+  double sec = 0.0;
+  if (mode == GPQ_DRIVE_PLAIN) {
+    for (uint32_t i = 0; i < n; ++i) {
+      gpq_queue_config qc{};
+      qc.kind = GPQ_LAQ;
+      qc.num_lps = inst[i].num_lps;
+      qc.capacity_per_lp = capacity_per_lp;
+      qc.stage_per_lp = stage_per_lp;
+      qc.lookahead = inst[i].lookahead;
+      qc.horizon = inst[i].end_time;
+      qc.block_threads = 256;
+      std::unique_ptr<QueueBase> q(make_queue(qc));
+      GPQ_TRY(q->init_common());
+      GPQ_TRY(q->init());
+      gpq_phold_config pc{inst[i].num_lps, inst[i].init_per_lp, inst[i].seed, inst[i].lookahead,
+                          inst[i].end_time, inst[i].p_self_u32, 0u, 1u};
+      PholdModel* raw = nullptr;
+      GPQ_TRY(PholdModel::create(q.get(), pc, &raw));
+      std::unique_ptr<PholdModel> m(raw);
+      GPQ_TRY(m->init());
+      gpq_des_report rep;
+      GPQ_TRY(m->run(GPQ_DRIVE_PLAIN, 1, &rep));
+      sec += rep.sim_seconds;
+      GPQ_TRY(m->trace_hash(&hash[i]));
+      events[i] = rep.events;
+    }
+  } else {
+    uint64_t total = 0;
+    for (const auto& c : inst) total += ceil_div(c.num_lps, kBlk) * (uint64_t)kBlk;
+    GPQ_CHECK_ARG(total < (1ull << 31) / capacity_per_lp, "phold batch: too many LPs for one pool");
+    PholdPool* raw = nullptr;
+    GPQ_TRY(PholdPool::create((uint32_t)total, capacity_per_lp, stage_per_lp, n, &raw));
+    std::unique_ptr<PholdPool> pool(raw);
+    std::vector<uint32_t> slot_of(n), req_of(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      bool ok = false;
+      GPQ_TRY(pool->admit(inst[i], &slot_of[i], &ok));
+      GPQ_CHECK_ARG(ok, "phold batch: pool admission failed");
+      req_of[slot_of[i]] = i;
+    }
+    GPQ_TRY(pool->prepare(batch ? batch : 16));
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<uint32_t> fin;
+    for (;;) {
+      pool->finished(&fin);
+      if (pool->error_flags_host()) {
+        set_error("phold batch: device error flags " + std::to_string(pool->error_flags_host()));
+        return GPQ_ERR_CAPACITY;
+      }
+      for (uint32_t s : fin) GPQ_TRY(pool->retire(s, &hash[req_of[s]], &events[req_of[s]]));
+      if (!pool->any_active()) break;
+      GPQ_TRY(pool->launch_async());
+      while (!pool->replay_done()) {
+      }
+    }
+    sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (seconds) *seconds = sec;
+  return GPQ_OK;
+}
+
+gpq_status PholdBatch::results(uint64_t* hashes, uint64_t* ev) {
+  for (size_t i = 0; i < inst.size(); ++i) {
+    if (hashes) hashes[i] = hash[i];
+    if (ev) ev[i] = events[i];
+  }
+  return GPQ_OK;
+}
+
+}  // namespace gpq

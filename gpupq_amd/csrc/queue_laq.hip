@@ -248,6 +248,12 @@ gpq_status LaqQueue::init() {
 }
 
 gpq_status LaqQueue::post() {
+  GPQ_TRY(merge_only());
+  return publish_window();
+}
+
+gpq_status LaqQueue::merge_only() {
+  static_assert(kLaqBlock == (int)LaqQueue::kBlockLps, "LAQ post block size");
   const dim3 grid(partial_slots), block(kLaqBlock);
   if (group_ == 16)
     hipLaunchKernelGGL(laq_post_kernel<16>, grid, block, 0, stream, v_, partial);
@@ -256,7 +262,7 @@ gpq_status LaqQueue::post() {
   else
     hipLaunchKernelGGL(laq_post_kernel<64>, grid, block, 0, stream, v_, partial);
   GPQ_HIP(hipGetLastError());
-  return publish_window();
+  return GPQ_OK;
 }
 
 gpq_status LaqQueue::pending(uint64_t* n) { return sum_u32(v_.len, cfg.num_lps, sum_, n); }

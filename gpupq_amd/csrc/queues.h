@@ -23,6 +23,10 @@ class LaqQueue final : public QueueBase {
   gpq_status length(uint32_t lp, uint32_t* n) override;
   gpq_status clear_lp(uint32_t lp) override;
   LaqView view() const { return v_; }
+  // The merge kernel of post() alone: one partial minimum per 256-LP block in `partial`,
+  // no window publication (callers that keep several windows publish their own).
+  gpq_status merge_only();
+  static constexpr uint32_t kBlockLps = 256;
 
  private:
   LaqView v_{};

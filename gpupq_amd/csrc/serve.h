@@ -3,6 +3,7 @@
 #pragma once
 
 #include "astar.h"
+#include "phold_batch.h"
 
 namespace gpq {
 

@@ -272,8 +272,26 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
     }
     peak_inflight = 1;
   } else {
-    std::vector<std::unique_ptr<Instance>> running, retired;
-    uint64_t retired_bytes = 0;
+    // PHOLD requests share one pool sized by the HBM budget; a request is admitted when a
+    // free LP range of its size exists, between graph replays of the pool.
+    uint32_t max_lps = 1;
+    uint32_t phold_n = 0;
+    for (const auto& r : req)
+      if (r.kind == 0) { max_lps = std::max(max_lps, r.size); ++phold_n; }
+    std::unique_ptr<PholdPool> pool;
+    if (phold_n) {
+      const uint64_t per_lp = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage);
+      uint64_t cap = c.hbm_budget / per_lp;
+      cap = std::max<uint64_t>(cap, ((uint64_t)max_lps + 255) / 256 * 256);
+      cap = std::min<uint64_t>(cap, ((1ull << 31) / kPholdCapacity) - 256);
+      PholdPool* raw = nullptr;
+      GPQ_TRY(PholdPool::create((uint32_t)cap, kPholdCapacity, kPholdStage, std::min<uint32_t>(phold_n, 1024), &raw));
+      pool.reset(raw);
+      GPQ_TRY(pool->prepare(c.phold_batch));
+    }
+    std::vector<uint32_t> slot_req(pool ? 1024 : 0, 0xFFFFFFFFu);
+    std::vector<uint32_t> fin;
+    const uint64_t per_lp_bytes = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage);
     std::deque<uint32_t> waiting;          // arrived, not admitted
     std::vector<uint32_t> astar_ready;     // admitted A* queries not yet launched
     uint32_t next_arrival = 0;
@@ -281,7 +299,25 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
       bool progress = false;
       const double now = now_ms(t0);
       while (next_arrival < n && req[next_arrival].arrival_ms <= now) waiting.push_back(next_arrival++);
-      // admission in arrival order; A* bypasses a PHOLD request that waits for memory
+      const bool pool_idle = pool && pool->replay_done();
+      // PHOLD: retire finished instances once the pool's replay is complete
+      if (pool_idle) {
+        if (pool->error_flags_host()) {
+          set_error("serve: PHOLD pool overflowed (flags " + std::to_string(pool->error_flags_host()) + ")");
+          return GPQ_ERR_CAPACITY;
+        }
+        pool->finished(&fin);
+        for (uint32_t s : fin) {
+          const uint32_t i = slot_req[s];
+          GPQ_TRY(pool->retire(s, &req[i].result, &req[i].work));
+          req[i].finish_ms = now_ms(t0);
+          inflight_bytes -= req[i].bytes;
+          --inflight;
+          ++completed;
+          progress = true;
+        }
+      }
+      // admission in arrival order; A* bypasses a PHOLD request that waits for LPs
       bool phold_blocked = false;
       for (auto it = waiting.begin(); it != waiting.end();) {
         const uint32_t i = *it;
@@ -293,21 +329,29 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
           progress = true;
           continue;
         }
-        const uint64_t need = phold_bytes(req[i].size);
-        if (phold_blocked || (inflight_bytes + need > c.hbm_budget && inflight > 0)) {
+        if (phold_blocked || !pool_idle) {
+          ++it;
+          continue;
+        }
+        gpq_phold_instance pi{req[i].size, c.phold_init_per_lp, seeds[i], c.phold_end_time, 0.1f, 0x80000000u};
+        uint32_t slot = 0;
+        bool ok = false;
+        GPQ_TRY(pool->admit(pi, &slot, &ok));
+        if (!ok) {
           phold_blocked = true;
           ++it;
           continue;
         }
+        slot_req[slot] = i;
         req[i].start_ms = now_ms(t0);
-        std::unique_ptr<Instance> inst;
-        GPQ_TRY(start_phold(c, req[i], seeds[i], c.phold_batch, &inst));
-        inst->req = i;
-        GPQ_TRY(inst->m->launch_async(inst->ev));
-        inflight_bytes += inst->bytes;
+        req[i].bytes = ((uint64_t)req[i].size + 255) / 256 * 256 * per_lp_bytes;
+        inflight_bytes += req[i].bytes;
         ++inflight;
-        running.push_back(std::move(inst));
         it = waiting.erase(it);
+        progress = true;
+      }
+      if (pool_idle && pool->any_active()) {
+        GPQ_TRY(pool->launch_async());
         progress = true;
       }
       // A* launches on free lanes
@@ -328,35 +372,6 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
       }
       peak_inflight = std::max(peak_inflight, inflight);
       peak_bytes = std::max(peak_bytes, inflight_bytes);
-      // PHOLD instances: next replay or completion
-      for (size_t j = 0; j < running.size();) {
-        Instance& I = *running[j];
-        const hipError_t e = hipEventQuery(I.ev);
-        if (e == hipErrorNotReady) { ++j; continue; }
-        GPQ_HIP(e);
-        const WindowState w = *I.q->host_ws;
-        if (!w.done && !w.error) {
-          GPQ_TRY(I.m->launch_async(I.ev));
-          ++j;
-          continue;
-        }
-        GPQ_TRY(finish_phold(I, &req[I.req]));
-        req[I.req].finish_ms = now_ms(t0);
-        inflight_bytes -= I.bytes;
-        --inflight;
-        ++completed;
-        // hipFree waits for the whole device, so finished instances are released in bulk
-        // once their memory would exceed the budget, not one by one between replays.
-        retired_bytes += I.bytes;
-        retired.push_back(std::move(running[j]));
-        running[j] = std::move(running.back());
-        running.pop_back();
-        if (retired_bytes > c.hbm_budget) {
-          retired.clear();
-          retired_bytes = 0;
-        }
-        progress = true;
-      }
       // A* lanes
       for (auto& L : lanes) {
         if (!L.busy) continue;

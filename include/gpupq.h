@@ -278,6 +278,31 @@ gpq_status gpq_astar_batch_timing(gpq_astar_batch* b, double* start_us, double* 
 /* Obstacle map of query q (side*side bytes). */
 gpq_status gpq_astar_batch_grid(gpq_astar_batch* b, uint32_t q, uint8_t* blocked);
 
+/* ---- Batched PHOLD: many instances of uneven size in one queue ---------------------------
+ * The instances share one local array queue, each owning a range of whole 256-LP blocks and
+ * its own window; one hipGraph replay of (model step, queue merge, per-instance window
+ * publication) advances all of them.  Each instance's trace hash and event count equal
+ * those of the same instance run alone (gpq_phold_* with drain on). */
+typedef struct {
+  uint32_t num_lps;
+  uint32_t init_per_lp;
+  uint32_t seed;
+  float end_time;
+  float lookahead;
+  uint32_t p_self_u32;
+} gpq_phold_instance;
+
+typedef struct gpq_phold_batch gpq_phold_batch;
+
+gpq_status gpq_phold_batch_create(const gpq_phold_instance* inst, uint32_t count, uint32_t capacity_per_lp,
+                                  uint32_t stage_per_lp, gpq_phold_batch** out);
+gpq_status gpq_phold_batch_destroy(gpq_phold_batch* b);
+/* GPQ_DRIVE_GRAPH: all instances in one pool, `batch` steps per graph replay.
+ * GPQ_DRIVE_PLAIN: each instance alone in its own queue, a host synchronization per step.
+ * seconds: wall clock of the step loops (population excluded). */
+gpq_status gpq_phold_batch_run(gpq_phold_batch* b, gpq_drive_mode mode, uint32_t batch, double* seconds);
+gpq_status gpq_phold_batch_results(gpq_phold_batch* b, uint64_t* trace_hashes, uint64_t* events);
+
 /* ---- Serving: a stream of mixed requests (PHOLD simulations and A* queries) --------------
  * Request i of the synthetic trace: Philox4x32-10 of counter {i, 0, 6, 0} under key {seed, 0}
  * gives the gap to the previous arrival (Exp with mean 1/rate; rate 0: all arrive at t = 0),

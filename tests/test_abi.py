@@ -57,7 +57,7 @@ STRUCTS = {
     "gpq_des_report": "DesReport", "gpq_queue_stats": "QueueStats", "gpq_kernel_profile": "KernelProfile",
     "gpq_astar_config": "AstarConfig", "gpq_astar_report": "AstarReport", "gpq_query_spec": "QuerySpec",
     "gpq_batch_tuning": "BatchTuning", "gpq_serve_config": "ServeConfig", "gpq_request": "Request",
-    "gpq_serve_report": "ServeReport",
+    "gpq_serve_report": "ServeReport", "gpq_phold_instance": "PholdInstance",
 }
 
 
