@@ -14,6 +14,8 @@ Beside the JSON line's value:
                 committed in profiles/ (null when absent)
   cpu_baseline  the sequential std::priority_queue oracle on a bounded sample of
                 the same workload (end time 0.5), one host core
+  phold_batch   32 PHOLD instances of uneven size (2^U[12,18] LPs) in one pool, against
+                each alone with a host sync per step
   astar_batch   512 synthetic grid queries (side 2^U[8,11], density U[0.1,0.3],
                 8-connected) in one persistent launch: queries/s, p50/p99 completion
                 time, against the plain one-query-per-launch path (host sync per
@@ -162,6 +164,28 @@ def astar_bench(gpq, log, plain_n=16, threads=256, cpu=True):
     return out
 
 
+def phold_batch_bench(gpq, log, instances=32, seed=5):
+    """Instances of uneven size (LPs 2^U[12,18], 16 events per LP, end time 20) advanced together
+    in one pool vs each alone through its own queue with a host sync per step."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    inst = [(int(round(2 ** rng.uniform(12, 18))), 16, int(rng.integers(1, 2 ** 31)), 20.0) for _ in range(instances)]
+    with gpq.PholdBatch(inst) as b:
+        b.run("graph", batch=16)                  # warm up (graph capture)
+        tg = b.run("graph", batch=16)
+        hg, eg = b.results()
+        tp = b.run("plain")
+        hp, ep = b.results()
+    if list(hg) != list(hp) or list(eg) != list(ep):
+        raise RuntimeError("batched PHOLD: pool and plain results differ")
+    ev = int(eg.sum())
+    out = {"instances": instances, "lps": "2^U[12,18] x 16, end time 20", "total_lps": sum(i[0] for i in inst),
+           "events": ev, "events_per_s": ev / tg, "plain_events_per_s": ev / tp, "batched_vs_plain": tp / tg,
+           "results_batched_vs_plain": "match"}
+    log("phold_batch: %s" % json.dumps(out))
+    return out
+
+
 def serving_bench(gpq, log, requests=64, rate=200.0):
     kw = dict(seed=7, rate=rate, phold_fraction=0.5)
     rep_s, served = gpq.serve_run(requests, mode="graph", **kw)
@@ -215,6 +239,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-astar", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--skip-phold", action="store_true", help="profiling aid: A* and serving parts only")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -242,6 +267,8 @@ def main():
 
     if args.skip_phold:
         line = {"metric": METRIC, "value": None, "unit": "events/s", "n_gpus": 1, "note": "PHOLD part skipped"}
+        if not args.no_batch:
+            line["phold_batch"] = phold_batch_bench(gpq, log)
         if not args.no_astar:
             line["astar_batch"] = astar_bench(gpq, log, cpu=not args.no_cpu)
         if not args.no_serving:
@@ -292,6 +319,8 @@ def main():
                                               for n, v in kernels.items() if n != dom}}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(wl, log)
+    if not args.no_batch:
+        line["phold_batch"] = phold_batch_bench(gpq, log)
     if not args.no_astar:
         line["astar_batch"] = astar_bench(gpq, log, cpu=not args.no_cpu)
     if not args.no_serving:

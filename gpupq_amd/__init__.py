@@ -259,6 +259,14 @@ class Astar:
     def model_handle_next(self):
         check(lib.gpq_astar_handle_next(self.handle), "astar model_handle_next")
 
+    def path(self):
+        """An optimal path of the finished search as (x, y) pairs, source first; [] if unreachable."""
+        n = ctypes.c_uint32()
+        check(lib.gpq_astar_path(self.handle, None, 0, ctypes.byref(n)), "astar path")
+        cells = np.zeros(max(n.value, 1), dtype=np.uint32)
+        check(lib.gpq_astar_path(self.handle, _ptr(cells), n.value, ctypes.byref(n)), "astar path")
+        return [(int(c) % self.width, int(c) // self.width) for c in cells[:n.value]]
+
     def run(self, mode="graph", batch=16):
         rep = _lib.AstarReport()
         check(lib.gpq_astar_run(self.handle, _lib.DRIVE_MODES[mode], batch, ctypes.byref(rep)), "astar run")
@@ -427,3 +435,16 @@ def serve_run(requests, mode="graph", **kw):
     out = (_lib.Request * requests)()
     check(lib.gpq_serve_run(ctypes.byref(cfg), _lib.DRIVE_MODES[mode], ctypes.byref(rep), out), "serve_run")
     return {k: getattr(rep, k) for k, _ in _lib.ServeReport._fields_}, _requests_dict(out)
+
+
+def write_ppm(filename, blocked, width, height, path=()):
+    """Binary PPM (P6) of a grid: obstacles black, free cells white, path cells red (the
+    reference's image dump, src/a_star.cu:596-690)."""
+    g = np.asarray(blocked, dtype=np.uint8).reshape(height, width)
+    img = np.full((height, width, 3), 255, dtype=np.uint8)
+    img[g != 0] = 0
+    for (x, y) in path:
+        img[y, x] = (255, 0, 0)
+    with open(filename, "wb") as f:
+        f.write(b"P6\n%d %d\n255\n" % (width, height))
+        f.write(img.tobytes())

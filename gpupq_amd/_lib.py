@@ -132,6 +132,7 @@ SIGNATURES = [
     ("gpq_astar_handle_next", ctypes.c_int, [c_void_p]),
     ("gpq_astar_run", ctypes.c_int, [c_void_p, ctypes.c_int, c_u32, ctypes.POINTER(AstarReport)]),
     ("gpq_astar_grid", ctypes.c_int, [c_void_p, c_void_p]),
+    ("gpq_astar_path", ctypes.c_int, [c_void_p, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     ("gpq_query_synthetic", ctypes.c_int, [c_u32, c_u32, ctypes.POINTER(QuerySpec)]),
     ("gpq_astar_batch_create", ctypes.c_int, [c_void_p, c_u32, ctypes.POINTER(BatchTuning), ctypes.POINTER(c_void_p)]),
     ("gpq_astar_batch_destroy", ctypes.c_int, [c_void_p]),

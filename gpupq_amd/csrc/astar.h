@@ -55,6 +55,7 @@ class AstarModel {
   gpq_status grid_bytes(uint8_t* out);
   gpq_status step();
   gpq_status run(gpq_drive_mode mode, uint32_t batch, gpq_astar_report* rep);
+  gpq_status path(uint32_t* cells, uint32_t cap, uint32_t* len);
 
   gpq_astar_config cfg{};
   QueueBase* q = nullptr;

@@ -322,4 +322,47 @@ gpq_status AstarModel::run(gpq_drive_mode mode, uint32_t batch, gpq_astar_report
   return GPQ_OK;
 }
 
+// Path of the last search (src/a_star.cu:90-94,417-438 backtrack the closed list): from the
+// target, repeatedly step to a free neighbour whose g plus the edge cost equals the cell's g;
+// the result runs source -> target and its edge costs sum to the optimal cost.
+gpq_status AstarModel::path(uint32_t* cells, uint32_t cap, uint32_t* len) {
+  const uint64_t ncell = (uint64_t)cfg.width * cfg.height;
+  std::vector<uint32_t> gh(ncell), bits((ncell + 31) / 32);
+  GPQ_HIP(hipMemcpyAsync(gh.data(), g, sizeof(uint32_t) * ncell, hipMemcpyDeviceToHost, q->stream));
+  GPQ_HIP(hipMemcpyAsync(bits.data(), blocked, sizeof(uint32_t) * bits.size(), hipMemcpyDeviceToHost, q->stream));
+  AstarState s;
+  GPQ_TRY(poll(&s));
+  *len = 0;
+  if (s.best == 0xFFFFFFFFu) return GPQ_OK;          // unreachable: empty path
+  const uint32_t src = cfg.sy * cfg.width + cfg.sx;
+  uint32_t cur = cfg.ty * cfg.width + cfg.tx;
+  uint32_t gcur = s.best;
+  std::vector<uint32_t> rev{cur};
+  static const int dx[8] = {1, -1, 0, 0, 1, -1, 1, -1}, dy[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+  while (cur != src) {
+    const int x = (int)(cur % cfg.width), y = (int)(cur / cfg.width);
+    bool stepped = false;
+    for (uint32_t k = 0; k < cfg.connectivity && !stepped; ++k) {
+      const int nx = x + dx[k], ny = y + dy[k];
+      if (nx < 0 || ny < 0 || nx >= (int)cfg.width || ny >= (int)cfg.height) continue;
+      const uint32_t v = (uint32_t)ny * cfg.width + (uint32_t)nx;
+      if ((bits[v >> 5] >> (v & 31)) & 1u) continue;
+      const uint32_t w = cfg.connectivity == 4 ? 1u : (k < 4 ? 10u : 14u);
+      if (gh[v] != 0xFFFFFFFFu && gh[v] + w == gcur) {
+        cur = v;
+        gcur = gh[v];
+        rev.push_back(v);
+        stepped = true;
+      }
+    }
+    if (!stepped || rev.size() > ncell) {
+      set_error("A*: no predecessor chain from the target (search not run to the end?)");
+      return GPQ_ERR_STATE;
+    }
+  }
+  *len = (uint32_t)rev.size();
+  for (uint32_t i = 0; i < *len && i < cap; ++i) cells[i] = rev[rev.size() - 1 - i];
+  return GPQ_OK;
+}
+
 }  // namespace gpq

@@ -60,6 +60,13 @@ gpq_status gpq_astar_grid(gpq_astar* a, uint8_t* blocked) {
   return a->impl->grid_bytes(blocked);
 }
 
+gpq_status gpq_astar_path(gpq_astar* a, uint32_t* cells, uint32_t cap, uint32_t* len) {
+  GPQ_NONNULL(a);
+  GPQ_NONNULL(len);
+  if (cap && !cells) { set_error("null cells"); return GPQ_ERR_INVALID; }
+  return a->impl->path(cells, cap, len);
+}
+
 gpq_status gpq_query_synthetic(uint32_t seed, uint32_t q, gpq_query_spec* out) {
   GPQ_NONNULL(out);
   return gpq::query_synthetic(seed, q, out);

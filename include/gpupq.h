@@ -232,6 +232,10 @@ gpq_status gpq_astar_handle_next(gpq_astar* a);
 gpq_status gpq_astar_run(gpq_astar* a, gpq_drive_mode mode, uint32_t batch, gpq_astar_report* rep);
 /* The grid the search ran on (w*h bytes). */
 gpq_status gpq_astar_grid(gpq_astar* a, uint8_t* blocked);
+/* An optimal path of the finished search, source to target, as cell indices y * width + x
+ * (the closed-list backtrack of src/a_star.cu:90-94,417-438).  *len = path length (0 when the
+ * target is unreachable); at most cap cells are written. */
+gpq_status gpq_astar_path(gpq_astar* a, uint32_t* cells, uint32_t cap, uint32_t* len);
 
 /* Batched A*: many independent queries, each searched by one workgroup with per-thread
  * local heaps; one persistent launch pulls queries from a device work counter (the reference's

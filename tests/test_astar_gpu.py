@@ -46,7 +46,21 @@ def test_single_query_matches_oracle(gpq, oracle, kind, conn):
             cost, _ = oracle.astar(ref_grid, w, h, conn, s, t)
             assert rep["cost"] == cost == oracle.dijkstra(ref_grid, w, h, conn, s, t)
             assert rep["error_flags"] == 0 and rep["expanded"] >= 1
+            if cost >= 0:
+                check_path(a.path(), ref_grid, w, conn, s, t, cost)
             a.model_finish()
+
+
+def check_path(path, grid, w, conn, s, t, cost):
+    """A valid path: source to target, free cells, unit steps, edge costs summing to the cost."""
+    assert path[0] == tuple(s) and path[-1] == tuple(t)
+    total = 0
+    for (x0, y0), (x1, y1) in zip(path, path[1:]):
+        assert grid[y1 * w + x1] == 0
+        dx, dy = abs(x1 - x0), abs(y1 - y0)
+        assert max(dx, dy) == 1 and (conn == 8 or dx + dy == 1)
+        total += 1 if conn == 4 else (14 if dx and dy else 10)
+    assert total == cost
 
 
 @pytest.mark.parametrize("conn", [4, 8])
@@ -73,6 +87,7 @@ def test_single_query_enclosed_target(gpq, oracle):
         a.model_init()
         a.set_grid(grid)
         assert a.run("graph", 4)["cost"] == -1 == oracle.dijkstra(grid.reshape(-1), w, h, 8, (0, 0), (21, 11))
+        assert a.path() == []
         a.model_finish()
 
 
@@ -102,6 +117,7 @@ def test_single_query_4096_grid(gpq, oracle):
         assert np.array_equal(grid, ref)
         assert rep["cost"] == oracle.bfs4(ref, w, h, (0, 0), (w - 1, h - 1))
         assert rep["error_flags"] == 0
+        check_path(a.path(), ref, w, 4, (0, 0), (w - 1, h - 1), rep["cost"])
         a.model_finish()
 
 
@@ -193,3 +209,17 @@ def test_batch_autotune_keeps_results(gpq):
         b.run("graph")
         c2, _ = b.results()
     assert np.array_equal(c1, c2)
+
+
+def test_path_image(gpq, tmp_path):
+    w, h = 48, 32
+    with open_lists(gpq, "lhq") as q:
+        a = gpq.Astar(q, w, h, connectivity=8, source=(1, 1), target=(46, 30), seed=4, threshold=0.2)
+        a.model_init()
+        a.run("graph", 8)
+        path = a.path()
+        f = tmp_path / "grid.ppm"
+        gpq.write_ppm(str(f), a.grid(), w, h, path)
+        data = f.read_bytes()
+        assert data.startswith(b"P6\n48 32\n255\n") and len(data) == len(b"P6\n48 32\n255\n") + 3 * w * h
+        a.model_finish()
