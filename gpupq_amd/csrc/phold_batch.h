@@ -53,12 +53,12 @@ class PholdPool {
   uint32_t free_lps() const;
 
  private:
-  gpq_status init_range(uint32_t slot);
   LaqQueue* q_ = nullptr;
   DeviceArena arena_;
   PoolInst* d_inst_ = nullptr;
   WindowState* d_ws_ = nullptr;
   uint32_t* d_lp_inst_ = nullptr;
+  uint8_t* d_block_live_ = nullptr;     // per 256-LP block: 1 while an instance owns it
   uint32_t* d_count_ = nullptr;
   uint64_t* d_hash_ = nullptr;
   WindowState* h_ws_ = nullptr;          // pinned mirror of d_ws_

@@ -33,6 +33,7 @@ uint64_t host_mix64_b(uint64_t x) {
 __global__ __launch_bounds__(256) void pool_step_kernel(LaqView q, const PoolInst* inst, const WindowState* ws,
                                                         const uint32_t* lp_inst, uint32_t* count, uint64_t* hash,
                                                         uint32_t capacity, uint32_t* error) {
+  if (!q.block_live[blockIdx.x]) return;                // 256 threads = one post block of LPs
   const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
   if (lp >= capacity) return;
   const uint32_t i = lp_inst[lp];
@@ -197,6 +198,9 @@ gpq_status PholdPool::create(uint32_t capacity_lps, uint32_t capacity_per_lp, ui
   GPQ_TRY(p->arena_.alloc(&p->d_inst_, max_instances));
   GPQ_TRY(p->arena_.alloc(&p->d_ws_, max_instances));
   GPQ_TRY(p->arena_.alloc(&p->d_lp_inst_, cap));
+  GPQ_TRY(p->arena_.alloc(&p->d_block_live_, cap / kBlk));
+  GPQ_HIP(hipMemsetAsync(p->d_block_live_, 0, cap / kBlk, p->q_->stream));
+  p->q_->set_block_live(p->d_block_live_);
   GPQ_TRY(p->arena_.alloc(&p->d_count_, cap));
   GPQ_TRY(p->arena_.alloc(&p->d_hash_, cap));
   GPQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_ws_), sizeof(WindowState) * max_instances,
@@ -253,6 +257,7 @@ gpq_status PholdPool::admit(const gpq_phold_instance& c, uint32_t* slot, bool* o
   }
   if (!found) return GPQ_OK;
   for (uint32_t b = start; b < start + need; ++b) block_used_[b] = 1;
+  GPQ_HIP(hipMemsetAsync(d_block_live_ + start, 1, need, q_->stream));
   PoolInst in{};
   in.base = start * kBlk;
   in.lps = c.num_lps;
@@ -352,6 +357,7 @@ gpq_status PholdPool::retire(uint32_t s, uint64_t* trace_hash, uint64_t* events)
   hipLaunchKernelGGL(pool_release_kernel, dim3(ceil_div(padded, 256)), dim3(256), 0, q_->stream, d_lp_inst_, in.base,
                      padded, q_->view());
   GPQ_HIP(hipGetLastError());
+  GPQ_HIP(hipMemsetAsync(d_block_live_ + in.block_begin, 0, in.block_end - in.block_begin, q_->stream));
   GPQ_HIP(hipStreamSynchronize(q_->stream));
   for (uint32_t b = in.block_begin; b < in.block_end; ++b) block_used_[b] = 0;
   uint64_t x = 0x6A09E667F3BCC908ull, ev = 0;

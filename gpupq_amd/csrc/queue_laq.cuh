@@ -25,6 +25,7 @@ struct LaqView {
   uint32_t* head;
   WindowState* ws;
   unsigned long long* stats;   // work counters, nullptr when not collected
+  const uint8_t* block_live;    // per 256-LP post block, 0 = holds no LP in use (skipped); nullptr = all live
   uint32_t num_lps;
   uint32_t cap_log2;
   uint32_t stage_cap;

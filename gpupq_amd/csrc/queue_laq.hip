@@ -146,6 +146,10 @@ __global__ __launch_bounds__(kLaqBlock, 5) void laq_post_kernel(LaqView q, uint3
   __shared__ uint32_t s_inserted;
   __shared__ uint32_t s_moved;
   const uint32_t lp = blockIdx.x * kLaqBlock + threadIdx.x;
+  if (q.block_live && !q.block_live[blockIdx.x]) {     // a block of unused LPs: nothing staged, no heads
+    if (threadIdx.x == 0) partial[blockIdx.x] = kInfBits;
+    return;
+  }
   if (threadIdx.x == 0) {
     s_count = 0;
     s_inserted = 0;

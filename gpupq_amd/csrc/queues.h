@@ -27,6 +27,8 @@ class LaqQueue final : public QueueBase {
   // no window publication (callers that keep several windows publish their own).
   gpq_status merge_only();
   static constexpr uint32_t kBlockLps = 256;
+  // Post blocks whose flag is 0 are skipped (their LPs hold nothing); nullptr = all live.
+  void set_block_live(const uint8_t* flags) { v_.block_live = flags; }
 
  private:
   LaqView v_{};

@@ -40,7 +40,7 @@ double now_ms(std::chrono::steady_clock::time_point t0) {
 
 gpq_serve_config with_defaults(const gpq_serve_config& c) {
   gpq_serve_config s = c;
-  if (!s.hbm_budget) s.hbm_budget = 32ull << 30;
+  if (!s.hbm_budget) s.hbm_budget = 4ull << 30;
   if (!s.phold_min_log2) s.phold_min_log2 = 12;
   if (!s.phold_max_log2) s.phold_max_log2 = 18;
   if (!s.phold_init_per_lp) s.phold_init_per_lp = 16;

@@ -318,7 +318,7 @@ typedef struct {
   uint32_t seed;
   double rate;                       /* arrivals per second */
   double phold_fraction;             /* share of PHOLD requests */
-  uint64_t hbm_budget;               /* bytes admitted PHOLD requests may hold at once (0: 32 GiB) */
+  uint64_t hbm_budget;               /* bytes of the PHOLD pool admitted requests share (0: 4 GiB) */
   uint32_t phold_min_log2, phold_max_log2;   /* 0: 12 and 18 */
   uint32_t phold_init_per_lp;        /* 0: 16 */
   float phold_end_time;              /* 0: 20 */
