@@ -187,16 +187,21 @@ def phold_batch_bench(gpq, log, instances=32, seed=5):
 
 
 def serving_bench(gpq, log, requests=64, rate=200.0):
-    kw = dict(seed=7, rate=rate, phold_fraction=0.5)
-    rep_s, served = gpq.serve_run(requests, mode="graph", **kw)
-    rep_p, plain = gpq.serve_run(requests, mode="plain", **kw)
-    if [r["result"] for r in served] != [r["result"] for r in plain]:
+    """The Poisson trace at `rate` (latency under load), the same trace with every request
+    arriving at once (saturated throughput), and the plain one-request-at-a-time path."""
+    kw = dict(seed=7, phold_fraction=0.5)
+    rep_s, served = gpq.serve_run(requests, mode="graph", rate=rate, **kw)
+    rep_x, sat = gpq.serve_run(requests, mode="graph", rate=0.0, **kw)
+    rep_p, plain = gpq.serve_run(requests, mode="plain", rate=0.0, **kw)
+    res = [r["result"] for r in plain]
+    if [r["result"] for r in served] != res or [r["result"] for r in sat] != res:
         raise RuntimeError("serving: served and plain results differ")
-    out = {"requests": requests, "rate_per_s": rate, "phold_lps": "2^U[12,18] x 16, end time 20",
-           "astar": "side 2^U[8,11], 8-connected",
-           "served": {k: rep_s[k] for k in ("requests_per_second", "p50_ms", "p99_ms", "phold_p99_ms",
-                                            "astar_p99_ms", "peak_inflight", "peak_bytes")},
+    keys = ("requests_per_second", "p50_ms", "p99_ms", "phold_p99_ms", "astar_p99_ms", "peak_inflight", "peak_bytes")
+    out = {"requests": requests, "phold_lps": "2^U[12,18] x 16, end time 20", "astar": "side 2^U[8,11], 8-connected",
+           "served_at_rate": dict({"rate_per_s": rate}, **{k: rep_s[k] for k in keys}),
+           "served_saturated": {k: rep_x[k] for k in keys},
            "plain": {k: rep_p[k] for k in ("requests_per_second", "p50_ms", "p99_ms")},
+           "served_vs_plain_throughput": rep_x["requests_per_second"] / rep_p["requests_per_second"],
            "results_served_vs_plain": "match"}
     log("serving: %s" % json.dumps(out))
     return out

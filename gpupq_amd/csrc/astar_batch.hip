@@ -70,6 +70,7 @@ struct BatchDev {
   uint32_t slot_base;                   // workgroup b of the launch uses slot slot_base + b
   uint64_t g_stride;
   uint32_t heap_cap, stage_cap, log2k;
+  uint32_t lds_rows;                    // heap levels held in LDS: entries [0, lds_rows)
 };
 
 __device__ __forceinline__ uint64_t make_key(uint32_t f, uint32_t h, uint32_t cell) {
@@ -91,40 +92,54 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
   return v;
 }
 
-// Binary heap of one thread: entry i at hp[i * K].  Returns the new top.
-__device__ __forceinline__ uint64_t heap_push(uint64_t* hp, uint32_t K, uint32_t& hn, uint64_t key, uint64_t top) {
+// Binary heap of one thread.  Entry i lives in LDS (row i of a slot-major [T][K] array)
+// for i < T -- the levels every push and pop touches -- and in HBM (row i of the slot's
+// [H][K] array) below; HBM rows < T only hold the LDS rows between plain-drive launches.
+struct ThreadHeap {
+  uint64_t* lds;      // lds_rows + tid
+  uint64_t* hbm;      // slot heap + tid
+  uint32_t K, T;
+  __device__ __forceinline__ uint64_t get(uint32_t i) const { return i < T ? lds[i * K] : hbm[(uint64_t)i * K]; }
+  __device__ __forceinline__ void set(uint32_t i, uint64_t v) const {
+    if (i < T) lds[i * K] = v;
+    else hbm[(uint64_t)i * K] = v;
+  }
+};
+
+// Returns the new top.
+__device__ __forceinline__ uint64_t heap_push(const ThreadHeap& hp, uint32_t& hn, uint64_t key, uint64_t top) {
   uint32_t i = hn++;
   while (i > 0) {
     const uint32_t p = (i - 1) >> 1;
-    const uint64_t pv = hp[(uint64_t)p * K];
+    const uint64_t pv = hp.get(p);
     if (pv <= key) break;
-    hp[(uint64_t)i * K] = pv;
+    hp.set(i, pv);
     i = p;
   }
-  hp[(uint64_t)i * K] = key;
+  hp.set(i, key);
   return key < top ? key : top;
 }
 
-__device__ __forceinline__ uint64_t heap_pop(uint64_t* hp, uint32_t K, uint32_t& hn) {
+__device__ __forceinline__ uint64_t heap_pop(const ThreadHeap& hp, uint32_t& hn) {
   --hn;
   if (hn == 0) return kNoKey;
-  const uint64_t last = hp[(uint64_t)hn * K];
+  const uint64_t last = hp.get(hn);
   uint32_t i = 0;
   uint64_t top = last;
   for (;;) {
     uint32_t c = 2 * i + 1;
     if (c >= hn) break;
-    uint64_t cv = hp[(uint64_t)c * K];
+    uint64_t cv = hp.get(c);
     if (c + 1 < hn) {
-      const uint64_t c2 = hp[(uint64_t)(c + 1) * K];
+      const uint64_t c2 = hp.get(c + 1);
       if (c2 < cv) { cv = c2; ++c; }
     }
     if (last <= cv) break;
-    hp[(uint64_t)i * K] = cv;
+    hp.set(i, cv);
     if (i == 0) top = cv;
     i = c;
   }
-  hp[(uint64_t)i * K] = last;
+  hp.set(i, last);
   return top;
 }
 
@@ -139,14 +154,14 @@ struct Shared {
 
 // Searches query `qidx` in `slot`.  max_iters = 0: to the end; otherwise at most that many
 // iterations, keeping the state in the slot (resume continues it).  Returns true when done.
-__device__ bool search_query(const BatchDev& d, Shared& sh, uint32_t* cnt, uint64_t* stage, uint32_t qidx,
-                             uint32_t slot, uint32_t max_iters, bool resume) {
+__device__ bool search_query(const BatchDev& d, Shared& sh, uint32_t* cnt, uint64_t* stage, uint64_t* lds_rows,
+                             uint32_t qidx, uint32_t slot, uint32_t max_iters, bool resume) {
   const uint32_t tid = threadIdx.x, K = blockDim.x, S = d.stage_cap;
   const BatchQuery Q = d.q[qidx];
   const uint32_t W = Q.side, cells = Q.side * Q.side;
   const uint32_t* bits = d.bits + Q.bit_off;
   uint32_t* g = d.g + (uint64_t)slot * d.g_stride;
-  uint64_t* hp = d.heap + (uint64_t)slot * d.heap_cap * K + tid;
+  const ThreadHeap hp{lds_rows + tid, d.heap + (uint64_t)slot * d.heap_cap * K + tid, K, d.lds_rows};
   const uint32_t source = Q.sy * W + Q.sx, target = Q.ty * W + Q.tx;
 
   uint32_t hn;
@@ -171,11 +186,12 @@ __device__ bool search_query(const BatchDev& d, Shared& sh, uint32_t* cnt, uint6
     if (source != target && tid == owner_of(source, d.log2k)) {
       __hip_atomic_store(&g[source], tag << kTagShift, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const uint32_t h0 = grid_heur(Q.conn, (int)Q.sx, (int)Q.sy, (int)Q.tx, (int)Q.ty);
-      top = heap_push(hp, K, hn, make_key(h0, h0, source), top);
+      top = heap_push(hp, hn, make_key(h0, h0, source), top);
     }
   } else {
     hn = d.hn[(uint64_t)slot * K + tid];
-    top = hn ? hp[0] : kNoKey;
+    for (uint32_t i = 0; i < hn && i < hp.T; ++i) hp.lds[i * K] = hp.hbm[(uint64_t)i * K];   // saved LDS rows
+    top = hn ? hp.get(0) : kNoKey;
     tag = d.slot[slot].tag;
     if (tid == 0) {
       sh.best = d.slot[slot].best;
@@ -193,7 +209,7 @@ __device__ bool search_query(const BatchDev& d, Shared& sh, uint32_t* cnt, uint6
     const uint32_t best_now = *(volatile uint32_t*)&sh.best;
     if (hn && key_f(top) < best_now) {
       const uint64_t key = top;
-      top = heap_pop(hp, K, hn);
+      top = heap_pop(hp, hn);
       const uint32_t cell = key_cell(key), f = key_f(key);
       const int x = (int)(cell % W), y = (int)(cell / W);
       const uint32_t g0 = f - grid_heur(Q.conn, x, y, (int)Q.tx, (int)Q.ty);
@@ -218,7 +234,7 @@ __device__ bool search_query(const BatchDev& d, Shared& sh, uint32_t* cnt, uint6
           if (pos < S) {
             stage[o * S + pos] = k2;
           } else if (hn < d.heap_cap) {
-            top = heap_push(hp, K, hn, k2, top);
+            top = heap_push(hp, hn, k2, top);
           } else {
             sh.overflow = 1;
           }
@@ -229,7 +245,7 @@ __device__ bool search_query(const BatchDev& d, Shared& sh, uint32_t* cnt, uint6
     // --- fold the staged inserts into the own list, reduce the smallest key --------------
     const uint32_t mine = cnt[tid] < S ? cnt[tid] : S;
     for (uint32_t i = 0; i < mine; ++i) {
-      if (hn < d.heap_cap) top = heap_push(hp, K, hn, stage[tid * S + i], top);
+      if (hn < d.heap_cap) top = heap_push(hp, hn, stage[tid * S + i], top);
       else sh.overflow = 1;
     }
     cnt[tid] = 0;
@@ -263,24 +279,29 @@ __device__ bool search_query(const BatchDev& d, Shared& sh, uint32_t* cnt, uint6
       d.slot[slot] = SlotState{sh.best, sh.overflow, tag, 0u, tot};
     }
   }
-  if (!finished) d.hn[(uint64_t)slot * K + tid] = hn;
+  if (!finished) {
+    d.hn[(uint64_t)slot * K + tid] = hn;
+    for (uint32_t i = 0; i < hn && i < hp.T; ++i) hp.hbm[(uint64_t)i * K] = hp.lds[i * K];   // save LDS rows
+  }
   __syncthreads();
   return finished;
 }
 
 // Persistent drive: each workgroup owns slot blockIdx.x and pulls queries until none is left.
+// Dynamic LDS: stage [K][S] u64, heap rows [T][K] u64, counts [K] u32.
 __global__ __launch_bounds__(1024) void astar_batch_kernel(BatchDev d) {
   extern __shared__ uint64_t lds[];
   __shared__ Shared sh;
   uint64_t* stage = lds;
-  uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + (size_t)blockDim.x * d.stage_cap);
+  uint64_t* rows = lds + (size_t)blockDim.x * d.stage_cap;
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(rows + (size_t)blockDim.x * d.lds_rows);
   for (;;) {
     if (threadIdx.x == 0) sh.qi = atomicAdd(d.counter, 1u);
     __syncthreads();
     const uint32_t qi = sh.qi;
     __syncthreads();
     if (qi >= d.n) break;
-    search_query(d, sh, cnt, stage, d.ids[qi], d.slot_base + blockIdx.x, 0, false);
+    search_query(d, sh, cnt, stage, rows, d.ids[qi], d.slot_base + blockIdx.x, 0, false);
   }
 }
 
@@ -289,12 +310,24 @@ __global__ __launch_bounds__(1024) void astar_batch_step_kernel(BatchDev d, uint
   extern __shared__ uint64_t lds[];
   __shared__ Shared sh;
   uint64_t* stage = lds;
-  uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + (size_t)blockDim.x * d.stage_cap);
-  const bool fin = search_query(d, sh, cnt, stage, d.ids[0], d.slot_base, 1, resume != 0);
+  uint64_t* rows = lds + (size_t)blockDim.x * d.stage_cap;
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(rows + (size_t)blockDim.x * d.lds_rows);
+  const bool fin = search_query(d, sh, cnt, stage, rows, d.ids[0], d.slot_base, 1, resume != 0);
   if (threadIdx.x == 0) *done = fin ? 1u : 0u;
 }
 
 __global__ void zero_u32_kernel(uint32_t* p) { *p = 0; }
+
+constexpr size_t kLdsBudget = 64 * 1024;
+
+size_t lds_bytes(uint32_t K, uint32_t S, uint32_t rows) { return (size_t)K * (S * 8 + rows * 8 + 4); }
+
+// Heap levels kept in LDS: the largest 2^d - 1 (d <= 5) that fits beside the staging area.
+uint32_t heap_lds_rows(uint32_t K, uint32_t S) {
+  for (uint32_t rows : {31u, 15u, 7u, 3u, 1u})
+    if (lds_bytes(K, S, rows) <= kLdsBudget) return rows;
+  return 0;
+}
 
 __global__ void init_slots_kernel(SlotState* s, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -351,12 +384,12 @@ gpq_status AstarBatch::create(const gpq_query_spec* sp, uint32_t count, const gp
   gpq_batch_tuning tu = t;
   if (!tu.threads_per_query) tu.threads_per_query = 256;
   if (!tu.heap_capacity) tu.heap_capacity = 2048;
-  if (!tu.stage_capacity) tu.stage_capacity = 16;
+  if (!tu.stage_capacity) tu.stage_capacity = 8;
   GPQ_CHECK_ARG(tu.threads_per_query >= 64 && tu.threads_per_query <= 1024 && is_pow2(tu.threads_per_query),
                 "batch: threads_per_query must be a power of two in [64, 1024]");
   GPQ_CHECK_ARG(tu.heap_capacity >= 16 && tu.heap_capacity <= (1u << 20), "batch: heap_capacity out of range");
   GPQ_CHECK_ARG(tu.stage_capacity >= 1 && tu.stage_capacity <= 64, "batch: stage_capacity must be in [1, 64]");
-  GPQ_CHECK_ARG((size_t)tu.threads_per_query * (tu.stage_capacity * 8 + 4) <= 64 * 1024,
+  GPQ_CHECK_ARG((size_t)tu.threads_per_query * (tu.stage_capacity * 8 + 4) <= kLdsBudget,
                 "batch: LDS staging (threads * (8 * stage + 4) bytes) must fit in 64 KiB");
   uint64_t max_cells = 0, words = 0;
   std::vector<BatchQuery> hq(count);
@@ -375,7 +408,8 @@ gpq_status AstarBatch::create(const gpq_query_spec* sp, uint32_t count, const gp
   int dev = 0, cus = 0;
   GPQ_HIP(hipGetDevice(&dev));
   GPQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const size_t lds = (size_t)tu.threads_per_query * (tu.stage_capacity * 8 + 4);
+  const uint32_t rows = heap_lds_rows(tu.threads_per_query, tu.stage_capacity);
+  const size_t lds = lds_bytes(tu.threads_per_query, tu.stage_capacity, rows);
   int per_cu = 0;
   GPQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, astar_batch_kernel, (int)tu.threads_per_query, lds));
   uint32_t resident = tu.max_resident ? tu.max_resident : (uint32_t)std::max(1, cus * std::max(per_cu, 1));
@@ -413,6 +447,7 @@ gpq_status AstarBatch::create(const gpq_query_spec* sp, uint32_t count, const gp
   d->heap_cap = tu.heap_capacity;
   d->stage_cap = tu.stage_capacity;
   d->log2k = log2u(tu.threads_per_query);
+  d->lds_rows = rows;
   d->counter = b->counter;
   std::vector<uint32_t> ids(count);
   for (uint32_t i = 0; i < count; ++i) ids[i] = i;
@@ -457,7 +492,7 @@ gpq_status AstarBatch::launch(const uint32_t* d_ids, uint32_t n, uint32_t slot_b
   d.counter = d_counter;
   d.slot_base = slot_base;
   const uint32_t K = tuning.threads_per_query;
-  const size_t lds = (size_t)K * (tuning.stage_capacity * 8 + 4);
+  const size_t lds = lds_bytes(K, tuning.stage_capacity, d.lds_rows);
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, d_counter);
   hipLaunchKernelGGL(astar_batch_kernel, dim3(std::max<uint32_t>(1, std::min(slots, n))), dim3(K), lds, s,
                      static_cast<BatchDev>(d));
@@ -475,7 +510,7 @@ gpq_status AstarBatch::plain_one(const uint32_t* d_id, uint32_t spec_index, uint
   d.n = 1;
   d.slot_base = slot;
   const uint32_t K = tuning.threads_per_query;
-  const size_t lds = (size_t)K * (tuning.stage_capacity * 8 + 4);
+  const size_t lds = lds_bytes(K, tuning.stage_capacity, d.lds_rows);
   const uint64_t step_cap = 64ull * specs[spec_index].side * specs[spec_index].side + 1024;
   for (uint64_t step = 0;; ++step) {
     if (step > step_cap) {
@@ -534,7 +569,7 @@ gpq_status batch_autotune(const gpq_query_spec* sp, uint32_t count, uint32_t sam
   gpq_batch_tuning pick{};
   for (uint32_t k : kThreads)
     for (uint32_t st : kStage) {
-      if ((size_t)k * (st * 8 + 4) > 64 * 1024) continue;
+      if ((size_t)k * (st * 8 + 4) > kLdsBudget) continue;
       gpq_batch_tuning t = base;
       t.threads_per_query = k;
       t.stage_capacity = st;
