@@ -212,9 +212,11 @@ def traffic_from_profiles(kernel, workload):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
-    d = json.load(open(path))
-    e = d.get(workload, {}).get(kernel)
-    return None if e is None else e.get("bytes_per_launch")
+    d = json.load(open(path)).get(workload, {})
+    for name, e in d.items():
+        if name == kernel or name.startswith(kernel + "<"):
+            return e.get("bytes_per_launch")
+    return None
 
 
 def copy_calibration():

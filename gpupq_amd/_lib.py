@@ -4,7 +4,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgpupq.so")
+LIB_PATH = os.environ.get("GPUPQ_LIB") or os.path.join(HERE, "libgpupq.so")   # GPUPQ_LIB: another build
 
 c_u32 = ctypes.c_uint32
 c_u64 = ctypes.c_uint64
