@@ -49,6 +49,7 @@ QUEUES = {
     "hq": dict(capacity_per_lp=40, stage_per_lp=1),
 }
 DRAIN = False
+FAR = 0            # LAQ two-tier mode: far-store records per LP (0 = one sorted ring per LP)
 
 
 def make(gpq, kind, wl, stats=False, checks=False):
@@ -57,6 +58,8 @@ def make(gpq, kind, wl, stats=False, checks=False):
         args["stage_per_lp"] = 32      # a drained window can send ~3 events to one LP on average
     if kind == "hq":
         args["node_size"] = max(wl["num_lps"], 32)   # one node holds one safe event per LP
+    if kind == "laq" and FAR:
+        args.update(capacity_per_lp=64, far_per_lp=FAR)
     q = gpq.Queue(kind, num_lps=wl["num_lps"], lookahead=0.1, horizon=wl["end_time"], block_threads=256,
                   collect_stats=stats, **args)
     m = gpq.Phold(q, init_per_lp=wl["init_per_lp"], seed=42, end_time=wl["end_time"], checks=checks, drain=DRAIN)
@@ -81,8 +84,9 @@ def roofline(gpq, kind, wl, log):
     L = wl["num_lps"]
     # queue post kernel: sorted insert of k records displacing d (16*(2k+2d)), one head word
     # per LP for the window scan, one partial minimum per 256-LP block
-    post_bytes = ((RECORD * (2 * d["inserted"] + 2 * d["displaced"]) + 4 * d["scanned_lps"] + d["aux_bytes"])
-                  / launches + 4 * ((L + 255) // 256))
+    scan = 12 if FAR else 4          # head (+ far minimum and far count in two-tier mode) per LP
+    post_bytes = ((RECORD * (2 * d["inserted"] + 2 * d["displaced"]) + scan * d["scanned_lps"] + d["aux_bytes"])
+                  / launches + scan * ((L + 255) // 256))
     # model step: one head word per LP, and per committed event the head record (16), the
     # per-LP state read+write (count 8, hash 16, ring indices 16, head 4 + next head 4),
     # the staging counter (4) and the staged record (16)
@@ -164,6 +168,29 @@ def astar_bench(gpq, log, plain_n=16, threads=256, cpu=True):
     return out
 
 
+def per_queue_bench(gpq, wl, log):
+    """The same workload through every queue kind behind the one API (one timed run each)."""
+    global DRAIN
+    out = {}
+    saved = DRAIN
+    for kind, drain in (("laq", False), ("lhq", True), ("lhq", False), ("lsq", True), ("gaq", False), ("hq", False)):
+        DRAIN = drain
+        q, m = make(gpq, kind, wl)
+        q.sync()
+        t0 = time.perf_counter()
+        rep = gpq.des_run(m, mode="graph", batch=64)
+        dt = time.perf_counter() - t0
+        h = m.trace_hash()
+        m.model_finish()
+        q.queue_finish()
+        key = kind + ("_drain" if drain else "_one_event")
+        out[key] = {"events_per_s": rep["events"] / dt, "seconds": dt, "windows": rep["windows"],
+                    "trace_hash": "%016x" % h}
+        log("per_queue %s: %.3e events/s, %d windows, hash %016x" % (key, rep["events"] / dt, rep["windows"], h))
+    DRAIN = saved
+    return out
+
+
 def phold_batch_bench(gpq, log, instances=32, seed=5):
     """Instances of uneven size (LPs 2^U[12,18], 16 events per LP, end time 20) advanced together
     in one pool vs each alone through its own queue with a host sync per step."""
@@ -189,7 +216,7 @@ def phold_batch_bench(gpq, log, instances=32, seed=5):
 def serving_bench(gpq, log, requests=64, rate=200.0):
     """The Poisson trace at `rate` (latency under load), the same trace with every request
     arriving at once (saturated throughput), and the plain one-request-at-a-time path."""
-    kw = dict(seed=7, phold_fraction=0.5)
+    kw = dict(seed=7, phold_fraction=0.5, hbm_budget=8 << 30)
     rep_s, served = gpq.serve_run(requests, mode="graph", rate=rate, **kw)
     rep_x, sat = gpq.serve_run(requests, mode="graph", rate=0.0, **kw)
     rep_p, plain = gpq.serve_run(requests, mode="plain", rate=0.0, **kw)
@@ -242,11 +269,14 @@ def main():
     ap.add_argument("--one-event", action="store_true",
                     help="commit one event per LP per window (the reference's step) instead of draining each "
                          "LP's safe window; the committed trace is the same")
+    ap.add_argument("--far", type=int, default=0,
+                    help="LAQ two-tier mode: far-store records per LP (0 = single sorted ring)")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-astar", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-queues", action="store_true", help="skip the same workload through the other queue kinds")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--skip-phold", action="store_true", help="profiling aid: A* and serving parts only")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -264,8 +294,9 @@ def main():
         print("[bench] " + msg, file=sys.stderr, flush=True)
 
     import gpupq_amd as gpq
-    global DRAIN
+    global DRAIN, FAR
     DRAIN = not args.one_event and args.queue in ("laq", "lhq", "lsq")
+    FAR = args.far
     wl = WORKLOADS[args.workload]
     golden = None
     gpath = os.path.join(ROOT, "tests", "golden", "phold_golden.json")
@@ -312,6 +343,7 @@ def main():
                    "init_per_lp": wl["init_per_lp"], "end_time": wl["end_time"], "lookahead": 0.1,
                    "drive": "hipgraph x%d" % args.batch, "events_per_step": total_events // args.steps,
                    "window": "drain each LP's safe window" if DRAIN else "one event per LP per window",
+                   "laq_far_per_lp": FAR,
                    "trace_hash_vs_cpu": parity},
     }
     if not args.no_roofline and args.queue in ("laq", "lhq", "lsq"):
@@ -326,6 +358,10 @@ def main():
                                               for n, v in kernels.items() if n != dom}}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(wl, log)
+    if not args.no_queues:
+        line["per_queue"] = per_queue_bench(gpq, wl, log)
+        hashes_q = {v["trace_hash"] for v in line["per_queue"].values()}
+        line["per_queue_hash_equal"] = len(hashes_q) == 1 and (golden is None or hashes_q == {golden["trace_hash"]})
     if not args.no_batch:
         line["phold_batch"] = phold_batch_bench(gpq, log)
     if not args.no_astar:

@@ -137,10 +137,10 @@ struct WindowState {
   uint32_t gmin;
   uint32_t wend;
   uint32_t done;
-  uint32_t reserved;
+  uint32_t near_end;         // LAQ two-tier: ts bits of the near horizon (events below it are in the rings)
   uint64_t windows;          // number of windows computed
   uint32_t error;            // GPQ_FLAG_* bits
-  uint32_t pad;
+  uint32_t near_next;        // LAQ two-tier: nonzero = the next post moves the near horizon here
 };
 
 enum : uint32_t {

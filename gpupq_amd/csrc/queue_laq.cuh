@@ -7,6 +7,10 @@
 //   stage [L][S]  Event   this step's inserts for LP lp (unsorted), count ins[lp]
 //   head  [L]     u32     ts bits of the ring head, +inf when empty (read coalesced by peek)
 //   rd, len, ins [L] u32
+// Two-tier mode (far_cap > 0): the ring keeps only events below the near horizon
+// ws->near_end; the others wait unsorted in
+//   far   [L][F]  Event   count nfar[lp], smallest ts bits far_min[lp] (+inf when empty)
+// and move into the ring when the horizon advances.
 // Inserts never touch the ring: they land in the staging slots and are merged by
 // queue_post.  The reference appends after fel_write and insertion-sorts in place
 // (src/local_array_queue.cu:486-513, src/sort.cu:329-356).
@@ -26,6 +30,10 @@ struct LaqView {
   WindowState* ws;
   unsigned long long* stats;   // work counters, nullptr when not collected
   const uint8_t* block_live;    // per 256-LP post block, 0 = holds no LP in use (skipped); nullptr = all live
+  Event* far;                   // two-tier mode only (far_cap > 0)
+  uint32_t* nfar;
+  uint32_t* far_min;
+  uint32_t far_cap;
   uint32_t num_lps;
   uint32_t cap_log2;
   uint32_t stage_cap;
@@ -56,13 +64,19 @@ struct LaqView {
     head[lp] = n ? ring[(lp << cap_log2) | r].ts : kInfBits;
   }
 
-  __device__ __forceinline__ bool is_empty(int lp) const { return len[lp] == 0 && ins[lp] == 0; }
+  __device__ __forceinline__ bool is_empty(int lp) const {
+    return len[lp] == 0 && ins[lp] == 0 && (far_cap == 0 || nfar[lp] == 0);
+  }
   __device__ __forceinline__ int length(int lp) const { return (int)len[lp]; }
   __device__ __forceinline__ void clear(int lp) const {
     const uint32_t mask = (1u << cap_log2) - 1u;
     rd[lp] = (rd[lp] + len[lp]) & mask;
     len[lp] = 0;
     head[lp] = kInfBits;
+    if (far_cap) {
+      nfar[lp] = 0;
+      far_min[lp] = kInfBits;
+    }
   }
   // PHOLD events carry no identity for ==, so this is a plain insert (as in the reference
   // without LOCAL_ARRAY_QUEUE_UPDATE_ELEMENTS, src/local_array_queue.cu:830-846).

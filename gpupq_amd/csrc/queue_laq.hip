@@ -9,13 +9,16 @@
 // waves merge the LPs that received inserts (coalesced reads of the displaced
 // tail, one shifted write per moved record), then the block folds the 256 heads
 // and the last block publishes gmin / window end / done.
+#include <algorithm>
+#include <cmath>
+
 #include "queues.h"
 
 namespace gpq {
 namespace {
 
 constexpr int kLaqBlock = 256;   // LPs per post block = threads per block (4 waves)
-constexpr int kSlots = 8;        // ring records held per lane: a group of G lanes covers 8G records
+constexpr int kSlots = 8;        // default ring records held per lane: a group of G lanes covers 8G records
 
 struct Key3 {
   uint32_t ts, src, seq;
@@ -56,12 +59,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 // the back.  The a with the largest bin is taken, so the fewest records move: a = 0 is
 // the reference's backward insertion (src/sort.cu:329-356), a = k its forward variant
 // (src/sort.cu:358-379), chosen here per LP and step.  Returns the records moved.
-template <int G>
-__device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, uint32_t k_total, uint32_t n, uint32_t r, int g,
-                                    int gbase, uint32_t* hist) {
+template <int G, int SL>
+__device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, const Event* stg, uint32_t k_total, uint32_t n,
+                                    uint32_t r, int g, int gbase, uint32_t* hist) {
   const uint32_t mask = (1u << q.cap_log2) - 1u;
   Event* ring = q.ring + ((size_t)lp << q.cap_log2);
-  const Event* stg = q.stage + (size_t)lp * q.stage_cap;
   uint32_t moved = 0;
   for (uint32_t done = 0; done < k_total;) {
     if (done) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");   // previous pass's ring writes
@@ -69,30 +71,34 @@ __device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, uint32_t k_to
     Event e{kInfBits, 0u, 0u, 0u};
     if (g < (int)k) e = load_event(&stg[done + g]);
     const uint32_t ns = (n + G - 1) / G;
-    Event old[kSlots];
+    // ring records of LP lp all carry dst = lp: only the key words are kept
+    Key3 old[SL];
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
+    for (int s = 0; s < SL; ++s) {
       const uint32_t j = (uint32_t)(s * G + g);
-      old[s] = Event{0u, 0u, 0u, 0u};
-      if ((uint32_t)s < ns && j < n) old[s] = load_event(&ring[(r + j) & mask]);
+      old[s] = Key3{0u, 0u, 0u};
+      if ((uint32_t)s < ns && j < n) {
+        const Event o = load_event(&ring[(r + j) & mask]);
+        old[s] = Key3{o.ts, o.src, o.seq};
+      }
     }
     // rank of each new event among the new events (keys are unique)
     uint32_t rank = 0;
-    uint32_t c[kSlots];
+    uint32_t c[SL];
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) c[s] = 0;
+    for (int s = 0; s < SL; ++s) c[s] = 0;
     for (uint32_t m = 0; m < k; ++m) {
       const Key3 nm = shfl_key(e, gbase + (int)m);
       rank += (g < (int)k && key_less3(nm, Key3{e.ts, e.src, e.seq})) ? 1u : 0u;
 #pragma unroll
-      for (int s = 0; s < kSlots; ++s)
+      for (int s = 0; s < SL; ++s)
         if ((uint32_t)s < ns) c[s] += ((uint32_t)(s * G + g) < n && key_less3(nm, old[s])) ? 1u : 0u;
     }
     // histogram of c over [0, k], k < G: bin b in lane b
     hist[g] = 0;
     wave_lds_sync();
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s)
+    for (int s = 0; s < SL; ++s)
       if ((uint32_t)s < ns && (uint32_t)(s * G + g) < n) atomicAdd(&hist[c[s]], 1u);
     wave_lds_sync();
     const uint32_t h = g <= (int)k ? hist[g] : 0u;
@@ -111,9 +117,10 @@ __device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, uint32_t k_to
     const uint32_t below = (uint32_t)__shfl((int)pre, gbase + (int)(rank < k ? rank : 0u), kWave);
     const uint32_t r_new = (r - best_a) & mask;
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
+    for (int s = 0; s < SL; ++s) {
       const uint32_t j = (uint32_t)(s * G + g);
-      if ((uint32_t)s < ns && j < n && c[s] != best_a) store_event(&ring[(r_new + j + c[s]) & mask], old[s]);
+      if ((uint32_t)s < ns && j < n && c[s] != best_a)
+        store_event(&ring[(r_new + j + c[s]) & mask], Event{old[s].ts, lp, old[s].src, old[s].seq});
     }
     if (g < (int)k) {
       const uint32_t f = rank + below;
@@ -137,7 +144,7 @@ __device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, uint32_t k_to
 // lists the LPs that received inserts; phase 2 merges them, one LP per group of G
 // lanes (kLaqBlock / G LPs in flight per block); phase 3 folds the block's heads into
 // the window reduction.
-template <int G>
+template <int G, int SL>
 __global__ __launch_bounds__(kLaqBlock, 5) void laq_post_kernel(LaqView q, uint32_t* partial) {
   __shared__ uint32_t s_work[kLaqBlock];
   __shared__ uint32_t s_k[kLaqBlock], s_n[kLaqBlock], s_r[kLaqBlock];
@@ -183,7 +190,9 @@ __global__ __launch_bounds__(kLaqBlock, 5) void laq_post_kernel(LaqView q, uint3
   uint32_t moved = 0;
   for (uint32_t i = (uint32_t)group; i < s_count; i += kLaqBlock / G) {
     const uint32_t t = s_work[i];
-    moved += laq_merge_group<G>(q, blockIdx.x * kLaqBlock + t, s_k[t], s_n[t], s_r[t], g, gbase, s_hist + group * G);
+    const uint32_t lpt = blockIdx.x * kLaqBlock + t;
+    moved += laq_merge_group<G, SL>(q, lpt, q.stage + (size_t)lpt * q.stage_cap, s_k[t], s_n[t], s_r[t], g, gbase,
+                                    s_hist + group * G);
   }
   if (q.stats && g == 0 && moved) atomicAdd(&s_moved, moved);
   __syncthreads();
@@ -198,6 +207,205 @@ __global__ __launch_bounds__(kLaqBlock, 5) void laq_post_kernel(LaqView q, uint3
   block_min_to_partial(v, partial);
 }
 
+// Two-tier post.  N = the near horizon this post uses (ws->near_next when the last window
+// reached the old horizon, else ws->near_end).  Per LP with staged events, or with far events
+// when the horizon moves: the candidates -- its far events first (when moving), then its
+// staged events -- split at N; those below N are listed in LDS and merged into the ring, the
+// others are written back to the far store (in place, compacted, then the staged ones after
+// them).  Partials per block: ring-head minimum, far minimum, far count.
+constexpr uint32_t kFarList = 64;          // near events one LP may take in one post (ring merge input)
+
+template <int G, int SL>
+__global__ __launch_bounds__(kLaqBlock) void laq_post_far_kernel(LaqView q, uint32_t* partial) {
+  __shared__ uint32_t s_work[kLaqBlock];
+  __shared__ uint32_t s_k[kLaqBlock], s_n[kLaqBlock], s_r[kLaqBlock], s_f[kLaqBlock];
+  __shared__ uint32_t s_hist[kLaqBlock];
+  __shared__ uint32_t s_count, s_inserted, s_moved;
+  __shared__ unsigned long long s_aux;
+  __shared__ Event s_near[kLaqBlock / G][kFarList];
+  const uint32_t nb = gridDim.x;
+  const uint32_t lp = blockIdx.x * kLaqBlock + threadIdx.x;
+  const uint32_t moving = q.ws->near_next;
+  const uint32_t N = moving ? moving : q.ws->near_end;
+  if (threadIdx.x == 0) {
+    s_count = 0;
+    s_inserted = 0;
+    s_moved = 0;
+    s_aux = 0;
+  }
+  __syncthreads();
+  if (lp < q.num_lps) {
+    const uint32_t staged = q.ins[lp];
+    const uint32_t f = q.nfar[lp];
+    if (q.stats && moving && f) atomicAdd(&s_aux, 32ull * f);   // far records read and written once
+    if (staged || (moving && f)) {
+      const uint32_t slot = atomicAdd(&s_count, 1u);
+      s_work[slot] = threadIdx.x;
+      s_k[threadIdx.x] = staged < q.stage_cap ? staged : q.stage_cap;
+      s_f[threadIdx.x] = f;
+      s_n[threadIdx.x] = q.len[lp];
+      s_r[threadIdx.x] = q.rd[lp];
+      if (q.stats) atomicAdd(&s_inserted, staged);
+    }
+  }
+  __syncthreads();
+  const int lane = lane_id();
+  const int g = lane % G;
+  const int gbase = lane - g;
+  const int group = (int)threadIdx.x / G;
+  const uint64_t gmask = (G == kWave ? ~0ull : ((1ull << G) - 1ull)) << gbase;
+  uint32_t moved = 0;
+  for (uint32_t i = (uint32_t)group; i < s_count; i += kLaqBlock / G) {
+    const uint32_t t = s_work[i];
+    const uint32_t lpt = blockIdx.x * kLaqBlock + t;
+    const uint32_t k = s_k[t], f = s_f[t];
+    const uint32_t cands = (moving ? f : 0u) + k;
+    const uint32_t far_keep0 = moving ? 0u : f;     // far slots kept untouched in front
+    Event* far = q.far + (size_t)lpt * q.far_cap;
+    const Event* stg = q.stage + (size_t)lpt * q.stage_cap;
+    uint32_t n_near = 0, n_far = far_keep0;
+    uint32_t fmin = moving ? kInfBits : q.far_min[lpt];
+    for (uint32_t base = 0; base < cands; base += G) {
+      const uint32_t c = base + (uint32_t)g;
+      Event e{kInfBits, 0u, 0u, 0u};
+      const bool valid = c < cands;
+      if (valid) {
+        const uint32_t nf = moving ? f : 0u;
+        e = load_event(c < nf ? &far[c] : &stg[c - nf]);
+      }
+      const bool near = valid && e.ts < N;
+      const bool farv = valid && !near;
+      const uint64_t bn = __ballot(near) & gmask, bf = __ballot(farv) & gmask;
+      const uint64_t below = (1ull << lane) - 1ull;
+      const uint32_t pn = n_near + (uint32_t)__popcll(bn & below);
+      const uint32_t pf = n_far + (uint32_t)__popcll(bf & below);
+      if (near) {
+        if (pn < kFarList) s_near[group][pn] = e;
+        else atomicOr(&q.ws->error, kErrQueueOverflow);
+      }
+      if (farv) {
+        if (pf < q.far_cap) store_event(&far[pf], e);
+        else atomicOr(&q.ws->error, kErrQueueOverflow);
+        fmin = e.ts < fmin ? e.ts : fmin;
+      }
+      n_near += (uint32_t)__popcll(bn);
+      n_far += (uint32_t)__popcll(bf);
+    }
+    // group minimum of fmin
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+      const uint32_t o = (uint32_t)__shfl_xor((int)fmin, off, kWave);
+      fmin = o < fmin ? o : fmin;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const uint32_t nn = n_near < kFarList ? n_near : kFarList;
+    const uint32_t n = s_n[t];
+    if (n + nn > (1u << q.cap_log2)) {
+      if (g == 0) atomicOr(&q.ws->error, kErrQueueOverflow);
+    } else if (nn) {
+      moved += laq_merge_group<G, SL>(q, lpt, &s_near[group][0], nn, n, s_r[t], g, gbase, s_hist + group * G);
+    }
+    if (g == 0) {
+      q.ins[lpt] = 0;
+      q.nfar[lpt] = n_far < q.far_cap ? n_far : q.far_cap;
+      q.far_min[lpt] = fmin;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (q.stats && g == 0 && moved) atomicAdd(&s_moved, moved);
+  __syncthreads();
+  if (q.stats && threadIdx.x == 0) {
+    const uint32_t lps_here = min((uint32_t)kLaqBlock, q.num_lps - blockIdx.x * kLaqBlock);
+    atomicAdd(&q.stats[0], (unsigned long long)s_inserted);
+    atomicAdd(&q.stats[1], (unsigned long long)s_moved);
+    atomicAdd(&q.stats[2], (unsigned long long)s_count);
+    atomicAdd(&q.stats[3], (unsigned long long)lps_here);
+    if (s_aux) atomicAdd(&q.stats[4], s_aux);
+  }
+  // partials: ring-head minimum, far minimum, far count
+  __shared__ uint32_t s_w[3][kLaqBlock / kWave];
+  uint32_t hv = kInfBits, fv = kInfBits, fc = 0;
+  if (lp < q.num_lps) {
+    hv = q.head[lp];
+    fv = q.far_min[lp];
+    fc = q.nfar[lp];
+  }
+  hv = wave_min_u32(hv);
+  fv = wave_min_u32(fv);
+  fc = wave_sum_u32(fc);
+  if (lane == 0) {
+    s_w[0][threadIdx.x / kWave] = hv;
+    s_w[1][threadIdx.x / kWave] = fv;
+    s_w[2][threadIdx.x / kWave] = fc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t a = s_w[0][0], b = s_w[1][0], cc = s_w[2][0];
+    for (int w = 1; w < kLaqBlock / kWave; ++w) {
+      a = s_w[0][w] < a ? s_w[0][w] : a;
+      b = s_w[1][w] < b ? s_w[1][w] : b;
+      cc += s_w[2][w];
+    }
+    partial[blockIdx.x] = a;
+    partial[nb + blockIdx.x] = b;
+    partial[2 * nb + blockIdx.x] = cc;
+  }
+}
+
+// One block: gmin = min(ring heads, far minima); the window ends at
+// min(gmin + lookahead, horizon, near horizon); a window that reaches the near horizon
+// schedules its move to gmin + lookahead + step for the next post.
+__global__ void laq_far_publish_kernel(const uint32_t* partial, uint32_t nb, WindowState* ws, float lookahead,
+                                       float horizon, float step) {
+  __shared__ uint32_t s_a[16], s_b[16], s_c[16];
+  uint32_t a = kInfBits, b = kInfBits, c = 0;
+  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
+    a = partial[i] < a ? partial[i] : a;
+    b = partial[nb + i] < b ? partial[nb + i] : b;
+    c += partial[2 * nb + i];
+  }
+  a = wave_min_u32(a);
+  b = wave_min_u32(b);
+  c = wave_sum_u32(c);
+  if (lane_id() == 0) {
+    s_a[threadIdx.x / kWave] = a;
+    s_b[threadIdx.x / kWave] = b;
+    s_c[threadIdx.x / kWave] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) {
+      a = s_a[i] < a ? s_a[i] : a;
+      b = s_b[i] < b ? s_b[i] : b;
+      c += s_c[i];
+    }
+    a = s_a[0] < a ? s_a[0] : a;
+    b = s_b[0] < b ? s_b[0] : b;
+    if (ws->near_next) {
+      ws->near_end = ws->near_next;
+      ws->near_next = 0;
+    }
+    const uint32_t g = a < b ? a : b;
+    const float gf = bits_to_f(g);
+    float wend = gf + lookahead;
+    if (horizon < wend) wend = horizon;
+    const float nearf = bits_to_f(ws->near_end);
+    if (nearf < wend) wend = nearf;
+    if (g != kInfBits && gf + lookahead > nearf) {
+      float next = gf + lookahead + step;
+      if (!(next > nearf)) next = nearf + step;
+      ws->near_next = f_to_bits(next);
+    }
+    ws->gmin = g;
+    ws->wend = f_to_bits(wend);
+    ws->done = (g == kInfBits || !(gf < horizon)) ? 1u : 0u;
+    ws->windows += 1;
+    (void)c;
+  }
+}
+
 __global__ void laq_init_kernel(LaqView q) {
   const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
   if (lp < q.num_lps) {
@@ -205,12 +413,17 @@ __global__ void laq_init_kernel(LaqView q) {
     q.len[lp] = 0;
     q.ins[lp] = 0;
     q.head[lp] = kInfBits;
+    if (q.far_cap) {
+      q.nfar[lp] = 0;
+      q.far_min[lp] = kInfBits;
+    }
   }
   if (lp == 0) {
     q.ws->gmin = kInfBits;
     q.ws->wend = 0;
     q.ws->done = 0;
-    q.ws->reserved = 0;
+    q.ws->near_end = 0;
+    q.ws->near_next = 0;
     q.ws->windows = 0;
     q.ws->error = 0;
   }
@@ -233,7 +446,7 @@ gpq_status LaqQueue::init() {
   v_.stage_cap = cfg.stage_per_lp;
   // lanes per LP merge: enough to hold the run (kSlots records per lane); staged events are
   // merged G - 1 at a time
-  group_ = 16;
+  group_ = 8;
   while (group_ < 64 && group_ * (uint32_t)kSlots < cfg.capacity_per_lp) group_ *= 2;
   GPQ_TRY(arena.alloc(&v_.ring, (size_t)L << v_.cap_log2));
   GPQ_TRY(arena.alloc(&v_.stage, (size_t)L * v_.stage_cap));
@@ -245,7 +458,19 @@ gpq_status LaqQueue::init() {
   v_.ws = ws;
   v_.stats = stats;
   partial_slots = ceil_div(L, kLaqBlock);
-  GPQ_TRY(arena.alloc(&partial, partial_slots));
+  if (cfg.far_per_lp) {
+    GPQ_CHECK_ARG(std::isfinite(cfg.lookahead) && cfg.lookahead > 0.0f, "LAQ two-tier: needs a finite lookahead");
+    GPQ_CHECK_ARG(cfg.far_per_lp <= 4096, "LAQ two-tier: far_per_lp must be <= 4096");
+    GPQ_CHECK_ARG(group_ <= 16, "LAQ two-tier: capacity_per_lp must be <= 128");
+    GPQ_CHECK_ARG(cfg.stage_per_lp <= kFarList, "LAQ two-tier: stage_per_lp must be <= 96");
+    GPQ_CHECK_ARG((uint64_t)L * cfg.far_per_lp < (1ull << 32), "LAQ two-tier: num_lps * far_per_lp too large");
+    v_.far_cap = cfg.far_per_lp;
+    far_step_ = cfg.far_step > 0.0f ? cfg.far_step : 4.0f * cfg.lookahead;
+    GPQ_TRY(arena.alloc(&v_.far, (size_t)L * v_.far_cap));
+    GPQ_TRY(arena.alloc(&v_.nfar, L));
+    GPQ_TRY(arena.alloc(&v_.far_min, L));
+  }
+  GPQ_TRY(arena.alloc(&partial, (v_.far_cap ? 3 : 1) * (size_t)partial_slots));
   hipLaunchKernelGGL(laq_init_kernel, dim3(ceil_div(L, 256)), dim3(256), 0, stream, v_);
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
@@ -253,28 +478,54 @@ gpq_status LaqQueue::init() {
 
 gpq_status LaqQueue::post() {
   GPQ_TRY(merge_only());
-  return publish_window();
+  if (!v_.far_cap) return publish_window();
+  if (profile_mark) GPQ_HIP(hipEventRecord(profile_mark, stream));
+  hipLaunchKernelGGL(laq_far_publish_kernel, dim3(1), dim3(1024), 0, stream, partial, partial_slots, ws, cfg.lookahead,
+                     cfg.horizon, far_step_);
+  GPQ_HIP(hipGetLastError());
+  return GPQ_OK;
 }
 
 gpq_status LaqQueue::merge_only() {
   static_assert(kLaqBlock == (int)LaqQueue::kBlockLps, "LAQ post block size");
   const dim3 grid(partial_slots), block(kLaqBlock);
-  if (group_ == 16)
-    hipLaunchKernelGGL(laq_post_kernel<16>, grid, block, 0, stream, v_, partial);
+  if (v_.far_cap) {
+    if (group_ == 8)
+      hipLaunchKernelGGL((laq_post_far_kernel<8, kSlots>), grid, block, 0, stream, v_, partial);
+    else
+      hipLaunchKernelGGL((laq_post_far_kernel<16, kSlots>), grid, block, 0, stream, v_, partial);
+    GPQ_HIP(hipGetLastError());
+    return GPQ_OK;
+  }
+  if (group_ == 8)
+    hipLaunchKernelGGL((laq_post_kernel<8, kSlots>), grid, block, 0, stream, v_, partial);
+  else if (group_ == 16)
+    hipLaunchKernelGGL((laq_post_kernel<16, kSlots>), grid, block, 0, stream, v_, partial);
   else if (group_ == 32)
-    hipLaunchKernelGGL(laq_post_kernel<32>, grid, block, 0, stream, v_, partial);
+    hipLaunchKernelGGL((laq_post_kernel<32, kSlots>), grid, block, 0, stream, v_, partial);
   else
-    hipLaunchKernelGGL(laq_post_kernel<64>, grid, block, 0, stream, v_, partial);
+    hipLaunchKernelGGL((laq_post_kernel<64, kSlots>), grid, block, 0, stream, v_, partial);
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }
 
-gpq_status LaqQueue::pending(uint64_t* n) { return sum_u32(v_.len, cfg.num_lps, sum_, n); }
+gpq_status LaqQueue::pending(uint64_t* n) {
+  GPQ_TRY(sum_u32(v_.len, cfg.num_lps, sum_, n));
+  if (v_.far_cap) {
+    uint64_t f = 0;
+    GPQ_TRY(sum_u32(v_.nfar, cfg.num_lps, sum_, &f));
+    *n += f;
+  }
+  return GPQ_OK;
+}
 
 gpq_status LaqQueue::length(uint32_t lp, uint32_t* n) {
   GPQ_CHECK_ARG(lp < cfg.num_lps, "lp out of range");
+  uint32_t f = 0;
   GPQ_HIP(hipMemcpyAsync(n, v_.len + lp, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  if (v_.far_cap) GPQ_HIP(hipMemcpyAsync(&f, v_.nfar + lp, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   GPQ_HIP(hipStreamSynchronize(stream));
+  *n += f;
   return GPQ_OK;
 }
 
@@ -287,16 +538,30 @@ gpq_status LaqQueue::clear_lp(uint32_t lp) {
 
 gpq_status LaqQueue::lp_contents(uint32_t lp, Event* out, uint32_t cap, uint32_t* n) {
   GPQ_CHECK_ARG(lp < cfg.num_lps, "lp out of range");
-  uint32_t rd = 0, len = 0;
+  uint32_t rd = 0, len = 0, nf = 0;
   const uint32_t C = 1u << v_.cap_log2;
-  std::vector<Event> ring(C);
+  std::vector<Event> ring(C), far(v_.far_cap);
   GPQ_HIP(hipMemcpyAsync(&rd, v_.rd + lp, 4, hipMemcpyDeviceToHost, stream));
   GPQ_HIP(hipMemcpyAsync(&len, v_.len + lp, 4, hipMemcpyDeviceToHost, stream));
   GPQ_HIP(hipMemcpyAsync(ring.data(), v_.ring + ((size_t)lp << v_.cap_log2), sizeof(Event) * C,
                          hipMemcpyDeviceToHost, stream));
+  if (v_.far_cap) {
+    GPQ_HIP(hipMemcpyAsync(&nf, v_.nfar + lp, 4, hipMemcpyDeviceToHost, stream));
+    GPQ_HIP(hipMemcpyAsync(far.data(), v_.far + (size_t)lp * v_.far_cap, sizeof(Event) * v_.far_cap,
+                           hipMemcpyDeviceToHost, stream));
+  }
   GPQ_HIP(hipStreamSynchronize(stream));
-  *n = len;
-  for (uint32_t i = 0; i < len && i < cap; ++i) out[i] = ring[(rd + i) & (C - 1)];
+  std::vector<Event> all;
+  for (uint32_t i = 0; i < len; ++i) all.push_back(ring[(rd + i) & (C - 1)]);
+  std::vector<Event> rest(far.begin(), far.begin() + std::min(nf, v_.far_cap));   // far store: unsorted
+  std::sort(rest.begin(), rest.end(), [](const Event& a, const Event& b) {
+    if (a.ts != b.ts) return a.ts < b.ts;
+    if (a.src != b.src) return a.src < b.src;
+    return a.seq < b.seq;
+  });
+  all.insert(all.end(), rest.begin(), rest.end());
+  *n = (uint32_t)all.size();
+  for (uint32_t i = 0; i < *n && i < cap; ++i) out[i] = all[i];
   return GPQ_OK;
 }
 

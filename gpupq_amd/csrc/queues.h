@@ -33,6 +33,7 @@ class LaqQueue final : public QueueBase {
  private:
   LaqView v_{};
   uint64_t* sum_ = nullptr;
+  float far_step_ = 0.0f;
   uint32_t group_ = 16;   // lanes per LP in the post kernel
 };
 

@@ -63,6 +63,10 @@ typedef struct {
   float horizon;             /* events with ts >= horizon are never dequeued (MAX_TS analogue) */
   uint32_t block_threads;    /* launch width of the per-LP kernels (multiple of 64, <= 1024; 0 = 256) */
   uint32_t collect_stats;    /* 1: post kernels count algorithmic work (gpq_queue_stats_get); costs atomics */
+  uint32_t far_per_lp;       /* LAQ two-tier mode (0 = off): unsorted far store per LP for events at or
+                                beyond the near horizon; the sorted ring keeps only events below it */
+  float far_step;            /* LAQ two-tier: the near horizon moves to (minimum + lookahead + far_step)
+                                when a window reaches it (0 = 4 * lookahead) */
 } gpq_queue_config;
 
 typedef struct gpq_queue gpq_queue;

@@ -13,7 +13,11 @@ QUEUE_ARGS = {
     "lsq": dict(capacity_per_lp=128, stage_per_lp=16),
     "gaq": dict(capacity_per_lp=64, stage_per_lp=2),
     "hq": dict(capacity_per_lp=40, stage_per_lp=1),
+    # the local array queue in two-tier mode: a 64-record sorted ring below the near horizon,
+    # a 128-record unsorted far store above it
+    "laq2": dict(capacity_per_lp=64, stage_per_lp=16, far_per_lp=128),
 }
+KIND = {"laq2": "laq"}
 
 
 def _over(kind, L):
@@ -27,7 +31,7 @@ def run_gpu(gpq, kind, L, k, T, mode="graph", batch=16, checks=True, drain=False
     if drain:
         args["stage_per_lp"] = 32
     args.update(over)
-    q = gpq.Queue(kind, num_lps=L, lookahead=0.1, horizon=T, **args)
+    q = gpq.Queue(KIND.get(kind, kind), num_lps=L, lookahead=0.1, horizon=T, **args)
     m = gpq.Phold(q, init_per_lp=k, seed=42, end_time=T, checks=checks, drain=drain)
     m.model_init()
     assert q.queue_check_phold() == L * k
@@ -60,7 +64,7 @@ def test_device_philox_bit_exact(gpq, oracle):
             assert list(dev[4 * i:4 * i + 4]) == oracle.philox(c, list(key))
 
 
-LOCAL = ["laq", "lhq", "lsq"]
+LOCAL = ["laq", "lhq", "lsq", "laq2"]
 
 
 @pytest.mark.parametrize("drain", [False, True])
