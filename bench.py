@@ -50,6 +50,7 @@ QUEUES = {
 }
 DRAIN = False
 FAR = 0            # LAQ two-tier mode: far-store records per LP (0 = one sorted ring per LP)
+FAR_STEP = 0.0     # LAQ two-tier: near-horizon step (0 = 4 x lookahead)
 
 
 def make(gpq, kind, wl, stats=False, checks=False):
@@ -59,7 +60,7 @@ def make(gpq, kind, wl, stats=False, checks=False):
     if kind == "hq":
         args["node_size"] = max(wl["num_lps"], 32)   # one node holds one safe event per LP
     if kind == "laq" and FAR:
-        args.update(capacity_per_lp=64, far_per_lp=FAR)
+        args.update(capacity_per_lp=64, far_per_lp=FAR, far_step=FAR_STEP)
     q = gpq.Queue(kind, num_lps=wl["num_lps"], lookahead=0.1, horizon=wl["end_time"], block_threads=256,
                   collect_stats=stats, **args)
     m = gpq.Phold(q, init_per_lp=wl["init_per_lp"], seed=42, end_time=wl["end_time"], checks=checks, drain=DRAIN)
@@ -271,6 +272,7 @@ def main():
                          "LP's safe window; the committed trace is the same")
     ap.add_argument("--far", type=int, default=0,
                     help="LAQ two-tier mode: far-store records per LP (0 = single sorted ring)")
+    ap.add_argument("--far-step", type=float, default=0.0, help="LAQ two-tier: near-horizon step (0 = 4 x lookahead)")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-astar", action="store_true")
@@ -297,6 +299,8 @@ def main():
     global DRAIN, FAR
     DRAIN = not args.one_event and args.queue in ("laq", "lhq", "lsq")
     FAR = args.far
+    global FAR_STEP
+    FAR_STEP = args.far_step
     wl = WORKLOADS[args.workload]
     golden = None
     gpath = os.path.join(ROOT, "tests", "golden", "phold_golden.json")

@@ -67,12 +67,9 @@ __device__ __forceinline__ void phold_drain(const Q& q, const PholdDev& m, uint3
 }
 
 // Drain mode on the local array queue: the same commits in the same order as repeated
-// peek / set_done, with the LP's ring cursor, count and hash held in registers.  Records are
-// taken four at a time: four independent loads, the commits in order up to the first record
-// outside the window, then the four staging-slot atomics together and the four stores, so
-// one memory round trip covers four events instead of one.
+// peek / set_done, with the LP's ring cursor, count and hash held in registers and the
+// next record loaded while the current one is processed; written back once.
 __device__ __forceinline__ void phold_drain(const LaqView& q, const PholdDev& m, uint32_t lp) {
-  constexpr uint32_t B = 4;
   const uint32_t wend = q.ws->wend;
   if (!(q.head[lp] < wend)) return;
   const uint32_t mask = (1u << q.cap_log2) - 1u;
@@ -81,54 +78,27 @@ __device__ __forceinline__ void phold_drain(const LaqView& q, const PholdDev& m,
   uint32_t n = m.count[lp];
   uint64_t h = m.hash[lp];
   uint32_t last = m.last_ts ? m.last_ts[lp] : 0u;
-  uint32_t head = kInfBits;
-  for (;;) {
-    Event rec[B];
-#pragma unroll
-    for (uint32_t b = 0; b < B; ++b)
-      rec[b] = b < len ? load_event(&ring[(rd + b) & mask]) : Event{kInfBits, 0u, 0u, 0u};
-    Event out[B];
-    uint32_t nb = 0;
-#pragma unroll
-    for (uint32_t b = 0; b < B; ++b) {
-      if (nb == b && rec[b].ts < wend) {       // rec[b] is inside the window (records beyond len are +inf)
-        const Event cur = rec[b];
-        h = trace_step(h, cur.ts, cur.src, cur.seq);
-        if (m.last_ts) {
-          if (cur.ts < last) atomicOr(&m.ws->error, kErrBadState);
-          last = cur.ts;
-        }
-        const uint4 r = philox4x32_10(make_uint4(n, lp, 0u, 0u), make_uint2(m.seed, 0u));
-        const float ex = exp_from_u32(r.x);
-        const uint32_t dst = r.y < m.p_self ? lp : (uint32_t)(((uint64_t)r.z * m.num_lps) >> 32);
-        const float t_new = (bits_to_f(cur.ts) + m.lookahead) + ex;
-        out[b] = Event{f_to_bits(t_new), dst, lp, m.init_per_lp + n};
-        ++n;
-        nb = b + 1;
-      }
+  Event cur = load_event(&ring[rd]);
+  while (len && cur.ts < wend) {
+    const Event nxt = len > 1 ? load_event(&ring[(rd + 1) & mask]) : Event{kInfBits, 0u, 0u, 0u};
+    h = trace_step(h, cur.ts, cur.src, cur.seq);
+    if (m.last_ts) {
+      if (cur.ts < last) atomicOr(&m.ws->error, kErrBadState);
+      last = cur.ts;
     }
-    uint32_t pos[B];
-#pragma unroll
-    for (uint32_t b = 0; b < B; ++b)
-      if (b < nb) pos[b] = atomicAdd(&q.ins[out[b].dst], 1u);
-#pragma unroll
-    for (uint32_t b = 0; b < B; ++b) {
-      if (b < nb) {
-        if (pos[b] < q.stage_cap) store_event(&q.stage[(size_t)out[b].dst * q.stage_cap + pos[b]], out[b]);
-        else atomicOr(&q.ws->error, kErrStageOverflow);
-      }
-    }
-    rd = (rd + nb) & mask;
-    len -= nb;
-    if (nb < B) {
-      head = len ? rec[nb].ts : kInfBits;      // the first record outside the window
-      break;
-    }
-    if (!len) break;
+    const uint4 r = philox4x32_10(make_uint4(n, lp, 0u, 0u), make_uint2(m.seed, 0u));
+    const float ex = exp_from_u32(r.x);
+    const uint32_t dst = r.y < m.p_self ? lp : (uint32_t)(((uint64_t)r.z * m.num_lps) >> 32);
+    const float t_new = (bits_to_f(cur.ts) + m.lookahead) + ex;
+    q.insert(Event{f_to_bits(t_new), dst, lp, m.init_per_lp + n});
+    ++n;
+    rd = (rd + 1) & mask;
+    --len;
+    cur = nxt;
   }
   q.rd[lp] = rd;
   q.len[lp] = len;
-  q.head[lp] = head;
+  q.head[lp] = len ? cur.ts : kInfBits;
   m.count[lp] = n;
   m.hash[lp] = h;
   if (m.last_ts) m.last_ts[lp] = last;
