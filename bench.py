@@ -270,7 +270,7 @@ def main():
     ap.add_argument("--one-event", action="store_true",
                     help="commit one event per LP per window (the reference's step) instead of draining each "
                          "LP's safe window; the committed trace is the same")
-    ap.add_argument("--far", type=int, default=0,
+    ap.add_argument("--far", type=int, default=128,
                     help="LAQ two-tier mode: far-store records per LP (0 = single sorted ring)")
     ap.add_argument("--far-step", type=float, default=0.0, help="LAQ two-tier: near-horizon step (0 = 4 x lookahead)")
     ap.add_argument("--no-roofline", action="store_true")
@@ -353,7 +353,9 @@ def main():
     if not args.no_roofline and args.queue in ("laq", "lhq", "lsq"):
         dom, kernels, counters = roofline(gpq, args.queue, wl, log)
         k = kernels[dom]
-        kname = "laq_post_kernel" if args.queue == "laq" and dom == "queue_post" else dom
+        kname = dom
+        if args.queue == "laq" and dom == "queue_post":
+            kname = "laq_post_far_kernel" if FAR else "laq_post_kernel"
         line["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS,
                             "unit": "GB/s", "frac": k["gbps"] / HBM_PEAK_GBPS,
                             "traffic": traffic_from_profiles(kname, args.workload),
