@@ -126,14 +126,19 @@ def pct(v, p):
     return v[k]
 
 
-def astar_bench(gpq, log, plain_n=16, threads=256, cpu=True):
+def astar_bench(gpq, log, plain_n=16, cpu=True):
     specs = [gpq.query_synthetic(ASTAR_SEED, q) for q in range(ASTAR_QUERIES)]
-    with gpq.AstarBatch(specs, threads_per_query=threads) as b:
+    t0 = time.perf_counter()
+    tuning, tuned_s = gpq.autotune_batch(specs, sample=64)      # threads per query x staging, on 64 queries
+    tune_wall = time.perf_counter() - t0
+    log("astar autotune: %s (%.1f ms on the sample, %.2f s to tune)" % (tuning, 1e3 * tuned_s, tune_wall))
+    threads = tuning["threads_per_query"]
+    with gpq.AstarBatch(specs, **tuning) as b:
         b.run("graph")                                  # capture + warm up
         t = b.run("graph")
         start, finish = b.timing()
         costs, expanded = b.results()
-    with gpq.AstarBatch(specs[:plain_n], threads_per_query=threads) as b2:
+    with gpq.AstarBatch(specs[:plain_n], **tuning) as b2:
         tp = b2.run("plain")
         cp, _ = b2.results()
         b2.run("graph")
@@ -141,6 +146,7 @@ def astar_bench(gpq, log, plain_n=16, threads=256, cpu=True):
     if list(cp) != list(costs[:plain_n]):
         raise RuntimeError("A*: plain and batched costs differ")
     out = {"queries": ASTAR_QUERIES, "seed": ASTAR_SEED, "threads_per_query": threads,
+           "tuning": dict(tuning, autotune_seconds=tune_wall),
            "queries_per_s": ASTAR_QUERIES / t, "seconds": t,
            "expanded_per_s": float(expanded.sum()) / t,
            "p50_completion_ms": pct(finish, 50) / 1000.0, "p99_completion_ms": pct(finish, 99) / 1000.0,
