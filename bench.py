@@ -16,6 +16,7 @@ Beside the JSON line's value:
                 the same workload (end time 0.5), one host core
   phold_batch   32 PHOLD instances of uneven size (2^U[12,18] LPs) in one pool, against
                 each alone with a host sync per step
+  astar_single  the 4096 x 4096 grid query over 65536 open lists (a local heap queue)
   astar_batch   512 synthetic grid queries (side 2^U[8,11], density U[0.1,0.3],
                 8-connected) in one persistent launch: queries/s, p50/p99 completion
                 time, against the plain one-query-per-launch path (host sync per
@@ -204,7 +205,8 @@ def phold_batch_bench(gpq, log, instances=32, seed=5):
     import numpy as np
     rng = np.random.default_rng(seed)
     inst = [(int(round(2 ** rng.uniform(12, 18))), 16, int(rng.integers(1, 2 ** 31)), 20.0) for _ in range(instances)]
-    with gpq.PholdBatch(inst) as b:
+    kw = dict(capacity_per_lp=64, far_per_lp=FAR) if FAR else {}
+    with gpq.PholdBatch(inst, **kw) as b:
         b.run("graph", batch=16)                  # warm up (graph capture)
         tg = b.run("graph", batch=16)
         hg, eg = b.results()
@@ -214,9 +216,41 @@ def phold_batch_bench(gpq, log, instances=32, seed=5):
         raise RuntimeError("batched PHOLD: pool and plain results differ")
     ev = int(eg.sum())
     out = {"instances": instances, "lps": "2^U[12,18] x 16, end time 20", "total_lps": sum(i[0] for i in inst),
+           "laq_far_per_lp": FAR,
            "events": ev, "events_per_s": ev / tg, "plain_events_per_s": ev / tp, "batched_vs_plain": tp / tg,
            "results_batched_vs_plain": "match"}
     log("phold_batch: %s" % json.dumps(out))
+    return out
+
+
+def astar_single_bench(gpq, log, cpu=True, lists=1 << 16, window=0):
+    """One 4096 x 4096, 4-connected, Bernoulli(0.2) seed 7 query, (0,0) -> (4095,4095), its open
+    lists the LPs of a local heap queue; against the oracle's A* on one core."""
+    w = h = 4096
+    inf = float("inf")
+    q = gpq.Queue("lhq", num_lps=lists, capacity_per_lp=64, stage_per_lp=16, lookahead=inf, horizon=inf)
+    a = gpq.Astar(q, w, h, connectivity=4, seed=7, threshold=0.2, window=window)
+    a.model_init()
+    a.run("graph", 32)                                   # warm up (graph capture)
+    a.model_init()
+    rep = a.run("graph", 32)
+    grid = a.grid() if cpu else None
+    a.model_finish()
+    q.queue_finish()
+    out = {"grid": "4096x4096, 4-connected, Bernoulli(0.2), seed 7", "open_lists": lists, "queue": "lhq",
+           "cost": rep["cost"], "seconds": rep["seconds"], "steps": rep["steps"], "expanded": rep["expanded"],
+           "expanded_per_s": rep["expanded"] / rep["seconds"] if rep["seconds"] > 0 else 0.0}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import Oracle
+        o = Oracle()
+        t0 = time.perf_counter()
+        c, e = o.astar(grid, w, h, 4, (0, 0), (w - 1, h - 1))
+        dt = time.perf_counter() - t0
+        if c != rep["cost"]:
+            raise RuntimeError("A* single query: cost %d, oracle %d" % (rep["cost"], c))
+        out.update(cpu_seconds=dt, cpu_expanded=e, cost_vs_cpu="match", speedup_vs_cpu=dt / rep["seconds"])
+    log("astar_single: %s" % json.dumps(out))
     return out
 
 
@@ -378,6 +412,7 @@ def main():
         line["phold_batch"] = phold_batch_bench(gpq, log)
     if not args.no_astar:
         line["astar_batch"] = astar_bench(gpq, log, cpu=not args.no_cpu)
+        line["astar_single"] = astar_single_bench(gpq, log, cpu=not args.no_cpu)
     if not args.no_serving:
         line["serving"] = serving_bench(gpq, log, rate=args.serve_rate)
     print(json.dumps(line), flush=True)

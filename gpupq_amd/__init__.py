@@ -367,14 +367,14 @@ class PholdBatch:
     """Many PHOLD instances of uneven size advanced together in one local array queue;
     instances = [(num_lps, init_per_lp, seed, end_time), ...] (lookahead 0.1, p_self 0.5)."""
 
-    def __init__(self, instances, capacity_per_lp=128, stage_per_lp=32, lookahead=0.1, p_self=0.5):
+    def __init__(self, instances, capacity_per_lp=128, stage_per_lp=32, lookahead=0.1, p_self=0.5, far_per_lp=0):
         ps = min(int(round(p_self * 2 ** 32)), 0xFFFFFFFF)
         self.n = len(instances)
         self.inst = (_lib.PholdInstance * self.n)(*[_lib.PholdInstance(l, k, s, t, lookahead, ps)
                                                      for (l, k, s, t) in instances])
         h = ctypes.c_void_p()
-        check(lib.gpq_phold_batch_create(self.inst, self.n, capacity_per_lp, stage_per_lp, ctypes.byref(h)),
-              "phold_batch_create")
+        check(lib.gpq_phold_batch_create(self.inst, self.n, capacity_per_lp, stage_per_lp, far_per_lp,
+                                         ctypes.byref(h)), "phold_batch_create")
         self.handle = h
 
     def run(self, mode="graph", batch=16):

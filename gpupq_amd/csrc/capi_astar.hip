@@ -122,11 +122,11 @@ gpq_status gpq_astar_batch_grid(gpq_astar_batch* b, uint32_t q, uint8_t* blocked
 }
 
 gpq_status gpq_phold_batch_create(const gpq_phold_instance* inst, uint32_t count, uint32_t capacity_per_lp,
-                                  uint32_t stage_per_lp, gpq_phold_batch** out) {
+                                  uint32_t stage_per_lp, uint32_t far_per_lp, gpq_phold_batch** out) {
   GPQ_NONNULL(inst);
   GPQ_NONNULL(out);
   gpq::PholdBatch* b = nullptr;
-  GPQ_TRY(gpq::PholdBatch::create(inst, count, capacity_per_lp, stage_per_lp, &b));
+  GPQ_TRY(gpq::PholdBatch::create(inst, count, capacity_per_lp, stage_per_lp, far_per_lp, &b));
   *out = new gpq_phold_batch{b};
   return GPQ_OK;
 }

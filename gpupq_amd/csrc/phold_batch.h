@@ -25,12 +25,13 @@ struct PoolInst {
 class PholdPool {
  public:
   // capacity_lps: LPs the pool can hold at once (rounded up to whole 256-LP blocks).
+  // far_per_lp > 0: two-tier LAQ, each instance with its own near horizon.
   static gpq_status create(uint32_t capacity_lps, uint32_t capacity_per_lp, uint32_t stage_per_lp,
-                           uint32_t max_instances, PholdPool** out);
+                           uint32_t max_instances, PholdPool** out, uint32_t far_per_lp = 0);
   ~PholdPool();
 
   // Bytes of HBM one LP of the pool holds (ring, staging, queue words, model state).
-  static uint64_t bytes_per_lp(uint32_t capacity_per_lp, uint32_t stage_per_lp);
+  static uint64_t bytes_per_lp(uint32_t capacity_per_lp, uint32_t stage_per_lp, uint32_t far_per_lp = 0);
 
   // Admits an instance into a free LP range: false in *ok when none is large enough.
   gpq_status admit(const gpq_phold_instance& inst, uint32_t* slot, bool* ok);
@@ -59,6 +60,8 @@ class PholdPool {
   WindowState* d_ws_ = nullptr;
   uint32_t* d_lp_inst_ = nullptr;
   uint8_t* d_block_live_ = nullptr;     // per 256-LP block: 1 while an instance owns it
+  uint32_t* d_block_win_ = nullptr;     // two-tier: per block, the owning instance slot
+  bool far_ = false;
   uint32_t* d_count_ = nullptr;
   uint64_t* d_hash_ = nullptr;
   WindowState* h_ws_ = nullptr;          // pinned mirror of d_ws_
@@ -78,12 +81,12 @@ class PholdPool {
 class PholdBatch {
  public:
   static gpq_status create(const gpq_phold_instance* inst, uint32_t count, uint32_t capacity_per_lp,
-                           uint32_t stage_per_lp, PholdBatch** out);
+                           uint32_t stage_per_lp, uint32_t far_per_lp, PholdBatch** out);
   gpq_status run(gpq_drive_mode mode, uint32_t batch, double* seconds);
   gpq_status results(uint64_t* hashes, uint64_t* events);
 
   std::vector<gpq_phold_instance> inst;
-  uint32_t capacity_per_lp = 128, stage_per_lp = 32;
+  uint32_t capacity_per_lp = 128, stage_per_lp = 32, far_per_lp = 0;
   std::vector<uint64_t> hash, events;
 };
 

@@ -97,6 +97,62 @@ __global__ void pool_publish_kernel(const uint32_t* partial, const PoolInst* ins
   }
 }
 
+// Two-tier pool: one workgroup per instance; gmin = min(ring heads, far minima) over its
+// blocks, the window is capped at the instance's near horizon, and a window that reaches it
+// schedules the move to gmin + lookahead + 4 * lookahead for the next post.
+__global__ void pool_far_publish_kernel(const uint32_t* partial, uint32_t nb, const PoolInst* inst, WindowState* ws,
+                                        uint32_t slots) {
+  const uint32_t i = blockIdx.x;
+  if (i >= slots) return;
+  const PoolInst in = inst[i];
+  if (!in.active || ws[i].done) return;
+  __shared__ uint32_t s_a[4], s_b[4];
+  uint32_t a = kInfBits, b = kInfBits;
+  for (uint32_t k = in.block_begin + threadIdx.x; k < in.block_end; k += blockDim.x) {
+    a = partial[k] < a ? partial[k] : a;
+    b = partial[nb + k] < b ? partial[nb + k] : b;
+  }
+  a = wave_min_u32(a);
+  b = wave_min_u32(b);
+  if (lane_id() == 0) {
+    s_a[threadIdx.x / kWave] = a;
+    s_b[threadIdx.x / kWave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t k = 0; k < blockDim.x / kWave; ++k) {
+      a = s_a[k] < a ? s_a[k] : a;
+      b = s_b[k] < b ? s_b[k] : b;
+    }
+    WindowState w = ws[i];
+    if (w.near_next) {
+      w.near_end = w.near_next;
+      w.near_next = 0;
+    }
+    const uint32_t g = a < b ? a : b;
+    const float gf = bits_to_f(g);
+    float wend = gf + in.lookahead;
+    if (in.end_time < wend) wend = in.end_time;
+    const float nearf = bits_to_f(w.near_end);
+    if (nearf < wend) wend = nearf;
+    if (g != kInfBits && gf + in.lookahead > nearf) {
+      float next = gf + 5.0f * in.lookahead;
+      if (!(next > nearf)) next = nearf + 4.0f * in.lookahead;
+      w.near_next = f_to_bits(next);
+    }
+    w.gmin = g;
+    w.wend = f_to_bits(wend);
+    w.done = (g == kInfBits || !(gf < in.end_time)) ? 1u : 0u;
+    w.windows += 1;
+    ws[i] = w;
+  }
+}
+
+__global__ void pool_fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // Admission: the instance's LPs get their initial events directly in sorted order (the same
 // ring contents as init_per_lp rounds of populate + queue_post_init), padding LPs of its
 // last block are emptied, and the counters and hashes start at zero.
@@ -108,6 +164,10 @@ __global__ void pool_populate_kernel(LaqView q, PoolInst in, uint32_t slot, uint
   Event* ring = q.ring + ((size_t)lp << q.cap_log2);
   q.rd[lp] = 0;
   q.ins[lp] = 0;
+  if (q.far_cap) {
+    q.nfar[lp] = 0;
+    q.far_min[lp] = kInfBits;
+  }
   count[lp] = 0;
   hash[lp] = 0;
   if (local >= in.lps) {
@@ -150,8 +210,10 @@ __global__ void pool_first_window_kernel(const LaqView q, PoolInst in, WindowSta
     for (uint32_t k = 1; k < blockDim.x / kWave; ++k) g = s_w[k] < g ? s_w[k] : g;
     float wend = bits_to_f(g) + in.lookahead;
     if (in.end_time < wend) wend = in.end_time;
-    ws[slot] = WindowState{g, f_to_bits(wend), (g == kInfBits || !(bits_to_f(g) < in.end_time)) ? 1u : 0u, 0u, 1ull,
-                           0u, 0u};
+    // two-tier: every record is in the ring, so the near horizon starts past the first window
+    const uint32_t near_end = q.far_cap ? f_to_bits(bits_to_f(g) + 5.0f * in.lookahead) : 0u;
+    ws[slot] = WindowState{g, f_to_bits(wend), (g == kInfBits || !(bits_to_f(g) < in.end_time)) ? 1u : 0u, near_end,
+                           1ull, 0u, 0u};
   }
 }
 
@@ -162,6 +224,10 @@ __global__ void pool_release_kernel(uint32_t* lp_inst, uint32_t base, uint32_t n
     q.len[base + l] = 0;
     q.ins[base + l] = 0;
     q.head[base + l] = kInfBits;
+    if (q.far_cap) {
+      q.nfar[base + l] = 0;
+      q.far_min[base + l] = kInfBits;
+    }
   }
 }
 
@@ -173,12 +239,13 @@ __global__ void pool_init_kernel(uint32_t* lp_inst, uint32_t n, WindowState* ws,
 
 }  // namespace
 
-uint64_t PholdPool::bytes_per_lp(uint32_t capacity_per_lp, uint32_t stage_per_lp) {
-  return (uint64_t)(capacity_per_lp + stage_per_lp) * sizeof(Event) + 4ull * 4 + 4 + 4 + 8 + 4;
+uint64_t PholdPool::bytes_per_lp(uint32_t capacity_per_lp, uint32_t stage_per_lp, uint32_t far_per_lp) {
+  return (uint64_t)(capacity_per_lp + stage_per_lp + far_per_lp) * sizeof(Event) + 4ull * 4 + 4 + 4 + 8 + 4 +
+         (far_per_lp ? 8ull : 0ull);
 }
 
 gpq_status PholdPool::create(uint32_t capacity_lps, uint32_t capacity_per_lp, uint32_t stage_per_lp,
-                             uint32_t max_instances, PholdPool** out) {
+                             uint32_t max_instances, PholdPool** out, uint32_t far_per_lp) {
   GPQ_CHECK_ARG(capacity_lps >= 1 && max_instances >= 1, "pool: empty");
   const uint32_t cap = (uint32_t)ceil_div(capacity_lps, kBlk) * kBlk;
   gpq_queue_config qc{};
@@ -189,6 +256,7 @@ gpq_status PholdPool::create(uint32_t capacity_lps, uint32_t capacity_per_lp, ui
   qc.lookahead = 0.1f;
   qc.horizon = 1.0f;
   qc.block_threads = 256;
+  qc.far_per_lp = far_per_lp;
   auto p = std::make_unique<PholdPool>();
   p->q_ = new LaqQueue(qc);
   GPQ_TRY(p->q_->init_common());
@@ -201,6 +269,12 @@ gpq_status PholdPool::create(uint32_t capacity_lps, uint32_t capacity_per_lp, ui
   GPQ_TRY(p->arena_.alloc(&p->d_block_live_, cap / kBlk));
   GPQ_HIP(hipMemsetAsync(p->d_block_live_, 0, cap / kBlk, p->q_->stream));
   p->q_->set_block_live(p->d_block_live_);
+  p->far_ = far_per_lp != 0;
+  if (p->far_) {
+    GPQ_TRY(p->arena_.alloc(&p->d_block_win_, cap / kBlk));
+    GPQ_HIP(hipMemsetAsync(p->d_block_win_, 0, sizeof(uint32_t) * (cap / kBlk), p->q_->stream));
+    p->q_->set_block_windows(p->d_block_win_, p->d_ws_);
+  }
   GPQ_TRY(p->arena_.alloc(&p->d_count_, cap));
   GPQ_TRY(p->arena_.alloc(&p->d_hash_, cap));
   GPQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_ws_), sizeof(WindowState) * max_instances,
@@ -258,6 +332,11 @@ gpq_status PholdPool::admit(const gpq_phold_instance& c, uint32_t* slot, bool* o
   if (!found) return GPQ_OK;
   for (uint32_t b = start; b < start + need; ++b) block_used_[b] = 1;
   GPQ_HIP(hipMemsetAsync(d_block_live_ + start, 1, need, q_->stream));
+  if (far_) {
+    hipLaunchKernelGGL(pool_fill_u32_kernel, dim3(ceil_div(need, 256)), dim3(256), 0, q_->stream, d_block_win_ + start,
+                       need, s);
+    GPQ_HIP(hipGetLastError());
+  }
   PoolInst in{};
   in.base = start * kBlk;
   in.lps = c.num_lps;
@@ -296,7 +375,10 @@ gpq_status PholdPool::prepare(uint32_t batch) {
     hipLaunchKernelGGL(pool_step_kernel, dim3(ceil_div(capacity_, 256)), dim3(256), 0, q_->stream, v, d_inst_, d_ws_,
                        d_lp_inst_, d_count_, d_hash_, capacity_, &q_->ws->error);
     st = q_->merge_only();
-    if (st == GPQ_OK)
+    if (st == GPQ_OK && far_)
+      hipLaunchKernelGGL(pool_far_publish_kernel, dim3(max_inst_), dim3(256), 0, q_->stream, q_->partial,
+                         capacity_ / kBlk, d_inst_, d_ws_, max_inst_);
+    else if (st == GPQ_OK)
       hipLaunchKernelGGL(pool_publish_kernel, dim3(max_inst_), dim3(256), 0, q_->stream, q_->partial, d_inst_, d_ws_,
                          max_inst_);
   }
@@ -375,7 +457,7 @@ gpq_status PholdPool::retire(uint32_t s, uint64_t* trace_hash, uint64_t* events)
 // PholdBatch
 // ---------------------------------------------------------------------------------------
 gpq_status PholdBatch::create(const gpq_phold_instance* in, uint32_t count, uint32_t cap, uint32_t stage,
-                              PholdBatch** out) {
+                              uint32_t far, PholdBatch** out) {
   GPQ_CHECK_ARG(in && count >= 1, "phold batch: no instances");
   for (uint32_t i = 0; i < count; ++i) {
     GPQ_CHECK_ARG(in[i].num_lps >= 1 && in[i].init_per_lp >= 1, "phold batch: empty instance");
@@ -386,6 +468,7 @@ gpq_status PholdBatch::create(const gpq_phold_instance* in, uint32_t count, uint
   b->inst.assign(in, in + count);
   b->capacity_per_lp = cap ? cap : 128;
   b->stage_per_lp = stage ? stage : 32;
+  b->far_per_lp = far;
   b->hash.assign(count, 0);
   b->events.assign(count, 0);
   *out = b.release();
@@ -402,6 +485,7 @@ gpq_status PholdBatch::run(gpq_drive_mode mode, uint32_t batch, double* seconds)
       qc.num_lps = inst[i].num_lps;
       qc.capacity_per_lp = capacity_per_lp;
       qc.stage_per_lp = stage_per_lp;
+      qc.far_per_lp = far_per_lp;
       qc.lookahead = inst[i].lookahead;
       qc.horizon = inst[i].end_time;
       qc.block_threads = 256;
@@ -425,7 +509,7 @@ gpq_status PholdBatch::run(gpq_drive_mode mode, uint32_t batch, double* seconds)
     for (const auto& c : inst) total += ceil_div(c.num_lps, kBlk) * (uint64_t)kBlk;
     GPQ_CHECK_ARG(total < (1ull << 31) / capacity_per_lp, "phold batch: too many LPs for one pool");
     PholdPool* raw = nullptr;
-    GPQ_TRY(PholdPool::create((uint32_t)total, capacity_per_lp, stage_per_lp, n, &raw));
+    GPQ_TRY(PholdPool::create((uint32_t)total, capacity_per_lp, stage_per_lp, n, &raw, far_per_lp));
     std::unique_ptr<PholdPool> pool(raw);
     std::vector<uint32_t> slot_of(n), req_of(n);
     for (uint32_t i = 0; i < n; ++i) {

@@ -30,6 +30,10 @@ struct LaqView {
   WindowState* ws;
   unsigned long long* stats;   // work counters, nullptr when not collected
   const uint8_t* block_live;    // per 256-LP post block, 0 = holds no LP in use (skipped); nullptr = all live
+  // Two-tier mode with several windows (a pool of instances): the near horizon of post block b
+  // is that of block_ws[block_win[b]]; nullptr = ws (one window).
+  const uint32_t* block_win;
+  const WindowState* block_ws;
   Event* far;                   // two-tier mode only (far_cap > 0)
   uint32_t* nfar;
   uint32_t* far_min;

@@ -225,8 +225,17 @@ __global__ __launch_bounds__(kLaqBlock) void laq_post_far_kernel(LaqView q, uint
   __shared__ Event s_near[kLaqBlock / G][kFarList];
   const uint32_t nb = gridDim.x;
   const uint32_t lp = blockIdx.x * kLaqBlock + threadIdx.x;
-  const uint32_t moving = q.ws->near_next;
-  const uint32_t N = moving ? moving : q.ws->near_end;
+  if (q.block_live && !q.block_live[blockIdx.x]) {     // a block of unused LPs: nothing staged, no heads
+    if (threadIdx.x == 0) {
+      partial[blockIdx.x] = kInfBits;
+      partial[gridDim.x + blockIdx.x] = kInfBits;
+      partial[2 * gridDim.x + blockIdx.x] = 0;
+    }
+    return;
+  }
+  const WindowState* hw = q.block_win ? &q.block_ws[q.block_win[blockIdx.x]] : q.ws;
+  const uint32_t moving = hw->near_next;
+  const uint32_t N = moving ? moving : hw->near_end;
   if (threadIdx.x == 0) {
     s_count = 0;
     s_inserted = 0;

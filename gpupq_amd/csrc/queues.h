@@ -29,6 +29,11 @@ class LaqQueue final : public QueueBase {
   static constexpr uint32_t kBlockLps = 256;
   // Post blocks whose flag is 0 are skipped (their LPs hold nothing); nullptr = all live.
   void set_block_live(const uint8_t* flags) { v_.block_live = flags; }
+  // Two-tier mode, several windows: block b follows the near horizon of ws_array[win[b]].
+  void set_block_windows(const uint32_t* win, const WindowState* ws_array) {
+    v_.block_win = win;
+    v_.block_ws = ws_array;
+  }
 
  private:
   LaqView v_{};

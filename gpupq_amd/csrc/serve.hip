@@ -31,8 +31,9 @@ namespace gpq {
 
 namespace {
 
-constexpr uint32_t kPholdCapacity = 128;   // records per LP ring
+constexpr uint32_t kPholdCapacity = 64;    // records per LP ring (two-tier LAQ: below the near horizon)
 constexpr uint32_t kPholdStage = 32;       // staged inserts per LP and step
+constexpr uint32_t kPholdFar = 128;        // far-store records per LP
 
 double now_ms(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -56,6 +57,7 @@ gpq_queue_config phold_queue_config(uint32_t lps, float end_time) {
   q.num_lps = lps;
   q.capacity_per_lp = kPholdCapacity;
   q.stage_per_lp = kPholdStage;
+  q.far_per_lp = kPholdFar;
   q.lookahead = 0.1f;
   q.horizon = end_time;
   q.block_threads = 256;
@@ -75,9 +77,9 @@ gpq_phold_config phold_model_config(uint32_t lps, const gpq_serve_config& c, uin
   return p;
 }
 
-// HBM of one PHOLD request: ring + staging records, per-LP queue words, model state.
+// HBM of one PHOLD request: ring + staging + far records, per-LP queue words, model state.
 uint64_t phold_bytes(uint32_t lps) {
-  return (uint64_t)lps * ((kPholdCapacity + kPholdStage) * 16ull + 32ull + 16ull) + (1u << 20);
+  return (uint64_t)lps * ((kPholdCapacity + kPholdStage + kPholdFar) * 16ull + 40ull + 16ull) + (1u << 20);
 }
 
 struct Instance {
@@ -280,18 +282,19 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
       if (r.kind == 0) { max_lps = std::max(max_lps, r.size); ++phold_n; }
     std::unique_ptr<PholdPool> pool;
     if (phold_n) {
-      const uint64_t per_lp = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage);
+      const uint64_t per_lp = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage, kPholdFar);
       uint64_t cap = c.hbm_budget / per_lp;
       cap = std::max<uint64_t>(cap, ((uint64_t)max_lps + 255) / 256 * 256);
-      cap = std::min<uint64_t>(cap, ((1ull << 31) / kPholdCapacity) - 256);
+      cap = std::min<uint64_t>(cap, ((1ull << 31) / kPholdFar) - 256);
       PholdPool* raw = nullptr;
-      GPQ_TRY(PholdPool::create((uint32_t)cap, kPholdCapacity, kPholdStage, std::min<uint32_t>(phold_n, 1024), &raw));
+      GPQ_TRY(PholdPool::create((uint32_t)cap, kPholdCapacity, kPholdStage, std::min<uint32_t>(phold_n, 1024), &raw,
+                                kPholdFar));
       pool.reset(raw);
       GPQ_TRY(pool->prepare(c.phold_batch));
     }
     std::vector<uint32_t> slot_req(pool ? 1024 : 0, 0xFFFFFFFFu);
     std::vector<uint32_t> fin;
-    const uint64_t per_lp_bytes = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage);
+    const uint64_t per_lp_bytes = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage, kPholdFar);
     std::deque<uint32_t> waiting;          // arrived, not admitted
     std::vector<uint32_t> astar_ready;     // admitted A* queries not yet launched
     uint32_t next_arrival = 0;

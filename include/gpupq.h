@@ -302,8 +302,10 @@ typedef struct {
 
 typedef struct gpq_phold_batch gpq_phold_batch;
 
+/* capacity_per_lp / stage_per_lp: the LAQ sizes (0: 128 / 32); far_per_lp > 0: two-tier LAQ
+ * (every instance with its own near horizon; capacity_per_lp <= 128). */
 gpq_status gpq_phold_batch_create(const gpq_phold_instance* inst, uint32_t count, uint32_t capacity_per_lp,
-                                  uint32_t stage_per_lp, gpq_phold_batch** out);
+                                  uint32_t stage_per_lp, uint32_t far_per_lp, gpq_phold_batch** out);
 gpq_status gpq_phold_batch_destroy(gpq_phold_batch* b);
 /* GPQ_DRIVE_GRAPH: all instances in one pool, `batch` steps per graph replay.
  * GPQ_DRIVE_PLAIN: each instance alone in its own queue, a host synchronization per step.

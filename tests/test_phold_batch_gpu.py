@@ -17,8 +17,10 @@ INSTANCES = [
 ]
 
 
-def test_batch_matches_oracle_and_plain(gpq, oracle):
-    with gpq.PholdBatch(INSTANCES) as b:
+@pytest.mark.parametrize("tier", ["one", "two"])
+def test_batch_matches_oracle_and_plain(gpq, oracle, tier):
+    kw = {} if tier == "one" else dict(capacity_per_lp=64, far_per_lp=128)
+    with gpq.PholdBatch(INSTANCES, **kw) as b:
         b.run("graph", batch=8)
         hg, eg = b.results()
         b.run("plain")
