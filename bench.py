@@ -94,11 +94,18 @@ def roofline(gpq, kind, wl, log):
     # the staging counter (4) and the staged record (16)
     events_per_step = rep["events"] / max(d["post_launches"], 1)
     step_bytes = 4 * L + 88 * events_per_step
-    kernels = {
-        "queue_post": {"us": prof["post_kernel_us"], "alg_bytes": post_bytes},
-        "model_step": {"us": prof["step_kernel_us"], "alg_bytes": step_bytes},
-        "window_publish": {"us": prof["window_kernel_us"], "alg_bytes": 4 * ((L + 255) // 256) + 24},
-    }
+    if DRAIN == "fused":
+        # one kernel does the queue post's merge and the model step: the work of both
+        kernels = {
+            "merge_commit_step": {"us": prof["step_kernel_us"], "alg_bytes": post_bytes + step_bytes},
+            "window_publish": {"us": prof["window_kernel_us"], "alg_bytes": 80},
+        }
+    else:
+        kernels = {
+            "queue_post": {"us": prof["post_kernel_us"], "alg_bytes": post_bytes},
+            "model_step": {"us": prof["step_kernel_us"], "alg_bytes": step_bytes},
+            "window_publish": {"us": prof["window_kernel_us"], "alg_bytes": 4 * ((L + 255) // 256) + 24},
+        }
     for k in kernels.values():
         k["gbps"] = k["alg_bytes"] / (k["us"] * 1e-6) / 1e9 if k["us"] > 0 else 0.0
     dom = max(kernels, key=lambda k: kernels[k]["us"])
@@ -181,7 +188,10 @@ def per_queue_bench(gpq, wl, log):
     global DRAIN
     out = {}
     saved = DRAIN
-    for kind, drain in (("laq", False), ("lhq", True), ("lhq", False), ("lsq", True), ("gaq", False), ("hq", False)):
+    runs = [("laq", False), ("lhq", True), ("lhq", False), ("lsq", True), ("gaq", False), ("hq", False)]
+    if saved == "fused":
+        runs.insert(0, ("laq", True))             # the two-tier LAQ with a separate post kernel
+    for kind, drain in runs:
         DRAIN = drain
         q, m = make(gpq, kind, wl)
         q.sync()
@@ -254,19 +264,24 @@ def astar_single_bench(gpq, log, cpu=True, lists=1 << 16, window=0):
     return out
 
 
-def serving_bench(gpq, log, requests=64, rate=200.0):
-    """The Poisson trace at `rate` (latency under load), the same trace with every request
-    arriving at once (saturated throughput), and the plain one-request-at-a-time path."""
+def serving_bench(gpq, log, requests=64, rates=(50.0, 100.0, 200.0, 400.0)):
+    """The Poisson trace at each arrival rate in `rates` (latency under load), the same trace with
+    every request arriving at once (saturated throughput), and the plain one-request-at-a-time path."""
     kw = dict(seed=7, phold_fraction=0.5, hbm_budget=8 << 30)
-    rep_s, served = gpq.serve_run(requests, mode="graph", rate=rate, **kw)
-    rep_x, sat = gpq.serve_run(requests, mode="graph", rate=0.0, **kw)
     rep_p, plain = gpq.serve_run(requests, mode="plain", rate=0.0, **kw)
     res = [r["result"] for r in plain]
-    if [r["result"] for r in served] != res or [r["result"] for r in sat] != res:
-        raise RuntimeError("serving: served and plain results differ")
     keys = ("requests_per_second", "p50_ms", "p99_ms", "phold_p99_ms", "astar_p99_ms", "peak_inflight", "peak_bytes")
+    swept = []
+    for rate in rates:
+        rep_s, served = gpq.serve_run(requests, mode="graph", rate=rate, **kw)
+        if [r["result"] for r in served] != res:
+            raise RuntimeError("serving at %g/s: served and plain results differ" % rate)
+        swept.append(dict({"rate_per_s": rate}, **{k: rep_s[k] for k in keys}))
+    rep_x, sat = gpq.serve_run(requests, mode="graph", rate=0.0, **kw)
+    if [r["result"] for r in sat] != res:
+        raise RuntimeError("serving saturated: served and plain results differ")
     out = {"requests": requests, "phold_lps": "2^U[12,18] x 16, end time 20", "astar": "side 2^U[8,11], 8-connected",
-           "served_at_rate": dict({"rate_per_s": rate}, **{k: rep_s[k] for k in keys}),
+           "served_at_rate": swept,
            "served_saturated": {k: rep_x[k] for k in keys},
            "plain": {k: rep_p[k] for k in ("requests_per_second", "p50_ms", "p99_ms")},
            "served_vs_plain_throughput": rep_x["requests_per_second"] / rep_p["requests_per_second"],
@@ -307,9 +322,12 @@ def main():
     ap.add_argument("--queue", default="laq", choices=sorted(QUEUES))
     ap.add_argument("--workload", default="phold_2p20_k32_t50", choices=sorted(WORKLOADS))
     ap.add_argument("--batch", type=int, default=64, help="steps per hipGraph replay")
-    ap.add_argument("--one-event", action="store_true",
-                    help="commit one event per LP per window (the reference's step) instead of draining each "
-                         "LP's safe window; the committed trace is the same")
+    ap.add_argument("--step", default="fused", choices=("fused", "drain", "one-event"),
+                    help="fused: the two-tier LAQ merges the staged events and commits the window in one kernel "
+                         "per step (windows one lookahead apart); drain: a post kernel, then each LP commits its "
+                         "safe window; one-event: one event per LP per window (the reference's step).  The "
+                         "committed trace is the same")
+    ap.add_argument("--one-event", action="store_true", help="same as --step one-event")
     ap.add_argument("--far", type=int, default=128,
                     help="LAQ two-tier mode: far-store records per LP (0 = single sorted ring)")
     ap.add_argument("--far-step", type=float, default=0.0, help="LAQ two-tier: near-horizon step (0 = 4 x lookahead)")
@@ -319,7 +337,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-queues", action="store_true", help="skip the same workload through the other queue kinds")
-    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-rates", default="50,100,200,400", help="Poisson arrival rates (requests/s) to sweep")
     ap.add_argument("--skip-phold", action="store_true", help="profiling aid: A* and serving parts only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy five times (counter calibration)")
@@ -337,7 +355,11 @@ def main():
 
     import gpupq_amd as gpq
     global DRAIN, FAR
-    DRAIN = not args.one_event and args.queue in ("laq", "lhq", "lsq")
+    step = "one-event" if args.one_event else args.step
+    if step == "fused" and not (args.queue == "laq" and args.far):
+        step = "drain"                   # the merge-and-commit step exists for the two-tier LAQ only
+    DRAIN = False if step == "one-event" or args.queue not in ("laq", "lhq", "lsq") else \
+        ("fused" if step == "fused" else True)
     FAR = args.far
     global FAR_STEP
     FAR_STEP = args.far_step
@@ -353,8 +375,9 @@ def main():
             line["phold_batch"] = phold_batch_bench(gpq, log)
         if not args.no_astar:
             line["astar_batch"] = astar_bench(gpq, log, cpu=not args.no_cpu)
+            line["astar_single"] = astar_single_bench(gpq, log, cpu=not args.no_cpu)
         if not args.no_serving:
-            line["serving"] = serving_bench(gpq, log, rate=args.serve_rate)
+            line["serving"] = serving_bench(gpq, log, rates=[float(r) for r in args.serve_rates.split(",")])
         print(json.dumps(line), flush=True)
         return
     total_events, total_s, hashes = 0, 0.0, set()
@@ -386,7 +409,9 @@ def main():
         "config": {"workload": args.workload, "queue": args.queue, "num_lps": wl["num_lps"],
                    "init_per_lp": wl["init_per_lp"], "end_time": wl["end_time"], "lookahead": 0.1,
                    "drive": "hipgraph x%d" % args.batch, "events_per_step": total_events // args.steps,
-                   "window": "drain each LP's safe window" if DRAIN else "one event per LP per window",
+                   "window": ("merge and commit in one kernel per step, windows one lookahead apart"
+                              if DRAIN == "fused" else
+                              "drain each LP's safe window" if DRAIN else "one event per LP per window"),
                    "laq_far_per_lp": FAR,
                    "trace_hash_vs_cpu": parity},
     }
@@ -396,6 +421,8 @@ def main():
         kname = dom
         if args.queue == "laq" and dom == "queue_post":
             kname = "laq_post_far_kernel" if FAR else "laq_post_kernel"
+        if dom == "merge_commit_step":
+            kname = "phold_fused_kernel"
         line["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS,
                             "unit": "GB/s", "frac": k["gbps"] / HBM_PEAK_GBPS,
                             "traffic": traffic_from_profiles(kname, args.workload),
@@ -414,7 +441,7 @@ def main():
         line["astar_batch"] = astar_bench(gpq, log, cpu=not args.no_cpu)
         line["astar_single"] = astar_single_bench(gpq, log, cpu=not args.no_cpu)
     if not args.no_serving:
-        line["serving"] = serving_bench(gpq, log, rate=args.serve_rate)
+        line["serving"] = serving_bench(gpq, log, rates=[float(r) for r in args.serve_rates.split(",")])
     print(json.dumps(line), flush=True)
 
 

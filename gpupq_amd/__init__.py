@@ -153,11 +153,14 @@ class Phold:
 
     def __init__(self, queue, init_per_lp=16, seed=42, end_time=None, lookahead=None, p_self=0.5, checks=True,
                  drain=False):
+        """drain: False (one event per LP per step), True (each LP commits its whole safe window)
+        or "fused" (the same, with the LAQ merge inside the step kernel; two-tier LAQ only)."""
         self.queue = queue
         lookahead = queue.cfg.lookahead if lookahead is None else lookahead
         end_time = queue.cfg.horizon if end_time is None else end_time
         self.cfg = PholdConfig(queue.num_lps, init_per_lp, seed, lookahead, end_time,
-                               min(int(round(p_self * 2 ** 32)), 0xFFFFFFFF), 1 if checks else 0, 1 if drain else 0)
+                               min(int(round(p_self * 2 ** 32)), 0xFFFFFFFF), 1 if checks else 0,
+                               2 if drain == "fused" else (1 if drain else 0))
         h = ctypes.c_void_p()
         check(lib.gpq_phold_create(queue.handle, ctypes.byref(self.cfg), ctypes.byref(h)), "phold_create")
         self.handle = h

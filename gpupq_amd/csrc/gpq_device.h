@@ -141,6 +141,7 @@ struct WindowState {
   uint64_t windows;          // number of windows computed
   uint32_t error;            // GPQ_FLAG_* bits
   uint32_t near_next;        // LAQ two-tier: nonzero = the next post moves the near horizon here
+  uint32_t sel;              // PHOLD merge-and-commit step: staging buffer read by the next step (0 main, 1 alt)
 };
 
 enum : uint32_t {

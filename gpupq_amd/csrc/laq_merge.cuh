@@ -1,0 +1,241 @@
+// laq_merge.cuh -- device building blocks of the local array queue's merge, shared by the
+// queue's post kernels (queue_laq.hip) and the PHOLD step that merges and commits in one pass
+// (phold.hip).  Reference: src/local_array_queue.cu (the per-LP insertion sort :254-299,
+// src/sort.cu:329-380) and the heap-backed variant's near/far split (:515-681).
+#pragma once
+
+#include "queue_laq.cuh"
+
+namespace gpq {
+namespace laq {
+
+constexpr int kSlots = 8;                  // default ring records held per lane: a group of G lanes covers 8G records
+constexpr uint32_t kFarList = 64;          // two-tier: near events one LP may take in one pass (merge input)
+
+struct Key3 {
+  uint32_t ts, src, seq;
+};
+
+__device__ __forceinline__ Key3 shfl_key(const Event& e, int src_lane) {
+  return Key3{(uint32_t)__shfl((int)e.ts, src_lane, kWave), (uint32_t)__shfl((int)e.src, src_lane, kWave),
+              (uint32_t)__shfl((int)e.seq, src_lane, kWave)};
+}
+
+__device__ __forceinline__ bool key_less3(const Key3& a, const Event& b) {
+  if (a.ts != b.ts) return a.ts < b.ts;
+  if (a.src != b.src) return a.src < b.src;
+  return a.seq < b.seq;
+}
+
+__device__ __forceinline__ bool key_less3(const Key3& a, const Key3& b) {
+  if (a.ts != b.ts) return a.ts < b.ts;
+  if (a.src != b.src) return a.src < b.src;
+  return a.seq < b.seq;
+}
+
+// Staged event i of LP lp: from the LP's staging slots in HBM, or from a key-only list in LDS
+// (records staged for LP lp all carry dst = lp).
+__device__ __forceinline__ Event staged_at(const Event* stg, uint32_t i, uint32_t) { return load_event(&stg[i]); }
+__device__ __forceinline__ Event staged_at(const Key3* stg, uint32_t i, uint32_t lp) {
+  const Key3 k = stg[i];
+  return Event{k.ts, lp, k.src, k.seq};
+}
+
+// Orders the LDS histogram updates of the lanes of one wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One merge pass: the k <= G - 1 staged events stg[0 .. k) of LP lp go into its sorted run of
+// n records at ring slot r, by a group of G lanes (lane g of the group, group starting at wave
+// lane gbase).  The run is loaded at once (record j in lane j % G, register slot j / G), one
+// staged event per lane.  Old record j ends at final index j + c_j, c_j = number of new
+// events with a smaller key, and c is nondecreasing in j; the new event of rank a lands at
+// a + #{j : c_j <= a}.  A histogram of c over [0, k] (LDS, one bin per lane) gives both: its
+// inclusive prefix places the new events, and bin a counts the old records that stay put
+// when the run grows by a slots at the front and k - a at the back.  The a with the largest
+// bin is taken, so the fewest records move: a = 0 is the reference's backward insertion
+// (src/sort.cu:329-356), a = k its forward variant (src/sort.cu:358-379), chosen here per LP
+// and pass.  Updates n and r and returns the records moved.
+//
+// kDrain: the records below wend (final indices [0, d), d returned in *d_out) are taken out
+// of the run: it then starts at index d (n and r count from there), the head word is set to
+// the record at index d, and the taken records stay in the ring slots just before the new
+// start for the caller to commit.  Without kDrain the head changes only when a new event
+// lands at index 0.
+template <int G, int SL, bool kDrain, typename Stg>
+__device__ uint32_t laq_merge_pass(const LaqView& q, uint32_t lp, const Stg* stg, uint32_t k,
+                                    uint32_t& n, uint32_t& r, int g, int gbase, uint32_t* hist,
+                                    uint32_t wend, uint32_t* d_out) {
+  const uint32_t mask = (1u << q.cap_log2) - 1u;
+  Event* ring = q.ring + ((size_t)lp << q.cap_log2);
+  Event e{kInfBits, 0u, 0u, 0u};
+  if (g < (int)k) e = staged_at(stg, (uint32_t)g, lp);
+  const uint32_t ns = (n + G - 1) / G;
+  // ring records of LP lp all carry dst = lp: only the key words are kept
+  Key3 old[SL];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const uint32_t j = (uint32_t)(s * G + g);
+    old[s] = Key3{0u, 0u, 0u};
+    if ((uint32_t)s < ns && j < n) {
+      const Event o = load_event(&ring[(r + j) & mask]);
+      old[s] = Key3{o.ts, o.src, o.seq};
+    }
+  }
+  // rank of each new event among the new events (keys are unique)
+  uint32_t rank = 0;
+  uint32_t c[SL];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) c[s] = 0;
+  for (uint32_t m = 0; m < k; ++m) {
+    const Key3 nm = shfl_key(e, gbase + (int)m);
+    rank += (g < (int)k && key_less3(nm, Key3{e.ts, e.src, e.seq})) ? 1u : 0u;
+#pragma unroll
+    for (int s = 0; s < SL; ++s)
+      if ((uint32_t)s < ns) c[s] += ((uint32_t)(s * G + g) < n && key_less3(nm, old[s])) ? 1u : 0u;
+  }
+  // histogram of c over [0, k], k < G: bin b in lane b
+  hist[g] = 0;
+  wave_lds_sync();
+#pragma unroll
+  for (int s = 0; s < SL; ++s)
+    if ((uint32_t)s < ns && (uint32_t)(s * G + g) < n) atomicAdd(&hist[c[s]], 1u);
+  wave_lds_sync();
+  const uint32_t h = g <= (int)k ? hist[g] : 0u;
+  wave_lds_sync();
+  // a = argmax over bins (smallest a on ties); inclusive prefix of the bins
+  uint32_t best = g <= (int)k ? ((h << 8) | (255u - (uint32_t)g)) : 0u;
+  uint32_t pre = h;
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)best, off, kWave);
+    best = o > best ? o : best;
+    const uint32_t t = (uint32_t)__shfl((int)pre, gbase + (g >= off ? g - off : g), kWave);
+    if (g >= off) pre += t;
+  }
+  const uint32_t best_a = 255u - (best & 255u), best_stay = best >> 8;
+  const uint32_t below = (uint32_t)__shfl((int)pre, gbase + (int)(rank < k ? rank : 0u), kWave);
+  const uint32_t r_new = (r - best_a) & mask;
+  if constexpr (!kDrain) {
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const uint32_t j = (uint32_t)(s * G + g);
+      if ((uint32_t)s < ns && j < n && c[s] != best_a)
+        store_event(&ring[(r_new + j + c[s]) & mask], Event{old[s].ts, lp, old[s].src, old[s].seq});
+    }
+    if (g < (int)k) {
+      const uint32_t f = rank + below;
+      store_event(&ring[(r_new + f) & mask], e);
+      if (f == 0) q.head[lp] = e.ts;
+    }
+    n += k;
+    r = r_new;
+    return n - k - best_stay;
+  } else {
+    // d = records below wend, a prefix of the merged run
+    uint32_t lt = (g < (int)k && e.ts < wend) ? 1u : 0u;
+#pragma unroll
+    for (int s = 0; s < SL; ++s)
+      if ((uint32_t)s < ns && (uint32_t)(s * G + g) < n && old[s].ts < wend) ++lt;
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) lt += (uint32_t)__shfl_xor((int)lt, off, kWave);
+    const uint32_t d = lt, total = n + k;
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const uint32_t j = (uint32_t)(s * G + g);
+      if ((uint32_t)s < ns && j < n) {
+        const uint32_t fi = j + c[s];
+        if (c[s] != best_a)
+          store_event(&ring[(r_new + fi) & mask], Event{old[s].ts, lp, old[s].src, old[s].seq});
+        if (fi == d) q.head[lp] = old[s].ts;
+      }
+    }
+    if (g < (int)k) {
+      const uint32_t fi = rank + below;
+      store_event(&ring[(r_new + fi) & mask], e);
+      if (fi == d) q.head[lp] = e.ts;
+    }
+    if (g == 0 && d == total) q.head[lp] = kInfBits;
+    n = total - d;
+    r = (r_new + d) & mask;
+    *d_out = d;
+    return total - k - best_stay;
+  }
+}
+
+// Merges the k_total staged events of one LP into its sorted run of n records at ring slot r,
+// G - 1 per pass, and writes the LP's ring indices back.  Returns the records moved.
+template <int G, int SL, typename Stg>
+__device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, const Stg* stg, uint32_t k_total, uint32_t n,
+                                    uint32_t r, int g, int gbase, uint32_t* hist) {
+  uint32_t moved = 0;
+  for (uint32_t done = 0; done < k_total;) {
+    if (done) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");   // previous pass's ring writes
+    const uint32_t k = min(k_total - done, (uint32_t)G - 1u);
+    moved += laq_merge_pass<G, SL, false>(q, lp, stg + done, k, n, r, g, gbase, hist, 0u, nullptr);
+    done += k;
+  }
+  if (g == 0) {
+    q.rd[lp] = r;
+    q.len[lp] = n;
+    q.ins[lp] = 0;
+  }
+  return moved;
+}
+
+// Two-tier split of the candidates of LP lp -- its far records first when the near horizon
+// moves, then its k staged records -- at the horizon N, by a group of G lanes.  Keys below N
+// are listed in `near` (LDS, every record of LP lp has dst = lp); the others are compacted
+// into the far store, in place after the far records kept untouched (all of them when the
+// horizon does not move, none when it does).  Returns the number of near keys; *n_far and
+// *fmin get the far count and the smallest far ts bits (+inf when empty).
+template <int G>
+__device__ __forceinline__ uint32_t split_candidates(const LaqView& q, uint32_t lp, const Event* stg, uint32_t k,
+                                                     uint32_t f, bool moving, uint32_t N, Key3* near, int g,
+                                                     uint64_t gmask, uint32_t* n_far_out, uint32_t* fmin_out) {
+  const int lane = lane_id();
+  const uint32_t cands = (moving ? f : 0u) + k;
+  Event* far = q.far + (size_t)lp * q.far_cap;
+  uint32_t n_near = 0, n_far = moving ? 0u : f;
+  uint32_t fmin = moving ? kInfBits : q.far_min[lp];
+  for (uint32_t base = 0; base < cands; base += G) {
+    const uint32_t c = base + (uint32_t)g;
+    Event e{kInfBits, 0u, 0u, 0u};
+    const bool valid = c < cands;
+    if (valid) {
+      const uint32_t nf = moving ? f : 0u;
+      e = load_event(c < nf ? &far[c] : &stg[c - nf]);
+    }
+    const bool is_near = valid && e.ts < N;
+    const bool is_far = valid && !is_near;
+    const uint64_t bn = __ballot(is_near) & gmask, bf = __ballot(is_far) & gmask;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t pn = n_near + (uint32_t)__popcll(bn & below);
+    const uint32_t pf = n_far + (uint32_t)__popcll(bf & below);
+    if (is_near) {
+      if (pn < kFarList) near[pn] = Key3{e.ts, e.src, e.seq};
+      else atomicOr(&q.ws->error, kErrQueueOverflow);
+    }
+    if (is_far) {
+      if (pf < q.far_cap) store_event(&far[pf], e);
+      else atomicOr(&q.ws->error, kErrQueueOverflow);
+      fmin = e.ts < fmin ? e.ts : fmin;
+    }
+    n_near += (uint32_t)__popcll(bn);
+    n_far += (uint32_t)__popcll(bf);
+  }
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)fmin, off, kWave);
+    fmin = o < fmin ? o : fmin;
+  }
+  *n_far_out = n_far;
+  *fmin_out = fmin;
+  return n_near;
+}
+
+}  // namespace laq
+}  // namespace gpq

@@ -38,6 +38,10 @@ class PholdModel {
   gpq_status prepare_graph(uint32_t batch);
   gpq_status launch_async(hipEvent_t done);
   gpq_status profile(uint32_t batch, gpq_kernel_profile* out);
+  // drain mode 2: the merge-and-commit kernel (queue_post then only publishes the window),
+  // and the closing fold of the last step's staged events (a no-op in the other modes)
+  gpq_status launch_fused(uint32_t settle);
+  gpq_status settle();
 
   gpq_phold_config cfg{};
   QueueBase* q = nullptr;

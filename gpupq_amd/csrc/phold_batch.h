@@ -58,9 +58,8 @@ class PholdPool {
   DeviceArena arena_;
   PoolInst* d_inst_ = nullptr;
   WindowState* d_ws_ = nullptr;
-  uint32_t* d_lp_inst_ = nullptr;
   uint8_t* d_block_live_ = nullptr;     // per 256-LP block: 1 while an instance owns it
-  uint32_t* d_block_win_ = nullptr;     // two-tier: per block, the owning instance slot
+  uint32_t* d_block_win_ = nullptr;     // per block, the owning instance slot (valid while the block is live)
   bool far_ = false;
   uint32_t* d_count_ = nullptr;
   uint64_t* d_hash_ = nullptr;

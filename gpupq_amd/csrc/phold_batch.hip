@@ -20,7 +20,6 @@ namespace gpq {
 
 namespace {
 
-constexpr uint32_t kNoInst = 0xFFFFFFFFu;
 constexpr uint32_t kBlk = LaqQueue::kBlockLps;
 
 uint64_t host_mix64_b(uint64_t x) {
@@ -30,20 +29,20 @@ uint64_t host_mix64_b(uint64_t x) {
   return x;
 }
 
+// 256 threads = one post block of LPs; an instance owns whole blocks, so its slot, window and
+// parameters are block-uniform (block_win, filled at admission).
 __global__ __launch_bounds__(256) void pool_step_kernel(LaqView q, const PoolInst* inst, const WindowState* ws,
-                                                        const uint32_t* lp_inst, uint32_t* count, uint64_t* hash,
+                                                        const uint32_t* block_win, uint32_t* count, uint64_t* hash,
                                                         uint32_t capacity, uint32_t* error) {
-  if (!q.block_live[blockIdx.x]) return;                // 256 threads = one post block of LPs
-  const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
-  if (lp >= capacity) return;
-  const uint32_t i = lp_inst[lp];
-  if (i == kNoInst) return;
-  const WindowState w = ws[i];
-  if (w.done) return;
-  const uint32_t wend = w.wend;
-  if (!(q.head[lp] < wend)) return;
+  if (!q.block_live[blockIdx.x]) return;
+  const uint32_t i = block_win[blockIdx.x];
+  if (ws[i].done) return;
+  const uint32_t wend = ws[i].wend;
   const PoolInst in = inst[i];
+  const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t local = lp - in.base;
+  if (lp >= capacity || local >= in.lps) return;         // the instance's last block may be partly unused
+  if (!(q.head[lp] < wend)) return;
   const uint32_t mask = (1u << q.cap_log2) - 1u;
   const Event* ring = q.ring + ((size_t)lp << q.cap_log2);
   uint32_t rd = q.rd[lp], len = q.len[lp];
@@ -98,8 +97,8 @@ __global__ void pool_publish_kernel(const uint32_t* partial, const PoolInst* ins
 }
 
 // Two-tier pool: one workgroup per instance; gmin = min(ring heads, far minima) over its
-// blocks, the window is capped at the instance's near horizon, and a window that reaches it
-// schedules the move to gmin + lookahead + 4 * lookahead for the next post.
+// blocks, the window is capped at the instance's near horizon, and when the next window could
+// reach it, the move to gmin + 2 lookahead + 4 lookahead is scheduled for the next post.
 __global__ void pool_far_publish_kernel(const uint32_t* partial, uint32_t nb, const PoolInst* inst, WindowState* ws,
                                         uint32_t slots) {
   const uint32_t i = blockIdx.x;
@@ -135,8 +134,8 @@ __global__ void pool_far_publish_kernel(const uint32_t* partial, uint32_t nb, co
     if (in.end_time < wend) wend = in.end_time;
     const float nearf = bits_to_f(w.near_end);
     if (nearf < wend) wend = nearf;
-    if (g != kInfBits && gf + in.lookahead > nearf) {
-      float next = gf + 5.0f * in.lookahead;
+    if (g != kInfBits && gf + 2.0f * in.lookahead > nearf) {   // one window ahead, as the single queue does
+      float next = gf + 6.0f * in.lookahead;
       if (!(next > nearf)) next = nearf + 4.0f * in.lookahead;
       w.near_next = f_to_bits(next);
     }
@@ -156,8 +155,7 @@ __global__ void pool_fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
 // Admission: the instance's LPs get their initial events directly in sorted order (the same
 // ring contents as init_per_lp rounds of populate + queue_post_init), padding LPs of its
 // last block are emptied, and the counters and hashes start at zero.
-__global__ void pool_populate_kernel(LaqView q, PoolInst in, uint32_t slot, uint32_t* lp_inst, uint32_t* count,
-                                     uint64_t* hash, uint32_t padded) {
+__global__ void pool_populate_kernel(LaqView q, PoolInst in, uint32_t* count, uint64_t* hash, uint32_t padded) {
   const uint32_t local = blockIdx.x * blockDim.x + threadIdx.x;
   if (local >= padded) return;
   const uint32_t lp = in.base + local;
@@ -171,12 +169,10 @@ __global__ void pool_populate_kernel(LaqView q, PoolInst in, uint32_t slot, uint
   count[lp] = 0;
   hash[lp] = 0;
   if (local >= in.lps) {
-    lp_inst[lp] = kNoInst;
     q.len[lp] = 0;
     q.head[lp] = kInfBits;
     return;
   }
-  lp_inst[lp] = slot;
   const uint32_t k = in.init_per_lp;
   for (uint32_t j = 0; j < k; ++j) {
     const uint4 r = philox4x32_10(make_uint4(j, local, 1u, 0u), make_uint2(in.seed, 0u));
@@ -217,10 +213,9 @@ __global__ void pool_first_window_kernel(const LaqView q, PoolInst in, WindowSta
   }
 }
 
-__global__ void pool_release_kernel(uint32_t* lp_inst, uint32_t base, uint32_t n, LaqView q) {
+__global__ void pool_release_kernel(uint32_t base, uint32_t n, LaqView q) {
   const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l < n) {
-    lp_inst[base + l] = kNoInst;
     q.len[base + l] = 0;
     q.ins[base + l] = 0;
     q.head[base + l] = kInfBits;
@@ -231,16 +226,15 @@ __global__ void pool_release_kernel(uint32_t* lp_inst, uint32_t base, uint32_t n
   }
 }
 
-__global__ void pool_init_kernel(uint32_t* lp_inst, uint32_t n, WindowState* ws, uint32_t slots) {
+__global__ void pool_init_kernel(WindowState* ws, uint32_t slots) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) lp_inst[i] = kNoInst;
   if (i < slots) ws[i] = WindowState{kInfBits, 0u, 1u, 0u, 0ull, 0u, 0u};
 }
 
 }  // namespace
 
 uint64_t PholdPool::bytes_per_lp(uint32_t capacity_per_lp, uint32_t stage_per_lp, uint32_t far_per_lp) {
-  return (uint64_t)(capacity_per_lp + stage_per_lp + far_per_lp) * sizeof(Event) + 4ull * 4 + 4 + 4 + 8 + 4 +
+  return (uint64_t)(capacity_per_lp + stage_per_lp + far_per_lp) * sizeof(Event) + 4ull * 4 + 4 + 8 +
          (far_per_lp ? 8ull : 0ull);
 }
 
@@ -265,16 +259,13 @@ gpq_status PholdPool::create(uint32_t capacity_lps, uint32_t capacity_per_lp, ui
   p->max_inst_ = max_instances;
   GPQ_TRY(p->arena_.alloc(&p->d_inst_, max_instances));
   GPQ_TRY(p->arena_.alloc(&p->d_ws_, max_instances));
-  GPQ_TRY(p->arena_.alloc(&p->d_lp_inst_, cap));
   GPQ_TRY(p->arena_.alloc(&p->d_block_live_, cap / kBlk));
   GPQ_HIP(hipMemsetAsync(p->d_block_live_, 0, cap / kBlk, p->q_->stream));
   p->q_->set_block_live(p->d_block_live_);
   p->far_ = far_per_lp != 0;
-  if (p->far_) {
-    GPQ_TRY(p->arena_.alloc(&p->d_block_win_, cap / kBlk));
-    GPQ_HIP(hipMemsetAsync(p->d_block_win_, 0, sizeof(uint32_t) * (cap / kBlk), p->q_->stream));
-    p->q_->set_block_windows(p->d_block_win_, p->d_ws_);
-  }
+  GPQ_TRY(p->arena_.alloc(&p->d_block_win_, cap / kBlk));
+  GPQ_HIP(hipMemsetAsync(p->d_block_win_, 0, sizeof(uint32_t) * (cap / kBlk), p->q_->stream));
+  if (p->far_) p->q_->set_block_windows(p->d_block_win_, p->d_ws_);
   GPQ_TRY(p->arena_.alloc(&p->d_count_, cap));
   GPQ_TRY(p->arena_.alloc(&p->d_hash_, cap));
   GPQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_ws_), sizeof(WindowState) * max_instances,
@@ -284,9 +275,8 @@ gpq_status PholdPool::create(uint32_t capacity_lps, uint32_t capacity_per_lp, ui
   p->block_used_.assign(cap / kBlk, 0);
   GPQ_HIP(hipMemcpyAsync(p->d_inst_, p->inst_.data(), sizeof(PoolInst) * max_instances, hipMemcpyHostToDevice,
                          p->q_->stream));
-  const uint32_t n = std::max(cap, max_instances);
-  hipLaunchKernelGGL(pool_init_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, p->q_->stream, p->d_lp_inst_, cap,
-                     p->d_ws_, max_instances);
+  hipLaunchKernelGGL(pool_init_kernel, dim3(ceil_div(max_instances, 256)), dim3(256), 0, p->q_->stream, p->d_ws_,
+                     max_instances);
   GPQ_HIP(hipGetLastError());
   GPQ_HIP(hipMemcpyAsync(p->h_ws_, p->d_ws_, sizeof(WindowState) * max_instances, hipMemcpyDeviceToHost,
                          p->q_->stream));
@@ -332,11 +322,9 @@ gpq_status PholdPool::admit(const gpq_phold_instance& c, uint32_t* slot, bool* o
   if (!found) return GPQ_OK;
   for (uint32_t b = start; b < start + need; ++b) block_used_[b] = 1;
   GPQ_HIP(hipMemsetAsync(d_block_live_ + start, 1, need, q_->stream));
-  if (far_) {
-    hipLaunchKernelGGL(pool_fill_u32_kernel, dim3(ceil_div(need, 256)), dim3(256), 0, q_->stream, d_block_win_ + start,
-                       need, s);
-    GPQ_HIP(hipGetLastError());
-  }
+  hipLaunchKernelGGL(pool_fill_u32_kernel, dim3(ceil_div(need, 256)), dim3(256), 0, q_->stream, d_block_win_ + start,
+                     need, s);
+  GPQ_HIP(hipGetLastError());
   PoolInst in{};
   in.base = start * kBlk;
   in.lps = c.num_lps;
@@ -351,8 +339,8 @@ gpq_status PholdPool::admit(const gpq_phold_instance& c, uint32_t* slot, bool* o
   inst_[s] = in;
   const LaqView v = q_->view();
   const uint32_t padded = need * kBlk;
-  hipLaunchKernelGGL(pool_populate_kernel, dim3(ceil_div(padded, 256)), dim3(256), 0, q_->stream, v, in, s,
-                     d_lp_inst_, d_count_, d_hash_, padded);
+  hipLaunchKernelGGL(pool_populate_kernel, dim3(ceil_div(padded, 256)), dim3(256), 0, q_->stream, v, in, d_count_,
+                     d_hash_, padded);
   hipLaunchKernelGGL(pool_first_window_kernel, dim3(1), dim3(1024), 0, q_->stream, v, in, d_ws_, s);
   GPQ_HIP(hipGetLastError());
   GPQ_HIP(hipMemcpyAsync(d_inst_ + s, &inst_[s], sizeof(PoolInst), hipMemcpyHostToDevice, q_->stream));
@@ -373,7 +361,7 @@ gpq_status PholdPool::prepare(uint32_t batch) {
   gpq_status st = GPQ_OK;
   for (uint32_t i = 0; i < batch && st == GPQ_OK; ++i) {
     hipLaunchKernelGGL(pool_step_kernel, dim3(ceil_div(capacity_, 256)), dim3(256), 0, q_->stream, v, d_inst_, d_ws_,
-                       d_lp_inst_, d_count_, d_hash_, capacity_, &q_->ws->error);
+                       d_block_win_, d_count_, d_hash_, capacity_, &q_->ws->error);
     st = q_->merge_only();
     if (st == GPQ_OK && far_)
       hipLaunchKernelGGL(pool_far_publish_kernel, dim3(max_inst_), dim3(256), 0, q_->stream, q_->partial,
@@ -436,8 +424,8 @@ gpq_status PholdPool::retire(uint32_t s, uint64_t* trace_hash, uint64_t* events)
   inst_[s].active = 0;
   GPQ_HIP(hipMemcpyAsync(d_inst_ + s, &inst_[s], sizeof(PoolInst), hipMemcpyHostToDevice, q_->stream));
   const uint32_t padded = (in.block_end - in.block_begin) * kBlk;
-  hipLaunchKernelGGL(pool_release_kernel, dim3(ceil_div(padded, 256)), dim3(256), 0, q_->stream, d_lp_inst_, in.base,
-                     padded, q_->view());
+  hipLaunchKernelGGL(pool_release_kernel, dim3(ceil_div(padded, 256)), dim3(256), 0, q_->stream, in.base, padded,
+                     q_->view());
   GPQ_HIP(hipGetLastError());
   GPQ_HIP(hipMemsetAsync(d_block_live_ + in.block_begin, 0, in.block_end - in.block_begin, q_->stream));
   GPQ_HIP(hipStreamSynchronize(q_->stream));

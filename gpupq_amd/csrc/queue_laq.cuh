@@ -34,6 +34,10 @@ struct LaqView {
   // is that of block_ws[block_win[b]]; nullptr = ws (one window).
   const uint32_t* block_win;
   const WindowState* block_ws;
+  // Second staging buffer of the PHOLD merge-and-commit step (nullptr otherwise): a step reads
+  // one buffer while the events it schedules land in the other.
+  Event* stage_alt;
+  uint32_t* ins_alt;
   Event* far;                   // two-tier mode only (far_cap > 0)
   uint32_t* nfar;
   uint32_t* far_min;

@@ -12,133 +12,15 @@
 #include <algorithm>
 #include <cmath>
 
+#include "laq_merge.cuh"
 #include "queues.h"
 
 namespace gpq {
 namespace {
 
+using namespace laq;
+
 constexpr int kLaqBlock = 256;   // LPs per post block = threads per block (4 waves)
-constexpr int kSlots = 8;        // default ring records held per lane: a group of G lanes covers 8G records
-
-struct Key3 {
-  uint32_t ts, src, seq;
-};
-
-__device__ __forceinline__ Key3 shfl_key(const Event& e, int src_lane) {
-  return Key3{(uint32_t)__shfl((int)e.ts, src_lane, kWave), (uint32_t)__shfl((int)e.src, src_lane, kWave),
-              (uint32_t)__shfl((int)e.seq, src_lane, kWave)};
-}
-
-__device__ __forceinline__ bool key_less3(const Key3& a, const Event& b) {
-  if (a.ts != b.ts) return a.ts < b.ts;
-  if (a.src != b.src) return a.src < b.src;
-  return a.seq < b.seq;
-}
-
-__device__ __forceinline__ bool key_less3(const Key3& a, const Key3& b) {
-  if (a.ts != b.ts) return a.ts < b.ts;
-  if (a.src != b.src) return a.src < b.src;
-  return a.seq < b.seq;
-}
-
-// Orders the LDS histogram updates of the lanes of one wave.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Merges the k staged events of one LP into its sorted run of n records at ring slot r,
-// using a group of G lanes (lane g of the group, group starting at wave lane gbase).
-// The run is loaded at once (record j in lane j % G, register slot j / G); staged events
-// are taken G - 1 at a time (one per lane).  Old record j ends at final index j + c_j,
-// c_j = number of new events with a smaller key, and c is nondecreasing in j; the new
-// event of rank a lands at a + #{j : c_j <= a}.  A histogram of c over [0, k] (LDS, one
-// bin per lane) gives both: its inclusive prefix places the new events, and bin a counts
-// the old records that stay put when the run grows by a slots at the front and k - a at
-// the back.  The a with the largest bin is taken, so the fewest records move: a = 0 is
-// the reference's backward insertion (src/sort.cu:329-356), a = k its forward variant
-// (src/sort.cu:358-379), chosen here per LP and step.  Returns the records moved.
-template <int G, int SL>
-__device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, const Event* stg, uint32_t k_total, uint32_t n,
-                                    uint32_t r, int g, int gbase, uint32_t* hist) {
-  const uint32_t mask = (1u << q.cap_log2) - 1u;
-  Event* ring = q.ring + ((size_t)lp << q.cap_log2);
-  uint32_t moved = 0;
-  for (uint32_t done = 0; done < k_total;) {
-    if (done) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");   // previous pass's ring writes
-    const uint32_t k = min(k_total - done, (uint32_t)G - 1u);
-    Event e{kInfBits, 0u, 0u, 0u};
-    if (g < (int)k) e = load_event(&stg[done + g]);
-    const uint32_t ns = (n + G - 1) / G;
-    // ring records of LP lp all carry dst = lp: only the key words are kept
-    Key3 old[SL];
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const uint32_t j = (uint32_t)(s * G + g);
-      old[s] = Key3{0u, 0u, 0u};
-      if ((uint32_t)s < ns && j < n) {
-        const Event o = load_event(&ring[(r + j) & mask]);
-        old[s] = Key3{o.ts, o.src, o.seq};
-      }
-    }
-    // rank of each new event among the new events (keys are unique)
-    uint32_t rank = 0;
-    uint32_t c[SL];
-#pragma unroll
-    for (int s = 0; s < SL; ++s) c[s] = 0;
-    for (uint32_t m = 0; m < k; ++m) {
-      const Key3 nm = shfl_key(e, gbase + (int)m);
-      rank += (g < (int)k && key_less3(nm, Key3{e.ts, e.src, e.seq})) ? 1u : 0u;
-#pragma unroll
-      for (int s = 0; s < SL; ++s)
-        if ((uint32_t)s < ns) c[s] += ((uint32_t)(s * G + g) < n && key_less3(nm, old[s])) ? 1u : 0u;
-    }
-    // histogram of c over [0, k], k < G: bin b in lane b
-    hist[g] = 0;
-    wave_lds_sync();
-#pragma unroll
-    for (int s = 0; s < SL; ++s)
-      if ((uint32_t)s < ns && (uint32_t)(s * G + g) < n) atomicAdd(&hist[c[s]], 1u);
-    wave_lds_sync();
-    const uint32_t h = g <= (int)k ? hist[g] : 0u;
-    wave_lds_sync();
-    // a = argmax over bins (smallest a on ties); inclusive prefix of the bins
-    uint32_t best = g <= (int)k ? ((h << 8) | (255u - (uint32_t)g)) : 0u;
-    uint32_t pre = h;
-#pragma unroll
-    for (int off = 1; off < G; off <<= 1) {
-      const uint32_t o = (uint32_t)__shfl_xor((int)best, off, kWave);
-      best = o > best ? o : best;
-      const uint32_t t = (uint32_t)__shfl((int)pre, gbase + (g >= off ? g - off : g), kWave);
-      if (g >= off) pre += t;
-    }
-    const uint32_t best_a = 255u - (best & 255u), best_stay = best >> 8;
-    const uint32_t below = (uint32_t)__shfl((int)pre, gbase + (int)(rank < k ? rank : 0u), kWave);
-    const uint32_t r_new = (r - best_a) & mask;
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const uint32_t j = (uint32_t)(s * G + g);
-      if ((uint32_t)s < ns && j < n && c[s] != best_a)
-        store_event(&ring[(r_new + j + c[s]) & mask], Event{old[s].ts, lp, old[s].src, old[s].seq});
-    }
-    if (g < (int)k) {
-      const uint32_t f = rank + below;
-      store_event(&ring[(r_new + f) & mask], e);
-      if (f == 0) q.head[lp] = e.ts;
-    }
-    moved += n - best_stay;
-    n += k;
-    r = r_new;
-    done += k;
-  }
-  if (g == 0) {
-    q.rd[lp] = r;
-    q.len[lp] = n;
-    q.ins[lp] = 0;
-  }
-  return moved;
-}
 
 // One block owns kLaqBlock LPs.  Phase 1 reads their counters coalesced into LDS and
 // lists the LPs that received inserts; phase 2 merges them, one LP per group of G
@@ -213,16 +95,15 @@ __global__ __launch_bounds__(kLaqBlock, 5) void laq_post_kernel(LaqView q, uint3
 // staged events -- split at N; those below N are listed in LDS and merged into the ring, the
 // others are written back to the far store (in place, compacted, then the staged ones after
 // them).  Partials per block: ring-head minimum, far minimum, far count.
-constexpr uint32_t kFarList = 64;          // near events one LP may take in one post (ring merge input)
 
 template <int G, int SL>
-__global__ __launch_bounds__(kLaqBlock) void laq_post_far_kernel(LaqView q, uint32_t* partial) {
+__global__ __launch_bounds__(kLaqBlock, 5) void laq_post_far_kernel(LaqView q, uint32_t* partial) {
   __shared__ uint32_t s_work[kLaqBlock];
   __shared__ uint32_t s_k[kLaqBlock], s_n[kLaqBlock], s_r[kLaqBlock], s_f[kLaqBlock];
   __shared__ uint32_t s_hist[kLaqBlock];
   __shared__ uint32_t s_count, s_inserted, s_moved;
   __shared__ unsigned long long s_aux;
-  __shared__ Event s_near[kLaqBlock / G][kFarList];
+  __shared__ Key3 s_near[kLaqBlock / G][kFarList];   // key words only: every record here has dst = its LP
   const uint32_t nb = gridDim.x;
   const uint32_t lp = blockIdx.x * kLaqBlock + threadIdx.x;
   if (q.block_live && !q.block_live[blockIdx.x]) {     // a block of unused LPs: nothing staged, no heads
@@ -268,58 +149,24 @@ __global__ __launch_bounds__(kLaqBlock) void laq_post_far_kernel(LaqView q, uint
     const uint32_t t = s_work[i];
     const uint32_t lpt = blockIdx.x * kLaqBlock + t;
     const uint32_t k = s_k[t], f = s_f[t];
-    const uint32_t cands = (moving ? f : 0u) + k;
-    const uint32_t far_keep0 = moving ? 0u : f;     // far slots kept untouched in front
-    Event* far = q.far + (size_t)lpt * q.far_cap;
     const Event* stg = q.stage + (size_t)lpt * q.stage_cap;
-    uint32_t n_near = 0, n_far = far_keep0;
-    uint32_t fmin = moving ? kInfBits : q.far_min[lpt];
-    for (uint32_t base = 0; base < cands; base += G) {
-      const uint32_t c = base + (uint32_t)g;
-      Event e{kInfBits, 0u, 0u, 0u};
-      const bool valid = c < cands;
-      if (valid) {
-        const uint32_t nf = moving ? f : 0u;
-        e = load_event(c < nf ? &far[c] : &stg[c - nf]);
-      }
-      const bool near = valid && e.ts < N;
-      const bool farv = valid && !near;
-      const uint64_t bn = __ballot(near) & gmask, bf = __ballot(farv) & gmask;
-      const uint64_t below = (1ull << lane) - 1ull;
-      const uint32_t pn = n_near + (uint32_t)__popcll(bn & below);
-      const uint32_t pf = n_far + (uint32_t)__popcll(bf & below);
-      if (near) {
-        if (pn < kFarList) s_near[group][pn] = e;
-        else atomicOr(&q.ws->error, kErrQueueOverflow);
-      }
-      if (farv) {
-        if (pf < q.far_cap) store_event(&far[pf], e);
-        else atomicOr(&q.ws->error, kErrQueueOverflow);
-        fmin = e.ts < fmin ? e.ts : fmin;
-      }
-      n_near += (uint32_t)__popcll(bn);
-      n_far += (uint32_t)__popcll(bf);
-    }
-    // group minimum of fmin
-#pragma unroll
-    for (int off = 1; off < G; off <<= 1) {
-      const uint32_t o = (uint32_t)__shfl_xor((int)fmin, off, kWave);
-      fmin = o < fmin ? o : fmin;
-    }
+    uint32_t n_far, fmin;
+    const uint32_t n_near =
+        split_candidates<G>(q, lpt, stg, k, f, moving != 0u, N, &s_near[group][0], g, gmask, &n_far, &fmin);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (g == 0) {                       // before the merge, so that n_far and fmin die here
+      q.ins[lpt] = 0;
+      q.nfar[lpt] = n_far < q.far_cap ? n_far : q.far_cap;
+      q.far_min[lpt] = fmin;
+    }
     const uint32_t nn = n_near < kFarList ? n_near : kFarList;
     const uint32_t n = s_n[t];
     if (n + nn > (1u << q.cap_log2)) {
       if (g == 0) atomicOr(&q.ws->error, kErrQueueOverflow);
     } else if (nn) {
       moved += laq_merge_group<G, SL>(q, lpt, &s_near[group][0], nn, n, s_r[t], g, gbase, s_hist + group * G);
-    }
-    if (g == 0) {
-      q.ins[lpt] = 0;
-      q.nfar[lpt] = n_far < q.far_cap ? n_far : q.far_cap;
-      q.far_min[lpt] = fmin;
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -364,8 +211,8 @@ __global__ __launch_bounds__(kLaqBlock) void laq_post_far_kernel(LaqView q, uint
 }
 
 // One block: gmin = min(ring heads, far minima); the window ends at
-// min(gmin + lookahead, horizon, near horizon); a window that reaches the near horizon
-// schedules its move to gmin + lookahead + step for the next post.
+// min(gmin + lookahead, horizon, near horizon); when the next window could reach the near
+// horizon, its move to gmin + 2 lookahead + step is scheduled for the next post.
 __global__ void laq_far_publish_kernel(const uint32_t* partial, uint32_t nb, WindowState* ws, float lookahead,
                                        float horizon, float step) {
   __shared__ uint32_t s_a[16], s_b[16], s_c[16];
@@ -402,8 +249,10 @@ __global__ void laq_far_publish_kernel(const uint32_t* partial, uint32_t nb, Win
     if (horizon < wend) wend = horizon;
     const float nearf = bits_to_f(ws->near_end);
     if (nearf < wend) wend = nearf;
-    if (g != kInfBits && gf + lookahead > nearf) {
-      float next = gf + lookahead + step;
+    // scheduled one window ahead: the next window, which can reach gmin + 2 lookahead, then
+    // starts below the moved horizon instead of being cut short at the old one
+    if (g != kInfBits && gf + 2.0f * lookahead > nearf) {
+      float next = gf + 2.0f * lookahead + step;
       if (!(next > nearf)) next = nearf + step;
       ws->near_next = f_to_bits(next);
     }
@@ -413,6 +262,43 @@ __global__ void laq_far_publish_kernel(const uint32_t* partial, uint32_t nb, Win
     ws->windows += 1;
     (void)c;
   }
+}
+
+// queue_post after a model step that merged and committed in one kernel (the PHOLD step of
+// drain mode 2, which owns the second staging buffer): the window advances by the lookahead (capped at the end
+// time); a window that reached the end time ends the run; a window beyond the near horizon
+// schedules the horizon's move (by `step`) for the next step; the staging buffers swap.
+// settle: only the buffer choice is reset (both buffers are empty after a settle pass).
+__global__ void laq_step_publish_kernel(WindowState* ws, float lookahead, float end_time, float step,
+                                        uint32_t settle) {
+  WindowState w = *ws;
+  if (settle) {
+    ws->sel = 0;
+    return;
+  }
+  if (w.done) return;
+  if (w.near_next) {
+    w.near_end = w.near_next;
+    w.near_next = 0;
+  }
+  const float we = bits_to_f(w.wend);
+  w.windows += 1;
+  w.sel ^= 1u;
+  w.gmin = w.wend;                                // a lower bound of the pending minimum
+  if (!(we < end_time)) {
+    w.done = 1;
+  } else {
+    float wn = we + lookahead;
+    if (end_time < wn) wn = end_time;
+    const float nf = bits_to_f(w.near_end);
+    if (wn > nf) {
+      float nx = nf + step;
+      if (!(nx >= wn)) nx = wn + step;
+      w.near_next = f_to_bits(nx);
+    }
+    w.wend = f_to_bits(wn);
+  }
+  *ws = w;
 }
 
 __global__ void laq_init_kernel(LaqView q) {
@@ -435,6 +321,7 @@ __global__ void laq_init_kernel(LaqView q) {
     q.ws->near_next = 0;
     q.ws->windows = 0;
     q.ws->error = 0;
+    q.ws->sel = 0;
   }
 }
 
@@ -486,6 +373,24 @@ gpq_status LaqQueue::init() {
 }
 
 gpq_status LaqQueue::post() {
+  if (v_.stage_alt) {        // the model step merged and committed: only the window moves
+    if (profile_mark) GPQ_HIP(hipEventRecord(profile_mark, stream));
+    hipLaunchKernelGGL(laq_step_publish_kernel, dim3(1), dim3(1), 0, stream, ws, cfg.lookahead, cfg.horizon,
+                       far_step_, 0u);
+    GPQ_HIP(hipGetLastError());
+    return GPQ_OK;
+  }
+  return merge_publish();
+}
+
+gpq_status LaqQueue::reset_stage_select() {
+  hipLaunchKernelGGL(laq_step_publish_kernel, dim3(1), dim3(1), 0, stream, ws, cfg.lookahead, cfg.horizon, far_step_,
+                     1u);
+  GPQ_HIP(hipGetLastError());
+  return GPQ_OK;
+}
+
+gpq_status LaqQueue::merge_publish() {
   GPQ_TRY(merge_only());
   if (!v_.far_cap) return publish_window();
   if (profile_mark) GPQ_HIP(hipEventRecord(profile_mark, stream));
@@ -518,12 +423,26 @@ gpq_status LaqQueue::merge_only() {
   return GPQ_OK;
 }
 
+gpq_status LaqQueue::enable_alt_stage() {
+  if (v_.stage_alt) return GPQ_OK;
+  GPQ_TRY(arena.alloc(&v_.stage_alt, (size_t)cfg.num_lps * v_.stage_cap));
+  GPQ_TRY(arena.alloc(&v_.ins_alt, cfg.num_lps));
+  GPQ_HIP(hipMemsetAsync(v_.ins_alt, 0, sizeof(uint32_t) * cfg.num_lps, stream));
+  return GPQ_OK;
+}
+
 gpq_status LaqQueue::pending(uint64_t* n) {
   GPQ_TRY(sum_u32(v_.len, cfg.num_lps, sum_, n));
+  uint64_t x = 0;
   if (v_.far_cap) {
-    uint64_t f = 0;
-    GPQ_TRY(sum_u32(v_.nfar, cfg.num_lps, sum_, &f));
-    *n += f;
+    GPQ_TRY(sum_u32(v_.nfar, cfg.num_lps, sum_, &x));
+    *n += x;
+  }
+  if (v_.ins_alt) {          // a merge-and-commit step leaves the events it scheduled staged
+    GPQ_TRY(sum_u32(v_.ins, cfg.num_lps, sum_, &x));
+    *n += x;
+    GPQ_TRY(sum_u32(v_.ins_alt, cfg.num_lps, sum_, &x));
+    *n += x;
   }
   return GPQ_OK;
 }
@@ -559,16 +478,30 @@ gpq_status LaqQueue::lp_contents(uint32_t lp, Event* out, uint32_t cap, uint32_t
     GPQ_HIP(hipMemcpyAsync(far.data(), v_.far + (size_t)lp * v_.far_cap, sizeof(Event) * v_.far_cap,
                            hipMemcpyDeviceToHost, stream));
   }
+  uint32_t ni[2] = {0, 0};
+  std::vector<Event> stg[2];
+  if (v_.ins_alt) {          // staged events of a merge-and-commit step, in either buffer
+    const Event* bufs[2] = {v_.stage, v_.stage_alt};
+    const uint32_t* cnts[2] = {v_.ins, v_.ins_alt};
+    for (int b = 0; b < 2; ++b) {
+      stg[b].resize(v_.stage_cap);
+      GPQ_HIP(hipMemcpyAsync(&ni[b], cnts[b] + lp, 4, hipMemcpyDeviceToHost, stream));
+      GPQ_HIP(hipMemcpyAsync(stg[b].data(), bufs[b] + (size_t)lp * v_.stage_cap, sizeof(Event) * v_.stage_cap,
+                             hipMemcpyDeviceToHost, stream));
+    }
+  }
   GPQ_HIP(hipStreamSynchronize(stream));
   std::vector<Event> all;
   for (uint32_t i = 0; i < len; ++i) all.push_back(ring[(rd + i) & (C - 1)]);
   std::vector<Event> rest(far.begin(), far.begin() + std::min(nf, v_.far_cap));   // far store: unsorted
-  std::sort(rest.begin(), rest.end(), [](const Event& a, const Event& b) {
+  for (int b = 0; b < 2; ++b)
+    rest.insert(rest.end(), stg[b].begin(), stg[b].begin() + std::min(ni[b], v_.stage_cap));
+  all.insert(all.end(), rest.begin(), rest.end());
+  std::sort(all.begin(), all.end(), [](const Event& a, const Event& b) {   // the ring part is sorted already
     if (a.ts != b.ts) return a.ts < b.ts;
     if (a.src != b.src) return a.src < b.src;
     return a.seq < b.seq;
   });
-  all.insert(all.end(), rest.begin(), rest.end());
   *n = (uint32_t)all.size();
   for (uint32_t i = 0; i < *n && i < cap; ++i) out[i] = all[i];
   return GPQ_OK;

@@ -18,17 +18,29 @@ class LaqQueue final : public QueueBase {
   explicit LaqQueue(const gpq_queue_config& c) : QueueBase(c) {}
   gpq_status init() override;
   gpq_status post() override;
+  gpq_status post_init() override { return merge_publish(); }
   gpq_status lp_contents(uint32_t lp, Event* out, uint32_t cap, uint32_t* n) override;
   gpq_status pending(uint64_t* n) override;
   gpq_status length(uint32_t lp, uint32_t* n) override;
   gpq_status clear_lp(uint32_t lp) override;
   LaqView view() const { return v_; }
+  // Merge of the staged inserts plus the window publication (queue_post without a model
+  // step that merges by itself).
+  gpq_status merge_publish();
   // The merge kernel of post() alone: one partial minimum per 256-LP block in `partial`,
   // no window publication (callers that keep several windows publish their own).
   gpq_status merge_only();
   static constexpr uint32_t kBlockLps = 256;
   // Post blocks whose flag is 0 are skipped (their LPs hold nothing); nullptr = all live.
   void set_block_live(const uint8_t* flags) { v_.block_live = flags; }
+  // Allocates the second staging buffer (stage_alt / ins_alt, empty) that a model step which
+  // merges and commits in one kernel needs; idempotent.  From then on post() only advances
+  // the window by the lookahead (the step did the merge), post_init() still merges.
+  gpq_status enable_alt_stage();
+  // After a closing merge of such a step: both buffers are empty, the next step reads stage.
+  gpq_status reset_stage_select();
+  uint32_t group() const { return group_; }
+  float far_step() const { return far_step_; }
   // Two-tier mode, several windows: block b follows the near horizon of ws_array[win[b]].
   void set_block_windows(const uint32_t* win, const WindowState* ws_array) {
     v_.block_win = win;

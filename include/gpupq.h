@@ -143,7 +143,12 @@ typedef struct {
   uint32_t drain;          /* 0: one safe event per LP per step, as src/phold.cu:81-117 does;
                               1: each LP commits all of its events below the window end in key
                               order in one step (per-LP queues only).  Events created in a step
-                              lie at or beyond the window end, so the committed trace is the same. */
+                              lie at or beyond the window end, so the committed trace is the same;
+                              2: as 1, with the queue's merge done inside the model step (one kernel
+                              per step) and each window ending one lookahead after the previous one;
+                              needs the two-tier LAQ with capacity_per_lp <= 64.  queue_post then
+                              only moves the window; events inserted from outside the model step
+                              are not supported in this mode.  Same committed trace. */
 } gpq_phold_config;
 
 gpq_status gpq_phold_create(gpq_queue* q, const gpq_phold_config* cfg, gpq_phold** out);

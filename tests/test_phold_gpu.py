@@ -67,12 +67,18 @@ def test_device_philox_bit_exact(gpq, oracle):
 LOCAL = ["laq", "lhq", "lsq", "laq2"]
 
 
-@pytest.mark.parametrize("drain", [False, True])
+def _skip_drain(kind, drain):
+    if drain and kind not in LOCAL:
+        pytest.skip("drain mode needs a per-LP queue")
+    if drain == "fused" and kind != "laq2":
+        pytest.skip("the merge-and-commit step needs the two-tier LAQ")
+
+
+@pytest.mark.parametrize("drain", [False, True, "fused"])
 @pytest.mark.parametrize("kind", sorted(QUEUE_ARGS))
 @pytest.mark.parametrize("L,k,T", [(1, 1, 20.0), (4, 2, 5.0), (64, 8, 20.0), (97, 3, 7.5), (4096, 16, 20.0)])
 def test_phold_matches_oracle_per_lp(gpq, oracle, kind, L, k, T, drain):
-    if drain and kind not in LOCAL:
-        pytest.skip("drain mode needs a per-LP queue")
+    _skip_drain(kind, drain)
     res, cnt, hsh = oracle.phold("seq", L, k, T)
     g = run_gpu(gpq, kind, L, k, T, drain=drain)
     assert g["flags"] == 0
@@ -111,6 +117,43 @@ def test_phold_plain_equals_graph(gpq):
     assert b["rep"]["windows"] == b["rep"]["steps"] + 16 and b["rep"]["steps"] % 7 == 0
 
 
+def test_phold_fused_plain_graph_and_windows(gpq, golden):
+    """The merge-and-commit step: host-driven steps, graph replays of an odd batch and the
+    drain-mode run agree; each window ends one lookahead after the previous one."""
+    gd = golden["phold_1024_k16_t100"]
+    a = run_gpu(gpq, "laq2", 1024, 16, 100.0, mode="plain", drain="fused")
+    b = run_gpu(gpq, "laq2", 1024, 16, 100.0, mode="graph", batch=7, drain="fused")
+    c = run_gpu(gpq, "laq2", 1024, 16, 100.0, mode="graph", drain=True)
+    for g in (a, b):
+        assert g["flags"] == 0 and g["pending"] == 1024 * 16
+        assert "%016x" % g["hash"] == gd["trace_hash"] == "%016x" % c["hash"]
+        assert [int(x) for x in g["cnt"]] == gd["lp_count"]
+    # 16 populate windows, then about (100 - first window end) / 0.1 steps
+    assert abs((a["rep"]["windows"] - 16) - 100.0 / 0.1) <= 3
+    assert b["rep"]["windows"] == a["rep"]["windows"]
+
+
+def test_phold_fused_contents_between_steps(gpq, oracle):
+    """Between host-driven steps the queue reports every pending event, staged ones included."""
+    L, k, T = 64, 8, 3.0
+    q = gpq.Queue("laq", num_lps=L, lookahead=0.1, horizon=T, **dict(QUEUE_ARGS["laq2"], stage_per_lp=32))
+    m = gpq.Phold(q, init_per_lp=k, seed=42, end_time=T, drain="fused")
+    m.model_init()
+    for _ in range(5):
+        m.model_handle_next()
+        q.queue_post()
+        assert q.queue_check_phold() == L * k
+        c = q.lp_contents(3)
+        ts = [gpq.bits_f32(e["ts"]) for e in c]
+        assert ts == sorted(ts)
+    rep = gpq.des_run(m, mode="plain")
+    res, cnt, hsh = oracle.phold("seq", L, k, T)
+    assert rep["events"] == res.events and m.trace_hash() == res.trace_hash
+    assert q.queue_check_phold() == L * k
+    m.model_finish()
+    q.queue_finish()
+
+
 @pytest.mark.parametrize("kind", sorted(QUEUE_ARGS))
 def test_phold_65536_lps_matches_golden(gpq, golden, kind):
     gd = golden["mid_l65536_k32_t10"]
@@ -120,11 +163,10 @@ def test_phold_65536_lps_matches_golden(gpq, golden, kind):
     assert g["pending"] == 65536 * 32
 
 
-@pytest.mark.parametrize("drain", [False, True])
+@pytest.mark.parametrize("drain", [False, True, "fused"])
 @pytest.mark.parametrize("kind", sorted(QUEUE_ARGS))
 def test_phold_2p20_lps_matches_golden(gpq, golden, kind, drain):
-    if drain and kind not in LOCAL:
-        pytest.skip("drain mode needs a per-LP queue")
+    _skip_drain(kind, drain)
     gd = golden.get("phold_2p20_k32_t50")
     if gd is None:
         pytest.skip("2^20-LP golden record not generated")
