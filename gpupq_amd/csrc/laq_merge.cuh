@@ -201,31 +201,42 @@ __device__ __forceinline__ uint32_t split_candidates(const LaqView& q, uint32_t 
   Event* far = q.far + (size_t)lp * q.far_cap;
   uint32_t n_near = 0, n_far = moving ? 0u : f;
   uint32_t fmin = moving ? kInfBits : q.far_min[lp];
-  for (uint32_t base = 0; base < cands; base += G) {
-    const uint32_t c = base + (uint32_t)g;
-    Event e{kInfBits, 0u, 0u, 0u};
-    const bool valid = c < cands;
-    if (valid) {
-      const uint32_t nf = moving ? f : 0u;
-      e = load_event(c < nf ? &far[c] : &stg[c - nf]);
+  // kChunk candidates per lane are loaded before any is placed, so their latencies overlap.
+  // In-place compaction stays safe: a far record is written to an index no larger than the
+  // candidate index being placed, and every candidate of the chunk is already loaded.
+  constexpr uint32_t kChunk = 4;
+  const uint32_t nf = moving ? f : 0u;
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t base = 0; base < cands; base += kChunk * G) {
+    Event ev[kChunk];
+#pragma unroll
+    for (uint32_t u = 0; u < kChunk; ++u) {
+      const uint32_t c = base + u * G + (uint32_t)g;
+      ev[u] = Event{kInfBits, 0u, 0u, 0u};
+      if (c < cands) ev[u] = load_event(c < nf ? &far[c] : &stg[c - nf]);
     }
-    const bool is_near = valid && e.ts < N;
-    const bool is_far = valid && !is_near;
-    const uint64_t bn = __ballot(is_near) & gmask, bf = __ballot(is_far) & gmask;
-    const uint64_t below = (1ull << lane) - 1ull;
-    const uint32_t pn = n_near + (uint32_t)__popcll(bn & below);
-    const uint32_t pf = n_far + (uint32_t)__popcll(bf & below);
-    if (is_near) {
-      if (pn < kFarList) near[pn] = Key3{e.ts, e.src, e.seq};
-      else atomicOr(&q.ws->error, kErrQueueOverflow);
+#pragma unroll
+    for (uint32_t u = 0; u < kChunk; ++u) {
+      if (base + u * G >= cands) break;       // uniform across the group
+      const Event& e = ev[u];
+      const bool valid = base + u * G + (uint32_t)g < cands;
+      const bool is_near = valid && e.ts < N;
+      const bool is_far = valid && !is_near;
+      const uint64_t bn = __ballot(is_near) & gmask, bf = __ballot(is_far) & gmask;
+      const uint32_t pn = n_near + (uint32_t)__popcll(bn & below);
+      const uint32_t pf = n_far + (uint32_t)__popcll(bf & below);
+      if (is_near) {
+        if (pn < kFarList) near[pn] = Key3{e.ts, e.src, e.seq};
+        else atomicOr(&q.ws->error, kErrQueueOverflow);
+      }
+      if (is_far) {
+        if (pf < q.far_cap) store_event(&far[pf], e);
+        else atomicOr(&q.ws->error, kErrQueueOverflow);
+        fmin = e.ts < fmin ? e.ts : fmin;
+      }
+      n_near += (uint32_t)__popcll(bn);
+      n_far += (uint32_t)__popcll(bf);
     }
-    if (is_far) {
-      if (pf < q.far_cap) store_event(&far[pf], e);
-      else atomicOr(&q.ws->error, kErrQueueOverflow);
-      fmin = e.ts < fmin ? e.ts : fmin;
-    }
-    n_near += (uint32_t)__popcll(bn);
-    n_far += (uint32_t)__popcll(bf);
   }
 #pragma unroll
   for (int off = 1; off < G; off <<= 1) {

@@ -230,33 +230,50 @@ __global__ __launch_bounds__(kFusedBlock, 4) void phold_fused_kernel(LaqView q, 
     laq::wave_lds_sync();                         // near[] and hist are free for the group's next LP
   }
   __syncthreads();                                // every merged run is in the ring
-  // commits: one thread per listed LP, its d records from ring slot `first` on, in key order
+  // commits: one thread per listed LP, its d records from ring slot `first` on, in key order,
+  // kCommitBatch at a time: the destinations depend only on the LP's event counter, so the
+  // batch's staging-slot atomics are issued together and their latencies overlap
+  constexpr uint32_t kCommitBatch = 4;
   if (threadIdx.x < s_count) {
     const uint32_t i = threadIdx.x;
     const uint32_t dm = s_meta[i], d = dm & 255u;
     if (d) {
       const uint32_t lpt = blockIdx.x * kFusedBlock + ((s_rt[i] >> 8) & 255u);
       const Event* ring = q.ring + ((size_t)lpt << q.cap_log2);
-      uint32_t pos = dm >> 8;
+      const uint32_t first = dm >> 8;
       const uint32_t cnt = s_cnt[i];
       uint64_t h = s_hash[i];
       uint32_t last = s_last[i];
-      Event cur = load_event(&ring[pos]);
-      for (uint32_t j = 0; j < d; ++j) {
-        pos = (pos + 1u) & mask;
-        const Event nxt = j + 1u < d ? load_event(&ring[pos]) : Event{kInfBits, 0u, 0u, 0u};
-        h = trace_step(h, cur.ts, cur.src, cur.seq);
-        if (m.last_ts) {
-          if (cur.ts < last) atomicOr(&q.ws->error, kErrBadState);
-          last = cur.ts;
-        }
-        const uint32_t nc = cnt + j;
-        const uint4 rr = philox4x32_10(make_uint4(nc, lpt, 0u, 0u), make_uint2(m.seed, 0u));
-        const float ex = exp_from_u32(rr.x);
-        const uint32_t dst = rr.y < m.p_self ? lpt : (uint32_t)(((uint64_t)rr.z * m.num_lps) >> 32);
-        const float t_new = (bits_to_f(cur.ts) + m.lookahead) + ex;
-        stage_insert(stg_out, ins_out, q.stage_cap, q.ws, Event{f_to_bits(t_new), dst, lpt, m.init_per_lp + nc});
-        cur = nxt;
+      for (uint32_t j0 = 0; j0 < d; j0 += kCommitBatch) {
+        Event rec[kCommitBatch];
+        uint32_t draw[kCommitBatch], dst[kCommitBatch], slot[kCommitBatch];
+#pragma unroll
+        for (uint32_t b = 0; b < kCommitBatch; ++b)
+          if (j0 + b < d) rec[b] = load_event(&ring[(first + j0 + b) & mask]);
+#pragma unroll
+        for (uint32_t b = 0; b < kCommitBatch; ++b)
+          if (j0 + b < d) {
+            const uint4 rr = philox4x32_10(make_uint4(cnt + j0 + b, lpt, 0u, 0u), make_uint2(m.seed, 0u));
+            draw[b] = rr.x;
+            dst[b] = rr.y < m.p_self ? lpt : (uint32_t)(((uint64_t)rr.z * m.num_lps) >> 32);
+            slot[b] = atomicAdd(&ins_out[dst[b]], 1u);
+          }
+#pragma unroll
+        for (uint32_t b = 0; b < kCommitBatch; ++b)
+          if (j0 + b < d) {
+            const Event& cur = rec[b];
+            h = trace_step(h, cur.ts, cur.src, cur.seq);
+            if (m.last_ts) {
+              if (cur.ts < last) atomicOr(&q.ws->error, kErrBadState);
+              last = cur.ts;
+            }
+            const float t_new = (bits_to_f(cur.ts) + m.lookahead) + exp_from_u32(draw[b]);
+            if (slot[b] < q.stage_cap)
+              store_event(&stg_out[(size_t)dst[b] * q.stage_cap + slot[b]],
+                          Event{f_to_bits(t_new), dst[b], lpt, m.init_per_lp + cnt + j0 + b});
+            else
+              atomicOr(&q.ws->error, kErrStageOverflow);
+          }
       }
       m.count[lpt] = cnt + d;
       m.hash[lpt] = h;
