@@ -48,6 +48,25 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Loads the run of n records of LP lp at ring slot r for laq_merge_pass<..., kPre = true>:
+// record j into lane j % G of the group, register slot j / G (key words only).
+template <int G, int SL>
+__device__ __forceinline__ void load_run(const LaqView& q, uint32_t lp, uint32_t n, uint32_t r, int g,
+                                         Key3* old) {
+  const uint32_t mask = (1u << q.cap_log2) - 1u;
+  const Event* ring = q.ring + ((size_t)lp << q.cap_log2);
+  const uint32_t ns = (n + G - 1) / G;
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    const uint32_t j = (uint32_t)(s * G + g);
+    old[s] = Key3{0u, 0u, 0u};
+    if ((uint32_t)s < ns && j < n) {
+      const Event o = load_event(&ring[(r + j) & mask]);
+      old[s] = Key3{o.ts, o.src, o.seq};
+    }
+  }
+}
+
 // One merge pass: the k <= G - 1 staged events stg[0 .. k) of LP lp go into its sorted run of
 // n records at ring slot r, by a group of G lanes (lane g of the group, group starting at wave
 // lane gbase).  The run is loaded at once (record j in lane j % G, register slot j / G), one
@@ -65,105 +84,31 @@ __device__ __forceinline__ void wave_lds_sync() {
 // the record at index d, and the taken records stay in the ring slots just before the new
 // start for the caller to commit.  Without kDrain the head changes only when a new event
 // lands at index 0.
-template <int G, int SL, bool kDrain, typename Stg>
+//
+// The pass exists twice with one body (laq_merge_pass.inc): laq_merge_pass, an ordinary function
+// (the post kernels, whose registers stay within the five-blocks-per-CU bound this way), and
+// laq_merge_pass_inl, force-inlined (the merge-and-commit kernel, measured faster this way).
+//
+// kPre: the run is not loaded here but taken from *preloaded, filled by load_run (with the same n
+// and r) before the caller's other loads, so that their latencies overlap.
+template <int SL>
+struct Run {
+  Key3 k[SL];
+};
+
+template <int G, int SL, bool kDrain, typename Stg, bool kPre = false>
 __device__ uint32_t laq_merge_pass(const LaqView& q, uint32_t lp, const Stg* stg, uint32_t k,
                                     uint32_t& n, uint32_t& r, int g, int gbase, uint32_t* hist,
-                                    uint32_t wend, uint32_t* d_out) {
-  const uint32_t mask = (1u << q.cap_log2) - 1u;
-  Event* ring = q.ring + ((size_t)lp << q.cap_log2);
-  Event e{kInfBits, 0u, 0u, 0u};
-  if (g < (int)k) e = staged_at(stg, (uint32_t)g, lp);
-  const uint32_t ns = (n + G - 1) / G;
-  // ring records of LP lp all carry dst = lp: only the key words are kept
-  Key3 old[SL];
-#pragma unroll
-  for (int s = 0; s < SL; ++s) {
-    const uint32_t j = (uint32_t)(s * G + g);
-    old[s] = Key3{0u, 0u, 0u};
-    if ((uint32_t)s < ns && j < n) {
-      const Event o = load_event(&ring[(r + j) & mask]);
-      old[s] = Key3{o.ts, o.src, o.seq};
-    }
-  }
-  // rank of each new event among the new events (keys are unique)
-  uint32_t rank = 0;
-  uint32_t c[SL];
-#pragma unroll
-  for (int s = 0; s < SL; ++s) c[s] = 0;
-  for (uint32_t m = 0; m < k; ++m) {
-    const Key3 nm = shfl_key(e, gbase + (int)m);
-    rank += (g < (int)k && key_less3(nm, Key3{e.ts, e.src, e.seq})) ? 1u : 0u;
-#pragma unroll
-    for (int s = 0; s < SL; ++s)
-      if ((uint32_t)s < ns) c[s] += ((uint32_t)(s * G + g) < n && key_less3(nm, old[s])) ? 1u : 0u;
-  }
-  // histogram of c over [0, k], k < G: bin b in lane b
-  hist[g] = 0;
-  wave_lds_sync();
-#pragma unroll
-  for (int s = 0; s < SL; ++s)
-    if ((uint32_t)s < ns && (uint32_t)(s * G + g) < n) atomicAdd(&hist[c[s]], 1u);
-  wave_lds_sync();
-  const uint32_t h = g <= (int)k ? hist[g] : 0u;
-  wave_lds_sync();
-  // a = argmax over bins (smallest a on ties); inclusive prefix of the bins
-  uint32_t best = g <= (int)k ? ((h << 8) | (255u - (uint32_t)g)) : 0u;
-  uint32_t pre = h;
-#pragma unroll
-  for (int off = 1; off < G; off <<= 1) {
-    const uint32_t o = (uint32_t)__shfl_xor((int)best, off, kWave);
-    best = o > best ? o : best;
-    const uint32_t t = (uint32_t)__shfl((int)pre, gbase + (g >= off ? g - off : g), kWave);
-    if (g >= off) pre += t;
-  }
-  const uint32_t best_a = 255u - (best & 255u), best_stay = best >> 8;
-  const uint32_t below = (uint32_t)__shfl((int)pre, gbase + (int)(rank < k ? rank : 0u), kWave);
-  const uint32_t r_new = (r - best_a) & mask;
-  if constexpr (!kDrain) {
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const uint32_t j = (uint32_t)(s * G + g);
-      if ((uint32_t)s < ns && j < n && c[s] != best_a)
-        store_event(&ring[(r_new + j + c[s]) & mask], Event{old[s].ts, lp, old[s].src, old[s].seq});
-    }
-    if (g < (int)k) {
-      const uint32_t f = rank + below;
-      store_event(&ring[(r_new + f) & mask], e);
-      if (f == 0) q.head[lp] = e.ts;
-    }
-    n += k;
-    r = r_new;
-    return n - k - best_stay;
-  } else {
-    // d = records below wend, a prefix of the merged run
-    uint32_t lt = (g < (int)k && e.ts < wend) ? 1u : 0u;
-#pragma unroll
-    for (int s = 0; s < SL; ++s)
-      if ((uint32_t)s < ns && (uint32_t)(s * G + g) < n && old[s].ts < wend) ++lt;
-#pragma unroll
-    for (int off = 1; off < G; off <<= 1) lt += (uint32_t)__shfl_xor((int)lt, off, kWave);
-    const uint32_t d = lt, total = n + k;
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-      const uint32_t j = (uint32_t)(s * G + g);
-      if ((uint32_t)s < ns && j < n) {
-        const uint32_t fi = j + c[s];
-        if (c[s] != best_a)
-          store_event(&ring[(r_new + fi) & mask], Event{old[s].ts, lp, old[s].src, old[s].seq});
-        if (fi == d) q.head[lp] = old[s].ts;
-      }
-    }
-    if (g < (int)k) {
-      const uint32_t fi = rank + below;
-      store_event(&ring[(r_new + fi) & mask], e);
-      if (fi == d) q.head[lp] = e.ts;
-    }
-    if (g == 0 && d == total) q.head[lp] = kInfBits;
-    n = total - d;
-    r = (r_new + d) & mask;
-    *d_out = d;
-    return total - k - best_stay;
-  }
+                                    uint32_t wend, uint32_t* d_out, const Run<SL>* preloaded = nullptr) {
+#include "laq_merge_pass.inc"
+}
+
+template <int G, int SL, bool kDrain, typename Stg, bool kPre = false>
+__device__ __forceinline__ uint32_t laq_merge_pass_inl(const LaqView& q, uint32_t lp, const Stg* stg, uint32_t k,
+                                                       uint32_t& n, uint32_t& r, int g, int gbase, uint32_t* hist,
+                                                       uint32_t wend, uint32_t* d_out,
+                                                       const Run<SL>* preloaded = nullptr) {
+#include "laq_merge_pass.inc"
 }
 
 // Merges the k_total staged events of one LP into its sorted run of n records at ring slot r,

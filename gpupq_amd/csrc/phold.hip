@@ -164,17 +164,21 @@ __global__ __launch_bounds__(kFusedBlock, 4) void phold_fused_kernel(LaqView q, 
   }
   __syncthreads();
   if (lp < q.num_lps) {
+    // every word in one round trip (almost every LP is listed in a PHOLD step)
     const uint32_t staged = ins_in[lp];
     const uint32_t f = q.nfar[lp];
     const uint32_t drainable = q.head[lp] < wend ? 1u : 0u;
+    const uint32_t len = q.len[lp], rd = q.rd[lp], cnt = m.count[lp];
+    const uint64_t hash = m.hash[lp];
+    const uint32_t last = m.last_ts ? m.last_ts[lp] : 0u;
     if (staged || (moving && f) || drainable) {
       const uint32_t slot = atomicAdd(&s_count, 1u);
       const uint32_t k = staged < q.stage_cap ? staged : q.stage_cap;
-      s_meta[slot] = k | (q.len[lp] << 7) | (f << 14);
-      s_rt[slot] = q.rd[lp] | (threadIdx.x << 8) | (drainable << 16);
-      s_cnt[slot] = m.count[lp];
-      s_hash[slot] = m.hash[lp];
-      if (m.last_ts) s_last[slot] = m.last_ts[lp];
+      s_meta[slot] = k | (len << 7) | (f << 14);
+      s_rt[slot] = rd | (threadIdx.x << 8) | (drainable << 16);
+      s_cnt[slot] = cnt;
+      s_hash[slot] = hash;
+      s_last[slot] = last;
       if (q.stats) {
         atomicAdd(&s_inserted, staged);
         if (moving && f) atomicAdd(&s_aux, 32ull * f);   // far records read and written once
@@ -197,6 +201,8 @@ __global__ __launch_bounds__(kFusedBlock, 4) void phold_fused_kernel(LaqView q, 
     uint32_t n = (meta >> 7) & 127u, r = rt & 255u;
     const uint32_t lpt = blockIdx.x * kFusedBlock + ((rt >> 8) & 255u);
     uint32_t nn = 0, d = 0;
+    laq::Run<SL> old;                             // the run's loads overlap the candidates' loads
+    laq::load_run<G, SL>(q, lpt, n, r, g, old.k);
     if (k || (moving && f)) {
       uint32_t n_far, fmin;
       const uint32_t n_near = laq::split_candidates<G>(q, lpt, stg_in + (size_t)lpt * q.stage_cap, k, f, moving != 0u,
@@ -214,13 +220,20 @@ __global__ __launch_bounds__(kFusedBlock, 4) void phold_fused_kernel(LaqView q, 
     } else if (nn || (rt >> 16)) {               // something to merge or to commit
       uint32_t done = 0;
       while (nn - done > (uint32_t)G - 1u) {
-        if (done) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");   // previous pass's ring writes
-        moved += laq::laq_merge_pass<G, SL, false>(q, lpt, near + done, (uint32_t)G - 1u, n, r, g, gbase, hist, 0u,
-                                                   nullptr);
+        if (done) {                               // the previous pass's ring writes, then the new run
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+          laq::load_run<G, SL>(q, lpt, n, r, g, old.k);
+        }
+        moved += laq::laq_merge_pass_inl<G, SL, false, laq::Key3, true>(q, lpt, near + done, (uint32_t)G - 1u, n, r, g,
+                                                                    gbase, hist, 0u, nullptr, &old);
         done += (uint32_t)G - 1u;
       }
-      if (done) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-      moved += laq::laq_merge_pass<G, SL, true>(q, lpt, near + done, nn - done, n, r, g, gbase, hist, wend, &d);
+      if (done) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        laq::load_run<G, SL>(q, lpt, n, r, g, old.k);
+      }
+      moved += laq::laq_merge_pass_inl<G, SL, true, laq::Key3, true>(q, lpt, near + done, nn - done, n, r, g, gbase, hist,
+                                                                 wend, &d, &old);
       if (g == 0) {
         q.rd[lpt] = r;
         q.len[lpt] = n;
