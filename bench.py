@@ -16,7 +16,7 @@ Beside the JSON line's value:
                 the same workload (end time 0.5), one host core
   phold_batch   32 PHOLD instances of uneven size (2^U[12,18] LPs) in one pool, against
                 each alone with a host sync per step
-  astar_single  the 4096 x 4096 grid query over 65536 open lists (a local heap queue)
+  astar_single  the 4096 x 4096 grid query: tile search, and 65536 open lists in a local heap queue
   astar_batch   512 synthetic grid queries (side 2^U[8,11], density U[0.1,0.3],
                 8-connected) in one persistent launch: queries/s, p50/p99 completion
                 time, against the plain one-query-per-launch path (host sync per
@@ -233,23 +233,34 @@ def phold_batch_bench(gpq, log, instances=32, seed=5):
     return out
 
 
-def astar_single_bench(gpq, log, cpu=True, lists=1 << 16, window=0):
-    """One 4096 x 4096, 4-connected, Bernoulli(0.2) seed 7 query, (0,0) -> (4095,4095), its open
-    lists the LPs of a local heap queue; against the oracle's A* on one core."""
+def astar_single_bench(gpq, log, cpu=True, tile=32):
+    """One 4096 x 4096, 4-connected, Bernoulli(0.2) seed 7 query, (0,0) -> (4095,4095): the tile
+    search (one workgroup per tile x tile square) and, for comparison, 65536 open lists in a
+    local heap queue; against the oracle's A* on one core."""
     w = h = 4096
     inf = float("inf")
-    q = gpq.Queue("lhq", num_lps=lists, capacity_per_lp=64, stage_per_lp=16, lookahead=inf, horizon=inf)
-    a = gpq.Astar(q, w, h, connectivity=4, seed=7, threshold=0.2, window=window)
-    a.model_init()
-    a.run("graph", 32)                                   # warm up (graph capture)
-    a.model_init()
-    rep = a.run("graph", 32)
-    grid = a.grid() if cpu else None
-    a.model_finish()
-    q.queue_finish()
-    out = {"grid": "4096x4096, 4-connected, Bernoulli(0.2), seed 7", "open_lists": lists, "queue": "lhq",
+
+    def one(lists, tile_side):
+        q = gpq.Queue("lhq", num_lps=lists, capacity_per_lp=64, stage_per_lp=16, lookahead=inf, horizon=inf)
+        a = gpq.Astar(q, w, h, connectivity=4, seed=7, threshold=0.2, tile=tile_side)
+        a.model_init()
+        a.run("graph", 32)                               # warm up (graph capture)
+        a.model_init()
+        rep = a.run("graph", 32)
+        grid = a.grid() if cpu else None
+        a.model_finish()
+        q.queue_finish()
+        return rep, grid
+
+    rep, grid = one(64, tile)
+    rep_q, _ = one(1 << 16, 0)
+    out = {"grid": "4096x4096, 4-connected, Bernoulli(0.2), seed 7", "search": "tiles %dx%d" % (tile, tile),
            "cost": rep["cost"], "seconds": rep["seconds"], "steps": rep["steps"], "expanded": rep["expanded"],
-           "expanded_per_s": rep["expanded"] / rep["seconds"] if rep["seconds"] > 0 else 0.0}
+           "closed": rep["closed"], "expanded_per_s": rep["expanded"] / rep["seconds"] if rep["seconds"] > 0 else 0.0,
+           "queue_lists": {"open_lists": 1 << 16, "queue": "lhq", "cost": rep_q["cost"], "seconds": rep_q["seconds"],
+                           "steps": rep_q["steps"], "expanded": rep_q["expanded"]}}
+    if rep_q["cost"] != rep["cost"]:
+        raise RuntimeError("A* single query: tile search %d, queue lists %d" % (rep["cost"], rep_q["cost"]))
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_lib import Oracle

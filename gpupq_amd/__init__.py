@@ -11,6 +11,7 @@ All work runs in hand-written HIP kernels in libgpupq.so (gfx950); importing
 this package without the built library raises ImportError.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -55,6 +56,7 @@ class Queue:
         h = ctypes.c_void_p()
         check(lib.gpq_queue_create(ctypes.byref(cfg), ctypes.byref(h)), "queue_init")
         self.handle = h
+        self._models = weakref.WeakSet()     # models on this queue: finished before it
 
     # -- queue.h
     def queue_post_init(self):
@@ -93,6 +95,8 @@ class Queue:
 
     def queue_finish(self):
         if self.handle:
+            for m in list(self._models):         # model_finish() (src/des.cu:220) before queue_finish() (:91)
+                m.model_finish()
             check(lib.gpq_queue_destroy(self.handle), "queue_finish")
             self.handle = None
 
@@ -164,6 +168,7 @@ class Phold:
         h = ctypes.c_void_p()
         check(lib.gpq_phold_create(queue.handle, ctypes.byref(self.cfg), ctypes.byref(h)), "phold_create")
         self.handle = h
+        queue._models.add(self)
 
     def model_init(self):
         check(lib.gpq_phold_init(self.handle), "model_init")
@@ -236,16 +241,17 @@ class Astar:
     integers: 4-connected unit steps (Manhattan h) or 8-connected 10/14 (octile h)."""
 
     def __init__(self, queue, width, height, connectivity=4, source=(0, 0), target=None, seed=7, threshold=0.2,
-                 stream_tag=0, query=0, window=0):
+                 stream_tag=0, query=0, window=0, tile=0):
         target = (width - 1, height - 1) if target is None else target
         thr = threshold if isinstance(threshold, int) else min(int(threshold * 2 ** 32), 0xFFFFFFFF)
         self.queue = queue
         self.width, self.height = width, height
         self.cfg = _lib.AstarConfig(width, height, connectivity, source[0], source[1], target[0], target[1], seed,
-                                    stream_tag, query, thr, min(window, 0xFFFFFFFF))
+                                    stream_tag, query, thr, min(window, 0xFFFFFFFF), tile)
         h = ctypes.c_void_p()
         check(lib.gpq_astar_create(queue.handle, ctypes.byref(self.cfg), ctypes.byref(h)), "astar_create")
         self.handle = h
+        queue._models.add(self)
 
     def model_init(self):
         check(lib.gpq_astar_init(self.handle), "astar model_init")

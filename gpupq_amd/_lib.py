@@ -52,7 +52,7 @@ class KernelProfile(ctypes.Structure):
 class AstarConfig(ctypes.Structure):
     _fields_ = [("width", c_u32), ("height", c_u32), ("connectivity", c_u32), ("sx", c_u32), ("sy", c_u32),
                 ("tx", c_u32), ("ty", c_u32), ("seed", c_u32), ("stream_tag", c_u32), ("query", c_u32),
-                ("threshold", c_u32), ("window", c_u32)]
+                ("threshold", c_u32), ("window", c_u32), ("tile", c_u32)]
 
 
 class AstarReport(ctypes.Structure):

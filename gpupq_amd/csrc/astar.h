@@ -30,6 +30,22 @@ struct AstarState {
   unsigned long long expanded;
   unsigned long long closed;
   unsigned long long steps;
+  // tile search: inbox parity read this step, window bound, smallest f pushed per parity,
+  // smallest f held back in the tiles' parking areas
+  uint32_t parity, fmax, fmin_next[2], park_min;
+};
+
+// Tile search (gpq_astar_config.tile > 0): per tile and parity an inbox of (cell, g) entries.
+struct AstarTiles {
+  uint2* inbox;                  // [2][tiles][cap]
+  uint32_t* count;               // [2][tiles]
+  uint32_t* active;              // [2][tiles]: tiles to search in the step of that parity
+  uint32_t* n_active;            // [2]
+  uint2* park;                   // [tiles][cap]: a tile's entries beyond the window, until it reaches them
+  uint32_t* park_count;          // [tiles]
+  uint32_t* park_fmin;           // [tiles]: smallest f parked (0xFFFFFFFF: none)
+  uint32_t* error;               // device error flags (the queue's window state)
+  uint32_t tiles, tiles_x, cap;
 };
 
 struct AstarDev {
@@ -70,6 +86,8 @@ class AstarModel {
   uint32_t* g = nullptr;
   uint32_t* closed = nullptr;
   AstarState* st = nullptr;
+  AstarTiles tiles_{};           // tile search only
+  gpq_status restart_tiles();
   AstarState host_st{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;

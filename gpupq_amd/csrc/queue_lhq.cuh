@@ -67,6 +67,7 @@ struct LhqView {
   }
   __device__ __forceinline__ void set_done(int index) const { (void)pop_root((uint32_t)index); }
 
+
   __device__ __forceinline__ bool is_empty(int lp) const { return len[lp] == 0 && ins[lp] == 0; }
   __device__ __forceinline__ int length(int lp) const { return (int)len[lp]; }
   __device__ __forceinline__ void clear(int lp) const {

@@ -82,7 +82,9 @@ const char* gpq_version(void);
 /* ---- host queue API (src/queue.h:86-115) ------------------------------------------------ */
 /* queue_init() (src/queue.h:87): allocates and clears the queue. */
 gpq_status gpq_queue_create(const gpq_queue_config* cfg, gpq_queue** out);
-/* queue_finish() (src/queue.h:103) plus release of all device memory. */
+/* queue_finish() (src/queue.h:103) plus release of all device memory.  Destroy every model
+ * created on the queue (gpq_phold_destroy, gpq_astar_destroy) first, as the reference calls
+ * model_finish() (src/des.cu:220) before queue_finish() (src/des.cu:91). */
 gpq_status gpq_queue_destroy(gpq_queue* q);
 /* queue_post_init() (src/queue.h:92): bulk maintenance after a chunk of initial inserts. */
 gpq_status gpq_queue_post_init(gpq_queue* q);
@@ -213,6 +215,13 @@ typedef struct {
   uint32_t threshold;        /* obstacle density * 2^32 */
   uint32_t window;           /* a list pops its best entry only if f <= (smallest f of all lists) + window;
                                 0xFFFFFFFF: every non-empty list pops each step */
+  uint32_t tile;             /* 0: the open lists are the queue's LPs, a cell's list a hash of the cell,
+                                each list popping its best entry per step (src/a_star.cu:325-374);
+                                16 or 32: the grid is cut into tile x tile squares, each with its own
+                                open list (an inbox in HBM), searched by one workgroup per step that
+                                expands its tile's entries inside the window layer after layer with
+                                the tile's g values in LDS; entries for other tiles go to their inboxes.
+                                The queue then only provides the stream and the error flags. */
 } gpq_astar_config;
 
 typedef struct {

@@ -51,6 +51,83 @@ def test_single_query_matches_oracle(gpq, oracle, kind, conn):
             a.model_finish()
 
 
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("tile", [16, 32])
+@pytest.mark.parametrize("window", [0, 25, 0xFFFFFFFF])
+def test_single_query_tile_search_matches_oracle(gpq, oracle, conn, tile, window):
+    """The tile search (one workgroup per tile, LDS frontier) keeps the exact cost."""
+    cases = [(97, 61, (3, 5), (90, 57), 0.25), (200, 150, (199, 0), (0, 149), 0.3), (64, 64, (0, 0), (63, 63), 0.2)]
+    for (w, h, s, t, p) in cases:
+        for seed in (1, 2, 3):
+            with open_lists(gpq, "lhq") as q:
+                a = gpq.Astar(q, w, h, connectivity=conn, source=s, target=t, seed=seed, threshold=p, tile=tile,
+                              window=window)
+                a.model_init()
+                ref_grid = oracle_grid(oracle, w, h, seed, 0, 0, thr(p), forced=(s, t))
+                rep = a.run("graph", 8)
+                cost, _ = oracle.astar(ref_grid, w, h, conn, s, t)
+                assert rep["cost"] == cost == oracle.dijkstra(ref_grid, w, h, conn, s, t)
+                assert rep["error_flags"] == 0 and rep["closed"] <= rep["expanded"]
+                if cost >= 0:
+                    check_path(a.path(), ref_grid, w, conn, s, t, cost)
+                a.model_finish()
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_single_query_tile_search_large(gpq, oracle, conn):
+    """Larger random grids near the percolation density: long detours, many f levels."""
+    w, h = 1000, 700
+    for seed, p in ((11, 0.35), (12, 0.2)):
+        s, t = (5, 690), (990, 3)
+        with open_lists(gpq, "lhq") as q:
+            a = gpq.Astar(q, w, h, connectivity=conn, source=s, target=t, seed=seed, threshold=p, tile=32)
+            a.model_init()
+            ref_grid = oracle_grid(oracle, w, h, seed, 0, 0, thr(p), forced=(s, t))
+            rep = a.run("graph", 16)
+            assert rep["cost"] == oracle.dijkstra(ref_grid, w, h, conn, s, t)
+            assert rep["error_flags"] == 0
+            if rep["cost"] >= 0:
+                check_path(a.path(), ref_grid, w, conn, s, t, rep["cost"])
+            a.model_finish()
+
+
+def test_single_query_tile_search_edges(gpq, oracle):
+    with open_lists(gpq, "lhq") as q:
+        a = gpq.Astar(q, 1, 1, source=(0, 0), target=(0, 0), threshold=0, tile=16)
+        a.model_init()
+        assert a.run("plain")["cost"] == 0
+        a.model_finish()
+        a = gpq.Astar(q, 40, 40, source=(17, 17), target=(17, 17), threshold=0.3, tile=16)
+        a.model_init()
+        assert a.run("graph", 4)["cost"] == 0
+        a.model_finish()
+        # an enclosed target: the search ends with nothing pending
+        w, h = 48, 40
+        grid = np.zeros(w * h, dtype=np.uint8)
+        for (x, y) in ((29, 20), (31, 20), (30, 19), (30, 21)):
+            grid[y * w + x] = 1
+        a = gpq.Astar(q, w, h, source=(1, 1), target=(30, 20), threshold=0, tile=16)
+        a.set_grid(grid)
+        rep = a.run("graph", 4)
+        assert rep["cost"] == -1 and a.path() == []
+        a.model_finish()
+        # plain == graph
+        a = gpq.Astar(q, 120, 90, connectivity=8, source=(2, 3), target=(110, 80), seed=5, threshold=0.25, tile=32)
+        a.model_init()
+        r1 = a.run("plain")
+        a.model_init()
+        r2 = a.run("graph", 5)
+        ref = oracle_grid(oracle, 120, 90, 5, 0, 0, thr(0.25), forced=((2, 3), (110, 80)))
+        assert r1["cost"] == r2["cost"] == oracle.dijkstra(ref, 120, 90, 8, (2, 3), (110, 80))
+        a.model_finish()
+
+
+def test_single_query_tile_must_be_16_or_32(gpq):
+    with open_lists(gpq, "lhq") as q:
+        with pytest.raises(Exception):
+            gpq.Astar(q, 32, 32, tile=8)
+
+
 def check_path(path, grid, w, conn, s, t, cost):
     """A valid path: source to target, free cells, unit steps, edge costs summing to the cost."""
     assert path[0] == tuple(s) and path[-1] == tuple(t)
@@ -105,11 +182,14 @@ def test_single_query_plain_equals_graph(gpq, oracle):
         a.model_finish()
 
 
-def test_single_query_4096_grid(gpq, oracle):
-    """The 4096 x 4096, 4-connected, Bernoulli(0.2) seed 7 query, (0,0) -> (4095,4095)."""
+@pytest.mark.parametrize("tile", [0, 16, 32])
+def test_single_query_4096_grid(gpq, oracle, tile):
+    """The 4096 x 4096, 4-connected, Bernoulli(0.2) seed 7 query, (0,0) -> (4095,4095): open
+    lists in a local heap queue (tile 0) and the tile search."""
     w = h = 4096
-    with gpq.Queue("lhq", num_lps=1 << 16, capacity_per_lp=64, stage_per_lp=16, lookahead=INF, horizon=INF) as q:
-        a = gpq.Astar(q, w, h, connectivity=4, seed=7, threshold=0.2)
+    with gpq.Queue("lhq", num_lps=64 if tile else 1 << 16, capacity_per_lp=64, stage_per_lp=16, lookahead=INF,
+                   horizon=INF) as q:
+        a = gpq.Astar(q, w, h, connectivity=4, seed=7, threshold=0.2, tile=tile)
         a.model_init()
         rep = a.run("graph", 32)
         grid = a.grid()
