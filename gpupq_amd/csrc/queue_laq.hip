@@ -13,6 +13,7 @@
 #include <cmath>
 
 #include "laq_merge.cuh"
+#include "phold_fused.cuh"
 #include "queues.h"
 
 namespace gpq {
@@ -265,40 +266,15 @@ __global__ void laq_far_publish_kernel(const uint32_t* partial, uint32_t nb, Win
 }
 
 // queue_post after a model step that merged and committed in one kernel (the PHOLD step of
-// drain mode 2, which owns the second staging buffer): the window advances by the lookahead (capped at the end
-// time); a window that reached the end time ends the run; a window beyond the near horizon
-// schedules the horizon's move (by `step`) for the next step; the staging buffers swap.
-// settle: only the buffer choice is reset (both buffers are empty after a settle pass).
+// drain mode 2, which owns the second staging buffer): advance_window.  settle: only the
+// buffer choice is reset (both buffers are empty after a settle pass).
 __global__ void laq_step_publish_kernel(WindowState* ws, float lookahead, float end_time, float step,
                                         uint32_t settle) {
-  WindowState w = *ws;
   if (settle) {
     ws->sel = 0;
     return;
   }
-  if (w.done) return;
-  if (w.near_next) {
-    w.near_end = w.near_next;
-    w.near_next = 0;
-  }
-  const float we = bits_to_f(w.wend);
-  w.windows += 1;
-  w.sel ^= 1u;
-  w.gmin = w.wend;                                // a lower bound of the pending minimum
-  if (!(we < end_time)) {
-    w.done = 1;
-  } else {
-    float wn = we + lookahead;
-    if (end_time < wn) wn = end_time;
-    const float nf = bits_to_f(w.near_end);
-    if (wn > nf) {
-      float nx = nf + step;
-      if (!(nx >= wn)) nx = wn + step;
-      w.near_next = f_to_bits(nx);
-    }
-    w.wend = f_to_bits(wn);
-  }
-  *ws = w;
+  advance_window(ws, lookahead, end_time, step);
 }
 
 __global__ void laq_init_kernel(LaqView q) {
