@@ -29,49 +29,84 @@ struct OutMap {
   }
 };
 
-// Number of A records among the first d outputs of the stable merge (ties: A first).
-__device__ uint32_t merge_split(const ASrc& A, uint32_t na, const Event* B, uint32_t nb, uint32_t d) {
-  uint32_t lo = d > nb ? d - nb : 0u;
-  uint32_t hi = d < na ? d : na;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (key_less(load_event(B + (d - 1u - mid)), A.get(mid))) hi = mid; else lo = mid + 1u;
+// Merge-path split: the number of A records among the first d outputs of the stable merge
+// (ties: A first) is the smallest i in [max(0, d - nb), min(d, na)] with B[d-1-i] < A[i].
+// Found for the block's two diagonals d0 and d1 at once, each by half of the block: every
+// round the 128 threads of a half probe 128 evenly spaced points of the remaining range and
+// keep the gap where the predicate turns true, so a range of n shrinks to nothing in
+// ceil(log_128 n) rounds of one load each, not log_2 n dependent loads.
+__device__ __forceinline__ void block_merge_splits(const ASrc& A, uint32_t na, const Event* B, uint32_t nb, uint32_t d0, uint32_t d1,
+                                   uint32_t* s_split) {
+  constexpr uint32_t kProbes = kThreads / 2;
+  __shared__ uint32_t s_lo[2], s_hi[2], s_first[2];
+  const uint32_t h = threadIdx.x / kProbes, k = threadIdx.x % kProbes;
+  const uint32_t d = h ? d1 : d0;
+  if (k == 0) {
+    s_lo[h] = d > nb ? d - nb : 0u;
+    s_hi[h] = d < na ? d : na;
   }
-  return lo;
+  __syncthreads();
+  while (s_lo[0] < s_hi[0] || s_lo[1] < s_hi[1]) {   // uniform: read after a barrier
+    const uint32_t lo = s_lo[h], hi = s_hi[h];
+    const uint32_t step = (hi - lo + kProbes - 1) / kProbes;
+    if (k == 0) s_first[h] = 0xFFFFFFFFu;
+    __syncthreads();
+    if (lo < hi) {
+      const uint32_t i = lo + k * step;
+      if (i < hi && key_less(load_event(B + (d - 1u - i)), A.get(i))) atomicMin(&s_first[h], k);
+    }
+    __syncthreads();
+    if (k == 0 && lo < hi) {
+      const uint32_t f = s_first[h];
+      if (f == 0xFFFFFFFFu) {
+        s_lo[h] = lo + (hi - lo - 1u) / step * step + 1u;   // past the last probe
+      } else {
+        s_hi[h] = lo + f * step;
+        if (f) s_lo[h] = lo + (f - 1u) * step + 1u;
+      }
+    }
+    __syncthreads();
+  }
+  if (k == 0) s_split[h] = s_lo[h];
+  __syncthreads();
 }
 
 // Merges output ranks [t0, t0 + kMergeTile) of merge(A, B) through LDS: the block's input
 // ranges are located by two merge-path searches, staged in LDS, each thread merges 8
 // outputs, and the results leave through LDS in coalesced 16-byte stores.
-__device__ void merge_tile(const ASrc& A, uint32_t na, const Event* B, uint32_t nb, uint32_t t0, const OutMap& out) {
-  __shared__ Event s[kMergeTile];
+// LDS slot of tile element i: one pad record after every 8, so the 8-record runs that
+// consecutive threads merge start in different banks.
+__device__ __forceinline__ uint32_t pad8(uint32_t i) { return i + (i >> 3); }
+
+__device__ __forceinline__ void merge_tile(const ASrc& A, uint32_t na, const Event* B, uint32_t nb, uint32_t t0, const OutMap& out) {
+  __shared__ Event s[kMergeTile + kMergeTile / 8];
   __shared__ uint32_t s_split[2];
   const uint32_t total = na + nb;
   const uint32_t t1 = t0 + kMergeTile < total ? t0 + kMergeTile : total;
-  if (threadIdx.x < 2) s_split[threadIdx.x] = merge_split(A, na, B, nb, threadIdx.x == 0 ? t0 : t1);
-  __syncthreads();
+  block_merge_splits(A, na, B, nb, t0, t1, s_split);
   const uint32_t i0 = s_split[0], i1 = s_split[1];
   const uint32_t j0 = t0 - i0;
   const uint32_t ca = i1 - i0, cb = (t1 - t0) - ca;
-  for (uint32_t x = threadIdx.x; x < ca; x += kThreads) s[x] = A.get(i0 + x);
-  for (uint32_t x = threadIdx.x; x < cb; x += kThreads) s[ca + x] = load_event(B + j0 + x);
+  for (uint32_t x = threadIdx.x; x < ca; x += kThreads) s[pad8(x)] = A.get(i0 + x);
+  for (uint32_t x = threadIdx.x; x < cb; x += kThreads) s[pad8(ca + x)] = load_event(B + j0 + x);
   __syncthreads();
   const uint32_t len = t1 - t0;
   const uint32_t d = threadIdx.x * kPerThread;
-  Event res[kPerThread];
+  uint4 res[kPerThread];
   uint32_t got = 0;
   if (d < len) {
     uint32_t lo = d > cb ? d - cb : 0u, hi = d < ca ? d : ca;
     while (lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (key_less(s[ca + (d - 1u - mid)], s[mid])) hi = mid; else lo = mid + 1u;
+      if (key_less(s[pad8(ca + (d - 1u - mid))], s[pad8(mid)])) hi = mid; else lo = mid + 1u;
     }
     uint32_t ia = lo, ib = d - lo;
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
       if (d + q < len) {
-        const bool take_a = ib >= cb || (ia < ca && !key_less(s[ca + ib], s[ia]));
-        res[q] = take_a ? s[ia] : s[ca + ib];
+        const bool take_a = ib >= cb || (ia < ca && !key_less(s[pad8(ca + ib)], s[pad8(ia)]));
+        const Event r = take_a ? s[pad8(ia)] : s[pad8(ca + ib)];
+        res[q] = make_uint4(r.ts, r.dst, r.src, r.seq);
         ia += take_a ? 1u : 0u;
         ib += take_a ? 0u : 1u;
         ++got;
@@ -81,9 +116,9 @@ __device__ void merge_tile(const ASrc& A, uint32_t na, const Event* B, uint32_t 
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kPerThread; ++q)
-    if ((uint32_t)q < got) s[d + q] = res[q];
+    if ((uint32_t)q < got) s[pad8(d + q)] = Event{res[q].x, res[q].y, res[q].z, res[q].w};
   __syncthreads();
-  for (uint32_t x = threadIdx.x; x < len; x += kThreads) store_event(out.at(t0 + x), s[x]);
+  for (uint32_t x = threadIdx.x; x < len; x += kThreads) store_event(out.at(t0 + x), s[pad8(x)]);
 }
 
 __global__ __launch_bounds__(kThreads) void tile_sort_kernel(const Event* in, Event* out, const uint32_t* d_n) {

@@ -48,7 +48,8 @@ def test_sort_matches_numpy(gpq, n):
     assert np.array_equal(out, keysort(a))
 
 
-@pytest.mark.parametrize("na,nb", [(0, 5), (5, 0), (1, 1), (3000, 10), (10, 3000), (70001, 50001), (1 << 18, 1 << 17)])
+@pytest.mark.parametrize("na,nb", [(0, 5), (5, 0), (1, 1), (3000, 10), (10, 3000), (70001, 50001), (1 << 18, 1 << 17),
+                                   ((1 << 20) + 5, 300), (300, (1 << 20) + 5)])
 def test_merge_is_stable_and_sorted(gpq, na, nb):
     rng = np.random.default_rng(na * 7 + nb)
     a = keysort(events(gpq, rng.integers(0, 500, na) * 1.0, src=rng.integers(0, 3, na), seq=rng.integers(0, 4, na)))
