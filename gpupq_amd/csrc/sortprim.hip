@@ -225,35 +225,35 @@ __global__ __launch_bounds__(kThreads) void select_count_kernel(const Event* con
   if (threadIdx.x == 0) counts[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
-// Exclusive scan of counts[0:nblk) into offsets[0:nblk); totals of marked / rest.
+// Exclusive scan of counts[0:nblk) into offsets[0:nblk); totals of marked / rest.  Thread t
+// owns a contiguous chunk: it sums the chunk (independent loads, all in flight at once), the
+// 1024 sums are scanned across the block, and the chunk's offsets follow from its share.
 __global__ __launch_bounds__(1024) void select_scan_kernel(const uint32_t* counts, uint32_t* offsets, uint32_t nblk,
                                                            const uint32_t* d_prefix, uint32_t* d_deq_n,
                                                            uint32_t* d_rest_n) {
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_carry;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (uint32_t base = 0; base < nblk; base += 1024) {
-    const uint32_t i = base + threadIdx.x;
-    const uint32_t v = i < nblk ? counts[i] : 0u;
-    uint32_t x = v;   // inclusive wave scan
-    for (int off = 1; off < kWave; off <<= 1) {
-      const uint32_t y = (uint32_t)__shfl_up((int)x, off, kWave);
-      if (lane_id() >= off) x += y;
-    }
-    if (lane_id() == kWave - 1) s_w[threadIdx.x / kWave] = x;
-    __syncthreads();
-    uint32_t wave_off = 0;
-    for (uint32_t w = 0; w < threadIdx.x / kWave; ++w) wave_off += s_w[w];
-    if (i < nblk) offsets[i] = s_carry + wave_off + x - v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t t = 0;
-      for (int w = 0; w < 16; ++w) t += s_w[w];
-      s_carry += t;
-    }
-    __syncthreads();
+  const uint32_t per = (nblk + blockDim.x - 1) / blockDim.x;
+  const uint32_t b0 = min(threadIdx.x * per, nblk), b1 = min(b0 + per, nblk);
+  uint32_t v = 0;
+#pragma unroll 8
+  for (uint32_t i = b0; i < b1; ++i) v += counts[i];
+  uint32_t x = v;   // inclusive wave scan of the chunk sums
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off, kWave);
+    if (lane_id() >= off) x += y;
   }
+  if (lane_id() == kWave - 1) s_w[threadIdx.x / kWave] = x;
+  __syncthreads();
+  uint32_t run = x - v;
+  for (uint32_t w = 0; w < threadIdx.x / kWave; ++w) run += s_w[w];
+  for (uint32_t i = b0; i < b1; ++i) {
+    const uint32_t c = counts[i];
+    offsets[i] = run;
+    run += c;
+  }
+  if (threadIdx.x == blockDim.x - 1) s_carry = run;
+  __syncthreads();
   if (threadIdx.x == 0) {
     *d_deq_n = s_carry;
     *d_rest_n = *d_prefix - s_carry;
