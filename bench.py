@@ -233,7 +233,7 @@ def phold_batch_bench(gpq, log, instances=32, seed=5):
     return out
 
 
-def astar_single_bench(gpq, log, cpu=True, tile=32):
+def astar_single_bench(gpq, log, cpu=True, tile=32, drive="graph"):
     """One 4096 x 4096, 4-connected, Bernoulli(0.2) seed 7 query, (0,0) -> (4095,4095): the tile
     search (one workgroup per tile x tile square) and, for comparison, 65536 open lists in a
     local heap queue; against the oracle's A* on one core."""
@@ -244,9 +244,9 @@ def astar_single_bench(gpq, log, cpu=True, tile=32):
         q = gpq.Queue("lhq", num_lps=lists, capacity_per_lp=64, stage_per_lp=16, lookahead=inf, horizon=inf)
         a = gpq.Astar(q, w, h, connectivity=4, seed=7, threshold=0.2, tile=tile_side)
         a.model_init()
-        a.run("graph", 32)                               # warm up (graph capture)
+        a.run(drive, 32)                                 # warm up (graph capture)
         a.model_init()
-        rep = a.run("graph", 32)
+        rep = a.run(drive, 32)
         grid = a.grid() if cpu else None
         a.model_finish()
         q.queue_finish()
@@ -255,6 +255,7 @@ def astar_single_bench(gpq, log, cpu=True, tile=32):
     rep, grid = one(64, tile)
     rep_q, _ = one(1 << 16, 0)
     out = {"grid": "4096x4096, 4-connected, Bernoulli(0.2), seed 7", "search": "tiles %dx%d" % (tile, tile),
+           "drive": drive,
            "cost": rep["cost"], "seconds": rep["seconds"], "steps": rep["steps"], "expanded": rep["expanded"],
            "closed": rep["closed"], "expanded_per_s": rep["expanded"] / rep["seconds"] if rep["seconds"] > 0 else 0.0,
            "queue_lists": {"open_lists": 1 << 16, "queue": "lhq", "cost": rep_q["cost"], "seconds": rep_q["seconds"],
@@ -348,6 +349,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-queues", action="store_true", help="skip the same workload through the other queue kinds")
+    ap.add_argument("--astar-drive", default="graph", choices=("graph", "plain"),
+                    help="single A* query: hipGraph replays, or one launch sequence per step (eager, for tracing)")
     ap.add_argument("--serve-rates", default="50,100,200,400", help="Poisson arrival rates (requests/s) to sweep")
     ap.add_argument("--skip-phold", action="store_true", help="profiling aid: A* and serving parts only")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -386,7 +389,7 @@ def main():
             line["phold_batch"] = phold_batch_bench(gpq, log)
         if not args.no_astar:
             line["astar_batch"] = astar_bench(gpq, log, cpu=not args.no_cpu)
-            line["astar_single"] = astar_single_bench(gpq, log, cpu=not args.no_cpu)
+            line["astar_single"] = astar_single_bench(gpq, log, cpu=not args.no_cpu, drive=args.astar_drive)
         if not args.no_serving:
             line["serving"] = serving_bench(gpq, log, rates=[float(r) for r in args.serve_rates.split(",")])
         print(json.dumps(line), flush=True)
@@ -450,7 +453,7 @@ def main():
         line["phold_batch"] = phold_batch_bench(gpq, log)
     if not args.no_astar:
         line["astar_batch"] = astar_bench(gpq, log, cpu=not args.no_cpu)
-        line["astar_single"] = astar_single_bench(gpq, log, cpu=not args.no_cpu)
+        line["astar_single"] = astar_single_bench(gpq, log, cpu=not args.no_cpu, drive=args.astar_drive)
     if not args.no_serving:
         line["serving"] = serving_bench(gpq, log, rates=[float(r) for r in args.serve_rates.split(",")])
     print(json.dumps(line), flush=True)
