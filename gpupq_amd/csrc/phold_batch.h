@@ -46,6 +46,9 @@ class PholdPool {
   bool any_active() const;
   // Reads the instance's per-LP counters and hashes, frees its range.
   gpq_status retire(uint32_t slot, uint64_t* trace_hash, uint64_t* events);
+  // retire() for several instances with one host round trip; trace_hash[k] and events[k]
+  // belong to slots[k].
+  gpq_status retire_many(const std::vector<uint32_t>& slots, uint64_t* trace_hash, uint64_t* events);
   uint32_t error_flags_host() const;
   gpq_status errors(uint32_t* flags);
 
@@ -64,6 +67,8 @@ class PholdPool {
   uint32_t* d_count_ = nullptr;
   uint64_t* d_hash_ = nullptr;
   WindowState* h_ws_ = nullptr;          // pinned mirror of d_ws_
+  uint32_t* h_count_ = nullptr;          // pinned staging of retired instances' counters, by LP
+  uint64_t* h_hash_ = nullptr;           // and hashes
   hipEvent_t done_ = nullptr;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;

@@ -10,8 +10,10 @@
 //   pool_publish_kernel  one workgroup per instance folds the partials of its blocks into
 //                        its window (gmin, window end, done).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
+#include <thread>
 #include <memory>
 
 #include "phold_batch.h"
@@ -270,6 +272,8 @@ gpq_status PholdPool::create(uint32_t capacity_lps, uint32_t capacity_per_lp, ui
   GPQ_TRY(p->arena_.alloc(&p->d_hash_, cap));
   GPQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_ws_), sizeof(WindowState) * max_instances,
                         hipHostMallocDefault));
+  GPQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_count_), sizeof(uint32_t) * cap, hipHostMallocDefault));
+  GPQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_hash_), sizeof(uint64_t) * cap, hipHostMallocDefault));
   GPQ_HIP(hipEventCreateWithFlags(&p->done_, hipEventDisableTiming));
   p->inst_.assign(max_instances, PoolInst{});
   p->block_used_.assign(cap / kBlk, 0);
@@ -291,6 +295,8 @@ PholdPool::~PholdPool() {
   if (graph_) (void)hipGraphDestroy(graph_);
   if (done_) (void)hipEventDestroy(done_);
   if (h_ws_) (void)hipHostFree(h_ws_);
+  if (h_count_) (void)hipHostFree(h_count_);
+  if (h_hash_) (void)hipHostFree(h_hash_);
   arena_.release();
   delete q_;
 }
@@ -414,30 +420,70 @@ uint32_t PholdPool::error_flags_host() const { return q_->host_ws->error; }
 gpq_status PholdPool::errors(uint32_t* flags) { return q_->error_flags(flags); }
 
 gpq_status PholdPool::retire(uint32_t s, uint64_t* trace_hash, uint64_t* events) {
-  GPQ_CHECK_ARG(s < max_inst_ && inst_[s].active, "pool: retire of an inactive slot");
+  return retire_many(std::vector<uint32_t>{s}, trace_hash, events);
+}
+
+gpq_status PholdPool::retire_many(const std::vector<uint32_t>& slots, uint64_t* trace_hash, uint64_t* events) {
   GPQ_CHECK_ARG(!pending_, "pool: retire while a replay is in flight");
-  const PoolInst in = inst_[s];
-  std::vector<uint32_t> c(in.lps);
-  std::vector<uint64_t> h(in.lps);
-  GPQ_HIP(hipMemcpyAsync(c.data(), d_count_ + in.base, sizeof(uint32_t) * in.lps, hipMemcpyDeviceToHost, q_->stream));
-  GPQ_HIP(hipMemcpyAsync(h.data(), d_hash_ + in.base, sizeof(uint64_t) * in.lps, hipMemcpyDeviceToHost, q_->stream));
-  inst_[s].active = 0;
-  GPQ_HIP(hipMemcpyAsync(d_inst_ + s, &inst_[s], sizeof(PoolInst), hipMemcpyHostToDevice, q_->stream));
-  const uint32_t padded = (in.block_end - in.block_begin) * kBlk;
-  hipLaunchKernelGGL(pool_release_kernel, dim3(ceil_div(padded, 256)), dim3(256), 0, q_->stream, in.base, padded,
-                     q_->view());
-  GPQ_HIP(hipGetLastError());
-  GPQ_HIP(hipMemsetAsync(d_block_live_ + in.block_begin, 0, in.block_end - in.block_begin, q_->stream));
-  GPQ_HIP(hipStreamSynchronize(q_->stream));
-  for (uint32_t b = in.block_begin; b < in.block_end; ++b) block_used_[b] = 0;
-  uint64_t x = 0x6A09E667F3BCC908ull, ev = 0;
-  for (uint32_t i = 0; i < in.lps; ++i) {
-    x = host_mix64_b(x ^ h[i]);
-    x = host_mix64_b(x ^ (((uint64_t)c[i] << 32) | i));
-    ev += c[i];
+  for (size_t k = 0; k < slots.size(); ++k) {
+    GPQ_CHECK_ARG(slots[k] < max_inst_ && inst_[slots[k]].active, "pool: retire of an inactive slot");
+    for (size_t j = 0; j < k; ++j) GPQ_CHECK_ARG(slots[j] != slots[k], "pool: slot retired twice");
   }
-  if (trace_hash) *trace_hash = x;
-  if (events) *events = ev;
+  if (slots.empty()) return GPQ_OK;
+  std::vector<PoolInst> in(slots.size());
+  // Every instance's counters and hashes come back in one round trip (pinned staging at the
+  // instance's own LP offset), then its range is released on the device.
+  for (size_t k = 0; k < slots.size(); ++k) {
+    const uint32_t s = slots[k];
+    in[k] = inst_[s];
+    GPQ_HIP(hipMemcpyAsync(h_count_ + in[k].base, d_count_ + in[k].base, sizeof(uint32_t) * in[k].lps,
+                           hipMemcpyDeviceToHost, q_->stream));
+    GPQ_HIP(hipMemcpyAsync(h_hash_ + in[k].base, d_hash_ + in[k].base, sizeof(uint64_t) * in[k].lps,
+                           hipMemcpyDeviceToHost, q_->stream));
+    inst_[s].active = 0;
+    GPQ_HIP(hipMemcpyAsync(d_inst_ + s, &inst_[s], sizeof(PoolInst), hipMemcpyHostToDevice, q_->stream));
+    const uint32_t padded = (in[k].block_end - in[k].block_begin) * kBlk;
+    hipLaunchKernelGGL(pool_release_kernel, dim3(ceil_div(padded, 256)), dim3(256), 0, q_->stream, in[k].base,
+                       padded, q_->view());
+    GPQ_HIP(hipGetLastError());
+    GPQ_HIP(hipMemsetAsync(d_block_live_ + in[k].block_begin, 0, in[k].block_end - in[k].block_begin, q_->stream));
+  }
+  GPQ_HIP(hipStreamSynchronize(q_->stream));
+  for (const auto& r : in)
+    for (uint32_t b = r.block_begin; b < r.block_end; ++b) block_used_[b] = 0;
+  // The trace hash of one instance is a sequential fold over its LPs (the single-instance
+  // definition); instances fold on separate host threads, largest first.
+  auto fold = [&](size_t k) {
+    const uint32_t* c = h_count_ + in[k].base;
+    const uint64_t* h = h_hash_ + in[k].base;
+    uint64_t x = 0x6A09E667F3BCC908ull, ev = 0;
+    for (uint32_t i = 0; i < in[k].lps; ++i) {
+      x = host_mix64_b(x ^ h[i]);
+      x = host_mix64_b(x ^ (((uint64_t)c[i] << 32) | i));
+      ev += c[i];
+    }
+    if (trace_hash) trace_hash[k] = x;
+    if (events) events[k] = ev;
+  };
+  uint64_t total = 0;
+  for (const auto& r : in) total += r.lps;
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const unsigned nthreads = (unsigned)std::min<uint64_t>({(uint64_t)hw, (uint64_t)slots.size(), total / 16384 + 1});
+  if (nthreads <= 1) {
+    for (size_t k = 0; k < in.size(); ++k) fold(k);
+    return GPQ_OK;
+  }
+  std::vector<size_t> order(in.size());
+  for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return in[a].lps > in[b].lps; });
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> pool;
+  pool.reserve(nthreads);
+  for (unsigned t = 0; t < nthreads; ++t)
+    pool.emplace_back([&]() {
+      for (size_t i; (i = next.fetch_add(1)) < order.size();) fold(order[i]);
+    });
+  for (auto& t : pool) t.join();
   return GPQ_OK;
 }
 
@@ -515,7 +561,14 @@ gpq_status PholdBatch::run(gpq_drive_mode mode, uint32_t batch, double* seconds)
         set_error("phold batch: device error flags " + std::to_string(pool->error_flags_host()));
         return GPQ_ERR_CAPACITY;
       }
-      for (uint32_t s : fin) GPQ_TRY(pool->retire(s, &hash[req_of[s]], &events[req_of[s]]));
+      if (!fin.empty()) {
+        std::vector<uint64_t> fh(fin.size()), fe(fin.size());
+        GPQ_TRY(pool->retire_many(fin, fh.data(), fe.data()));
+        for (size_t k = 0; k < fin.size(); ++k) {
+          hash[req_of[fin[k]]] = fh[k];
+          events[req_of[fin[k]]] = fe[k];
+        }
+      }
       if (!pool->any_active()) break;
       GPQ_TRY(pool->launch_async());
       while (!pool->replay_done()) {

@@ -300,6 +300,7 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
     // a request is admitted when a free LP range of its size exists, between graph replays.
     std::vector<uint32_t> slot_req(pool ? 1024 : 0, 0xFFFFFFFFu);
     std::vector<uint32_t> fin;
+    std::vector<uint64_t> fin_hash, fin_events;
     const uint64_t per_lp_bytes = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage, kPholdFar);
     std::deque<uint32_t> waiting;          // arrived, not admitted
     std::vector<uint32_t> astar_ready;     // admitted A* queries not yet launched
@@ -316,10 +317,15 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
           return GPQ_ERR_CAPACITY;
         }
         pool->finished(&fin);
-        for (uint32_t s : fin) {
-          const uint32_t i = slot_req[s];
-          GPQ_TRY(pool->retire(s, &req[i].result, &req[i].work));
-          req[i].finish_ms = now_ms(t0);
+        fin_hash.resize(fin.size());
+        fin_events.resize(fin.size());
+        GPQ_TRY(pool->retire_many(fin, fin_hash.data(), fin_events.data()));
+        const double done_ms = now_ms(t0);
+        for (size_t k = 0; k < fin.size(); ++k) {
+          const uint32_t i = slot_req[fin[k]];
+          req[i].result = fin_hash[k];
+          req[i].work = fin_events[k];
+          req[i].finish_ms = done_ms;
           inflight_bytes -= req[i].bytes;
           --inflight;
           ++completed;
