@@ -334,7 +334,7 @@ gpq_status LaqQueue::init() {
     GPQ_CHECK_ARG(std::isfinite(cfg.lookahead) && cfg.lookahead > 0.0f, "LAQ two-tier: needs a finite lookahead");
     GPQ_CHECK_ARG(cfg.far_per_lp <= 4096, "LAQ two-tier: far_per_lp must be <= 4096");
     GPQ_CHECK_ARG(group_ <= 16, "LAQ two-tier: capacity_per_lp must be <= 128");
-    GPQ_CHECK_ARG(cfg.stage_per_lp <= kFarList, "LAQ two-tier: stage_per_lp must be <= 96");
+    GPQ_CHECK_ARG(cfg.stage_per_lp <= kFarList, "LAQ two-tier: stage_per_lp must be <= 64");
     GPQ_CHECK_ARG((uint64_t)L * cfg.far_per_lp < (1ull << 32), "LAQ two-tier: num_lps * far_per_lp too large");
     v_.far_cap = cfg.far_per_lp;
     far_step_ = cfg.far_step > 0.0f ? cfg.far_step : 4.0f * cfg.lookahead;
