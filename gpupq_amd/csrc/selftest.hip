@@ -64,9 +64,12 @@ extern "C" gpq_status gpq_selftest_sort(const gpq_event* in, uint32_t n, gpq_eve
   GPQ_HIP(hipMalloc(&dn.p, sizeof(uint32_t)));
   GPQ_HIP(hipMemcpy(a.p, in, sizeof(Event) * n, hipMemcpyHostToDevice));
   GPQ_HIP(hipMemcpy(dn.p, &n, sizeof(uint32_t), hipMemcpyHostToDevice));
+  // A capacity above n, as the queues pass it: the merge passes whose width already covers n
+  // are skipped or reduced to one copy, by the parity of their number.
+  const uint32_t cap = n <= (1u << 28) ? 6u * n : n;
   Event* res = nullptr;
   GPQ_TRY(gpq::sort_records(static_cast<Event*>(a.p), static_cast<Event*>(b0.p), static_cast<Event*>(b1.p),
-                            static_cast<uint32_t*>(dn.p), n, 0, &res));
+                            static_cast<uint32_t*>(dn.p), cap, 0, &res));
   GPQ_HIP(hipMemcpy(out, res, sizeof(Event) * n, hipMemcpyDeviceToHost));
   return GPQ_OK;
 }

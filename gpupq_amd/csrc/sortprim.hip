@@ -121,39 +121,86 @@ __device__ __forceinline__ void merge_tile(const ASrc& A, uint32_t na, const Eve
   for (uint32_t x = threadIdx.x; x < len; x += kThreads) store_event(out.at(t0 + x), s[pad8(x)]);
 }
 
+__device__ __forceinline__ void compare_swap(Event& a, Event& b, bool up) {
+  if (up ? key_less(b, a) : key_less(a, b)) {
+    const Event t = a;
+    a = b;
+    b = t;
+  }
+}
+
+// Bitonic sort of one kSortTile-record tile.  Thread t holds records 8t .. 8t+7 in registers,
+// so every stage with partner distance j < 8 (30 of the 66) runs in registers with no barrier;
+// the stages with j >= 8 run in LDS, each thread exchanging 4 of the tile's 1024 pairs.  The
+// LDS tile is padded one record per 8 (pad8) so that the 8-record runs of neighbouring threads
+// start in different banks.
 __global__ __launch_bounds__(kThreads) void tile_sort_kernel(const Event* in, Event* out, const uint32_t* d_n) {
-  __shared__ Event s[kSortTile];
+  static_assert(kSortTile == kThreads * 8, "eight records per thread");
+  __shared__ Event s[kSortTile + kSortTile / 8];
   const uint32_t n = *d_n;
   const uint32_t base = blockIdx.x * kSortTile;
   if (base >= n) return;
-  for (uint32_t i = threadIdx.x; i < kSortTile; i += kThreads) s[i] = base + i < n ? load_event(in + base + i) : sentinel();
+  for (uint32_t i = threadIdx.x; i < kSortTile; i += kThreads)
+    s[pad8(i)] = base + i < n ? load_event(in + base + i) : sentinel();
   __syncthreads();
-  for (uint32_t k = 2; k <= kSortTile; k <<= 1) {           // bitonic sorting network
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = threadIdx.x; i < kSortTile; i += kThreads) {
-        const uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          const Event a = s[i], b = s[ixj];
-          const bool up = (i & k) == 0;
-          if (up ? key_less(b, a) : key_less(a, b)) {
-            s[i] = b;
-            s[ixj] = a;
-          }
-        }
+  const uint32_t t0 = threadIdx.x * 8u;
+  Event r[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) r[q] = s[pad8(t0 + q)];
+#pragma unroll
+  for (uint32_t k = 2; k <= 8; k <<= 1)
+#pragma unroll
+    for (uint32_t j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (uint32_t a = 0; a < 8; ++a)
+        if ((a & j) == 0) compare_swap(r[a], r[a | j], ((t0 + a) & k) == 0);
+  for (uint32_t k = 16; k <= kSortTile; k <<= 1) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[pad8(t0 + q)] = r[q];
+    __syncthreads();
+    for (uint32_t j = k >> 1; j >= 8; j >>= 1) {
+#pragma unroll
+      for (uint32_t q = 0; q < kSortTile / 2 / kThreads; ++q) {
+        const uint32_t pr = threadIdx.x + q * kThreads;
+        const uint32_t i = ((pr & ~(j - 1u)) << 1) | (pr & (j - 1u));   // pair (i, i + j), bit j of i clear
+        Event x = s[pad8(i)], y = s[pad8(i | j)];
+        compare_swap(x, y, (i & k) == 0);
+        s[pad8(i)] = x;
+        s[pad8(i | j)] = y;
       }
       __syncthreads();
     }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r[q] = s[pad8(t0 + q)];
+    const bool up = (t0 & k) == 0;   // bit k >= 16 of i is that of t0 for all eight
+#pragma unroll
+    for (uint32_t j = 4; j > 0; j >>= 1)
+#pragma unroll
+      for (uint32_t a = 0; a < 8; ++a)
+        if ((a & j) == 0) compare_swap(r[a], r[a | j], up);
   }
+  __syncthreads();   // every thread's last LDS stage reads are done
+#pragma unroll
+  for (int q = 0; q < 8; ++q) s[pad8(t0 + q)] = r[q];
+  __syncthreads();
   for (uint32_t i = threadIdx.x; i < kSortTile; i += kThreads)
-    if (base + i < n) store_event(out + base + i, s[i]);
+    if (base + i < n) store_event(out + base + i, s[pad8(i)]);
 }
 
-// Merges adjacent sorted runs of width w (2w is a multiple of kMergeTile).
+// Merges adjacent sorted runs of width w (2w is a multiple of kMergeTile); pass `pass` of
+// `passes`.  Once w >= n every remaining pass would only copy the one run, so those passes
+// do nothing when their number is even (the run already sits in the final buffer) and the
+// first of them copies when it is odd.
 __global__ __launch_bounds__(kThreads) void merge_pass_kernel(const Event* in, Event* out, const uint32_t* d_n,
-                                                              uint32_t w) {
+                                                              uint32_t w, uint32_t pass, uint32_t passes) {
   const uint32_t n = *d_n;
   const uint32_t t0 = blockIdx.x * kMergeTile;
   if (t0 >= n) return;
+  if (w >= n) {
+    uint32_t first = 0;   // the first pass whose width covers n
+    while (((uint64_t)kSortTile << first) < n) ++first;
+    if (((passes - first) & 1u) == 0 || pass != first) return;
+  }
   const uint32_t base = t0 / (2u * w) * (2u * w);
   const uint32_t na = n - base < w ? n - base : w;
   const uint32_t rest = n - base - na;
@@ -225,38 +272,74 @@ __global__ __launch_bounds__(kThreads) void select_count_kernel(const Event* con
   if (threadIdx.x == 0) counts[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
-// Exclusive scan of counts[0:nblk) into offsets[0:nblk); totals of marked / rest.  Thread t
-// owns a contiguous chunk: it sums the chunk (independent loads, all in flight at once), the
-// 1024 sums are scanned across the block, and the chunk's offsets follow from its share.
+// Exclusive scan of counts[0:nb) into offsets[0:nb), nb = the blocks that hold the scanned
+// prefix (the count and scatter kernels do nothing past it); totals of marked / rest.  The
+// counts are read in sweeps of kScanRounds x 1024: element r * 1024 + t of a sweep goes to
+// thread t (coalesced, all loads of a sweep in flight together); each (round, wave) pair's
+// total goes to LDS and one wave scans those totals in order.
+constexpr int kScanRounds = 8;
 __global__ __launch_bounds__(1024) void select_scan_kernel(const uint32_t* counts, uint32_t* offsets, uint32_t nblk,
                                                            const uint32_t* d_prefix, uint32_t* d_deq_n,
                                                            uint32_t* d_rest_n) {
-  __shared__ uint32_t s_w[16];
-  __shared__ uint32_t s_carry;
-  const uint32_t per = (nblk + blockDim.x - 1) / blockDim.x;
-  const uint32_t b0 = min(threadIdx.x * per, nblk), b1 = min(b0 + per, nblk);
-  uint32_t v = 0;
-#pragma unroll 8
-  for (uint32_t i = b0; i < b1; ++i) v += counts[i];
-  uint32_t x = v;   // inclusive wave scan of the chunk sums
-  for (int off = 1; off < kWave; off <<= 1) {
-    const uint32_t y = (uint32_t)__shfl_up((int)x, off, kWave);
-    if (lane_id() >= off) x += y;
+  constexpr uint32_t kWaves = 1024 / kWave;
+  constexpr uint32_t kParts = kScanRounds * kWaves;          // (round, wave) totals per sweep
+  static_assert(kParts % kWave == 0, "whole parts per lane");
+  __shared__ uint32_t s_part[kParts];
+  __shared__ uint32_t s_total;
+  const uint32_t p = *d_prefix;
+  const uint32_t want = (uint32_t)(((uint64_t)p + kSelectTile - 1) / kSelectTile);
+  const uint32_t nb = want < nblk ? want : nblk;
+  const uint32_t lane = (uint32_t)lane_id(), wv = threadIdx.x / kWave;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nb; base += kScanRounds * 1024u) {
+    uint32_t v[kScanRounds];
+#pragma unroll
+    for (int r = 0; r < kScanRounds; ++r) {
+      const uint32_t i = base + (uint32_t)r * 1024u + threadIdx.x;
+      v[r] = i < nb ? counts[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kScanRounds; ++r) {   // v[r] becomes the exclusive prefix inside the wave
+      uint32_t x = v[r];
+      for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, off, kWave);
+        if ((int)lane >= off) x += y;
+      }
+      if (lane == kWave - 1) s_part[r * kWaves + wv] = x;
+      v[r] = x - v[r];
+    }
+    __syncthreads();
+    if (wv == 0) {   // exclusive scan of the parts in (round, wave) order, kParts / 64 per lane
+      constexpr uint32_t kPer = kParts / kWave;
+      uint32_t sum = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < kPer; ++u) sum += s_part[lane * kPer + u];
+      uint32_t x = sum;
+      for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, off, kWave);
+        if ((int)lane >= off) x += y;
+      }
+      uint32_t run = x - sum;
+#pragma unroll
+      for (uint32_t u = 0; u < kPer; ++u) {
+        const uint32_t c = s_part[lane * kPer + u];
+        s_part[lane * kPer + u] = run;
+        run += c;
+      }
+      if (lane == kWave - 1) s_total = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kScanRounds; ++r) {
+      const uint32_t i = base + (uint32_t)r * 1024u + threadIdx.x;
+      if (i < nb) offsets[i] = carry + s_part[r * kWaves + wv] + v[r];
+    }
+    carry += s_total;
+    __syncthreads();   // s_part and s_total are rewritten by the next sweep
   }
-  if (lane_id() == kWave - 1) s_w[threadIdx.x / kWave] = x;
-  __syncthreads();
-  uint32_t run = x - v;
-  for (uint32_t w = 0; w < threadIdx.x / kWave; ++w) run += s_w[w];
-  for (uint32_t i = b0; i < b1; ++i) {
-    const uint32_t c = counts[i];
-    offsets[i] = run;
-    run += c;
-  }
-  if (threadIdx.x == blockDim.x - 1) s_carry = run;
-  __syncthreads();
   if (threadIdx.x == 0) {
-    *d_deq_n = s_carry;
-    *d_rest_n = *d_prefix - s_carry;
+    *d_deq_n = carry;
+    *d_rest_n = p - carry;
   }
 }
 
@@ -321,9 +404,12 @@ gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_
   GPQ_HIP(hipGetLastError());
   Event* cur = buf0;
   Event* other = buf1;
-  for (uint64_t w = kSortTile; w < cap; w *= 2) {
+  uint32_t passes = 0;
+  for (uint64_t w = kSortTile; w < cap; w *= 2) ++passes;
+  uint32_t pass = 0;
+  for (uint64_t w = kSortTile; w < cap; w *= 2, ++pass) {
     hipLaunchKernelGGL(merge_pass_kernel, dim3(grid_of(cap, kMergeTile)), dim3(kThreads), 0, stream, cur, other, d_n,
-                       (uint32_t)w);
+                       (uint32_t)w, pass, passes);
     GPQ_HIP(hipGetLastError());
     Event* t = cur;
     cur = other;

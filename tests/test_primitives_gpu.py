@@ -39,7 +39,7 @@ def test_sort_enqueue_buffer_fixture(gpq):
     assert [gpq.bits_f32(t) for t in out["ts"]] == [1.0, 5.0, 10.0, 20.0]
 
 
-@pytest.mark.parametrize("n", [1, 7, 2047, 2048, 2049, 5000, 65536 + 17, (1 << 20) + 3])
+@pytest.mark.parametrize("n", [1, 7, 2047, 2048, 2049, 5000, 65536 + 17, 300001, (1 << 20) + 3])
 def test_sort_matches_numpy(gpq, n):
     rng = np.random.default_rng(n)
     a = events(gpq, rng.integers(0, 1000, n) * 0.5, dst=rng.integers(0, 7, n), src=rng.integers(0, 50, n),
@@ -106,3 +106,21 @@ def test_select_matches_oracle(gpq, oracle, n, lps):
     assert np.array_equal(deq, s[marks.astype(bool)])
     scanned = pre
     assert np.array_equal(rest, s[:scanned][~marks[:scanned].astype(bool)])
+
+
+def test_select_large_prefix(gpq, oracle):
+    # 9 M records all inside the window: the per-block counts (one per 1024 records) take more
+    # than one sweep of the block scan (8 x 1024 counts per sweep)
+    n, lps = 9_000_000, 1 << 20
+    rng = np.random.default_rng(11)
+    ts = np.arange(n, dtype=np.float32)          # distinct, exact in float32
+    s = events(gpq, ts, dst=rng.integers(0, lps, n))
+    ts_bits = s["ts"].astype(np.int64)
+    wend = float(n + 1)
+    wend_bits = int(np.float32(wend).view(np.uint32))
+    k, marks = oracle.safe_select(ts_bits, s["dst"].astype(np.int32), lps, wend_bits - int(ts_bits[0]))
+    deq, rest, pre = dev_select(gpq, s, lps, wend)
+    assert pre == n and len(deq) == k
+    m = marks.astype(bool)
+    assert np.array_equal(deq, s[m])
+    assert np.array_equal(rest, s[~m])
