@@ -209,6 +209,22 @@ def per_queue_bench(gpq, wl, log):
     return out
 
 
+def phold_autotune_bench(gpq, wl, log, sample_end_time=5.0):
+    """The PHOLD layout autotuner on the workload's first `sample_end_time` time units: every
+    candidate layout's rate, the pick, and whether the benchmarked layout is the pick."""
+    t0 = time.perf_counter()
+    kw, drain, cands = gpq.phold_autotune(wl["num_lps"], init_per_lp=wl["init_per_lp"], seed=42,
+                                          end_time=wl["end_time"], sample_end_time=sample_end_time)
+    dt = time.perf_counter() - t0
+    bench_layout = (64, 32, FAR, DRAIN) if FAR else (QUEUES["laq"]["capacity_per_lp"], 32, 0, DRAIN)
+    pick = (kw["capacity_per_lp"], kw["stage_per_lp"], kw["far_per_lp"], drain)
+    out = {"sample_end_time": sample_end_time, "seconds": dt, "pick": dict(kw, drain=drain),
+           "benchmarked_layout_is_pick": bench_layout == pick and (kw["far_step"] == FAR_STEP or not FAR),
+           "candidates": cands}
+    log("phold_autotune: %s" % json.dumps(out))
+    return out
+
+
 def phold_batch_bench(gpq, log, instances=32, seed=5):
     """Instances of uneven size (LPs 2^U[12,18], 16 events per LP, end time 20) advanced together
     in one pool vs each alone through its own queue with a host sync per step."""
@@ -349,6 +365,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-queues", action="store_true", help="skip the same workload through the other queue kinds")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the PHOLD layout autotuner")
     ap.add_argument("--astar-drive", default="graph", choices=("graph", "plain"),
                     help="single A* query: hipGraph replays, or one launch sequence per step (eager, for tracing)")
     ap.add_argument("--serve-rates", default="50,100,200,400", help="Poisson arrival rates (requests/s) to sweep")
@@ -445,6 +462,8 @@ def main():
                                               for n, v in kernels.items() if n != dom}}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(wl, log)
+    if not args.no_autotune and args.queue == "laq":
+        line["phold_autotune"] = phold_autotune_bench(gpq, wl, log)
     if not args.no_queues:
         line["per_queue"] = per_queue_bench(gpq, wl, log)
         hashes_q = {v["trace_hash"] for v in line["per_queue"].values()}

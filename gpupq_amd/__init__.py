@@ -152,6 +152,28 @@ class Queue:
             pass
 
 
+def phold_autotune(num_lps, init_per_lp=32, seed=42, end_time=50.0, lookahead=0.1, p_self=0.5, sample_end_time=0.0,
+                   block_threads=256):
+    """The PHOLD layout autotuner (gpq_phold_autotune): times a fixed list of LAQ layouts on the
+    workload cut to end time sample_end_time (0 = 5) and returns (queue keyword arguments for
+    Queue("laq", ...), the Phold drain argument, [every candidate with its status and rate])."""
+    qc = QueueConfig(_lib.QUEUE_KINDS["laq"], num_lps, 0, 0, 0, 0, 0, lookahead, end_time, block_threads, 0, 0, 0.0)
+    pc = PholdConfig(num_lps, init_per_lp, seed, lookahead, end_time, min(int(round(p_self * 2 ** 32)), 0xFFFFFFFF),
+                     0, 0)
+    res = (_lib.PholdTuning * 16)()
+    n = ctypes.c_uint32()
+    check(lib.gpq_phold_autotune(ctypes.byref(pc), ctypes.byref(qc), sample_end_time, res, 16, ctypes.byref(n)),
+          "phold_autotune")
+    kw = dict(capacity_per_lp=qc.capacity_per_lp, stage_per_lp=qc.stage_per_lp, far_per_lp=qc.far_per_lp,
+              far_step=qc.far_step)
+    drain = {0: False, 1: True, 2: "fused"}[pc.drain]
+    cands = [dict(capacity_per_lp=r.capacity_per_lp, stage_per_lp=r.stage_per_lp, far_per_lp=r.far_per_lp,
+                  far_step=r.far_step, drain={0: False, 1: True, 2: "fused"}[r.drain],
+                  status={0: "ran", 1: "does not fit", 2: "trace differs"}[r.status],
+                  events_per_s=r.events_per_second) for r in res[:n.value]]
+    return kw, drain, cands
+
+
 class Phold:
     """PHOLD model on a queue (src/phold.cu, src/model.h:9-19)."""
 

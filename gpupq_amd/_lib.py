@@ -33,6 +33,11 @@ class PholdConfig(ctypes.Structure):
                 ("end_time", c_f32), ("p_self_u32", c_u32), ("checks", c_u32), ("drain", c_u32)]
 
 
+class PholdTuning(ctypes.Structure):
+    _fields_ = [("capacity_per_lp", c_u32), ("stage_per_lp", c_u32), ("far_per_lp", c_u32), ("far_step", c_f32),
+                ("drain", c_u32), ("status", c_u32), ("events_per_second", ctypes.c_double)]
+
+
 class DesReport(ctypes.Structure):
     _fields_ = [("steps", c_u64), ("windows", c_u64), ("events", c_u64), ("sim_seconds", ctypes.c_double),
                 ("events_per_second", ctypes.c_double), ("final_min_ts", c_f32), ("error_flags", c_u32)]
@@ -115,6 +120,8 @@ SIGNATURES = [
     ("gpq_queue_length", ctypes.c_int, [c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     ("gpq_queue_clear_lp", ctypes.c_int, [c_void_p, c_u32]),
     ("gpq_phold_create", ctypes.c_int, [c_void_p, ctypes.POINTER(PholdConfig), ctypes.POINTER(c_void_p)]),
+    ("gpq_phold_autotune", ctypes.c_int, [ctypes.POINTER(PholdConfig), ctypes.POINTER(QueueConfig), c_f32,
+                                          ctypes.POINTER(PholdTuning), c_u32, ctypes.POINTER(c_u32)]),
     ("gpq_phold_destroy", ctypes.c_int, [c_void_p]),
     ("gpq_phold_init", ctypes.c_int, [c_void_p]),
     ("gpq_phold_handle_next", ctypes.c_int, [c_void_p]),
@@ -166,7 +173,7 @@ def load(path=LIB_PATH):
     if _lib is not None:
         return _lib
     if not os.path.exists(path):
-        raise ImportError("gpupq_amd: %s is not built (run `python -m gpupq_amd.build`); "
+        raise ImportError("gpupq_amd: %s is not built (run `python gpupq_amd/build.py`); "
                           "there is no fallback implementation" % path)
     lib = ctypes.CDLL(path)
     for name, res, args in SIGNATURES:

@@ -1,6 +1,7 @@
 """Builds gpupq_amd/libgpupq.so from gpupq_amd/csrc/*.hip with hipcc for gfx950.
 
-    python -m gpupq_amd.build        (or python gpupq_amd/build.py)
+    python gpupq_amd/build.py        (or __graft_entry__.build(); `python -m gpupq_amd.build`
+                                      would first import the package, which needs the library)
 
 Every translation unit is compiled with -ffp-contract=off (bit-identical float
 arithmetic with the CPU reference: no fused multiply-adds) and with packed fp32

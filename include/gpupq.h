@@ -154,6 +154,29 @@ typedef struct {
 } gpq_phold_config;
 
 gpq_status gpq_phold_create(gpq_queue* q, const gpq_phold_config* cfg, gpq_phold** out);
+
+/* One layout the PHOLD autotuner timed, and its rate (0 unless status is 0). */
+typedef struct {
+  uint32_t capacity_per_lp;
+  uint32_t stage_per_lp;
+  uint32_t far_per_lp;
+  float far_step;               /* time units; 0 = 4 * lookahead */
+  uint32_t drain;
+  uint32_t status;              /* 0 ran; 1 could not hold the workload; 2 its trace hash differed
+                                   from the first layout's (never picked) */
+  double events_per_second;
+} gpq_phold_tuning;
+
+/* Autotuner (no reference counterpart; the reference fixes FEL_SIZE at compile time,
+ * src/local_array_queue.cuh:20): times a fixed list of LAQ layouts -- ring capacity per LP,
+ * staging slots, far store, near-horizon step and step kind -- on the model's own workload cut
+ * to end time sample_end_time (0 = 5, at most model->end_time), each in a fresh queue driven
+ * by GPQ_DRIVE_GRAPH, and writes the fastest into *queue (kind = GPQ_LAQ, capacity_per_lp,
+ * stage_per_lp, far_per_lp, far_step; the other fields are kept) and model->drain.  results
+ * (may be NULL) receives up to max_results candidates in the order timed, *n_results (may be
+ * NULL) their count.  GPQ_ERR_CAPACITY when no layout holds the workload. */
+gpq_status gpq_phold_autotune(gpq_phold_config* model, gpq_queue_config* queue, float sample_end_time,
+                              gpq_phold_tuning* results, uint32_t max_results, uint32_t* n_results);
 gpq_status gpq_phold_destroy(gpq_phold* m);
 /* model_init() (src/phold.cu:124-163): k initial events per LP, each chunk followed by queue_post_init(). */
 gpq_status gpq_phold_init(gpq_phold* m);

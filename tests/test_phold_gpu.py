@@ -175,3 +175,36 @@ def test_phold_2p20_lps_matches_golden(gpq, golden, kind, drain):
     assert g["rep"]["events"] == gd["events"]
     assert "%016x" % g["hash"] == gd["trace_hash"]
     assert g["pending"] == (1 << 20) * 32
+
+
+def test_phold_autotune_pick_matches_oracle(gpq, oracle):
+    # the autotuner times its layouts on a prefix (end time 2) and picks a ring / staging / far
+    # layout and step kind; every layout that ran committed the same prefix trace, and the full
+    # run with the pick matches the oracle per LP
+    L, k, T = 4096, 32, 8.0
+    kw, drain, cands = gpq.phold_autotune(L, init_per_lp=k, seed=42, end_time=T, sample_end_time=2.0)
+    assert len(cands) >= 2
+    ran = [c for c in cands if c["status"] == "ran"]
+    assert ran and all(c["events_per_s"] > 0 for c in ran)
+    assert not [c for c in cands if c["status"] == "trace differs"]
+    best = max(ran, key=lambda c: c["events_per_s"])
+    assert (best["capacity_per_lp"], best["stage_per_lp"], best["far_per_lp"], best["drain"]) == \
+        (kw["capacity_per_lp"], kw["stage_per_lp"], kw["far_per_lp"], drain)
+    res, cnt, hsh = oracle.phold("seq", L, k, T)
+    q = gpq.Queue("laq", num_lps=L, lookahead=0.1, horizon=T, **kw)
+    m = gpq.Phold(q, init_per_lp=k, seed=42, end_time=T, drain=drain)
+    m.model_init()
+    gpq.des_run(m, mode="graph", batch=16)
+    c, h = m.lp_state()
+    assert np.array_equal(c, cnt) and np.array_equal(h, hsh)
+    m.model_finish()
+    q.queue_finish()
+
+
+def test_phold_autotune_rejects_mismatched_sizes(gpq):
+    with pytest.raises(gpq.GpqError):
+        q = gpq.QueueConfig(0, 64, 0, 0, 0, 0, 0, 0.1, 1.0, 256, 0, 0, 0.0)
+        p = gpq.PholdConfig(128, 4, 1, 0.1, 1.0, 1 << 31, 0, 0)
+        import ctypes
+        gpq._lib.check(gpq.lib.gpq_phold_autotune(ctypes.byref(p), ctypes.byref(q), 0.0, None, 0, None),
+                       "phold_autotune")
