@@ -21,6 +21,8 @@ constexpr Candidate kCandidates[] = {
     {64, 16, 128, 0.0f, 2}, {64, 32, 128, 0.0f, 1}, {128, 32, 0, 0.0f, 1},
 };
 
+constexpr double kMargin = 1.02;
+
 // Events per second and trace hash of one candidate over the prefix; false when it cannot
 // hold the workload.
 bool time_candidate(const gpq_queue_config& qc, const gpq_phold_config& pc, double* rate, uint64_t* hash) {
@@ -86,7 +88,9 @@ extern "C" gpq_status gpq_phold_autotune(gpq_phold_config* model, gpq_queue_conf
     if (results && n < max_results)
       results[n] = gpq_phold_tuning{c.cap, c.stage, c.far, c.far_step_la * queue->lookahead, c.drain, status, rate};
     ++n;
-    if (rate > best) {
+    // candidates are listed in order of preference: a later one must be faster by more than
+    // the run-to-run spread of the timing (2%) to replace the pick
+    if (rate > (pick ? best * kMargin : 0.0)) {
       best = rate;
       pick = &c;
     }
