@@ -46,12 +46,12 @@ class Queue:
 
     def __init__(self, kind="laq", num_lps=1024, capacity_per_lp=64, stage_per_lp=16, node_size=0,
                  heap_nodes=0, total_capacity=0, lookahead=0.1, horizon=float("inf"), block_threads=256,
-                 collect_stats=False, far_per_lp=0, far_step=0.0):
+                 collect_stats=False, far_per_lp=0, far_step=0.0, update_ids=0, update_per_lp=0):
         self.kind = kind
         self.num_lps = num_lps
         cfg = QueueConfig(_lib.QUEUE_KINDS[kind], num_lps, capacity_per_lp, stage_per_lp, node_size,
                           heap_nodes, total_capacity, lookahead, horizon, block_threads, 1 if collect_stats else 0,
-                          far_per_lp, far_step)
+                          far_per_lp, far_step, update_ids, update_per_lp)
         self.cfg = cfg
         h = ctypes.c_void_p()
         check(lib.gpq_queue_create(ctypes.byref(cfg), ctypes.byref(h)), "queue_init")
@@ -106,6 +106,12 @@ class Queue:
     def insert(self, events):
         ev = np.ascontiguousarray(events, dtype=event_dtype)
         check(lib.gpq_queue_insert_host(self.handle, _ptr(ev), len(ev)), "queue_insert")
+
+    def insert_or_update(self, events):
+        """queue_insert_or_update (src/local_array_queue.cu:830-846): in decrease-key mode
+        (update_ids > 0) an id already in the queue gets the smaller key at the next post."""
+        ev = np.ascontiguousarray(events, dtype=event_dtype)
+        check(lib.gpq_queue_insert_or_update_host(self.handle, _ptr(ev), len(ev)), "queue_insert_or_update")
 
     def peek(self, set_done=False, max_items=None):
         max_items = max_items or max(self.num_lps, 1) * 4

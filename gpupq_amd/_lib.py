@@ -25,7 +25,8 @@ class QueueConfig(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("num_lps", c_u32), ("capacity_per_lp", c_u32),
                 ("stage_per_lp", c_u32), ("node_size", c_u32), ("heap_nodes", c_u32),
                 ("total_capacity", c_u64), ("lookahead", c_f32), ("horizon", c_f32),
-                ("block_threads", c_u32), ("collect_stats", c_u32), ("far_per_lp", c_u32), ("far_step", c_f32)]
+                ("block_threads", c_u32), ("collect_stats", c_u32), ("far_per_lp", c_u32), ("far_step", c_f32),
+                ("update_ids", c_u32), ("update_per_lp", c_u32)]
 
 
 class PholdConfig(ctypes.Structure):
@@ -114,6 +115,7 @@ SIGNATURES = [
     ("gpq_queue_error_flags", ctypes.c_int, [c_void_p, ctypes.POINTER(c_u32)]),
     ("gpq_queue_sync", ctypes.c_int, [c_void_p]),
     ("gpq_queue_insert_host", ctypes.c_int, [c_void_p, c_void_p, c_u32]),
+    ("gpq_queue_insert_or_update_host", ctypes.c_int, [c_void_p, c_void_p, c_u32]),
     ("gpq_queue_peek_host", ctypes.c_int, [c_void_p, ctypes.c_int, c_void_p, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     ("gpq_queue_lp_contents", ctypes.c_int, [c_void_p, c_u32, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     ("gpq_heap_node_contents", ctypes.c_int, [c_void_p, c_u32, c_void_p, c_u32, ctypes.POINTER(c_u32)]),

@@ -115,7 +115,8 @@ __global__ __launch_bounds__(256) void astar_step_kernel(Q q, AstarDev a) {
       const uint32_t g2 = g + nbr_cost(a.conn, k);
       if (atomicMin(&a.g[v], g2) <= g2) continue;
       const uint32_t h2 = grid_heur(a.conn, nx, ny, (int)a.tx, (int)a.ty);
-      q.insert(Event{g2 + h2, list_of(v, a.lists), h2, v});
+      const uint32_t list = list_of(v, a.lists);
+      q.insert_or_update(Event{g2 + h2, list, h2, v}, (int)list);   // decrease-key on a LAQ in that mode
     }
   }
   const uint32_t se = wave_sum_u32(expanded), sc = wave_sum_u32(newly_closed);
@@ -434,7 +435,7 @@ __global__ void astar_seed_kernel(Q q, AstarDev a, uint32_t source) {
   if (threadIdx.x || blockIdx.x) return;
   a.g[source] = 0;
   const uint32_t h0 = grid_heur(a.conn, (int)(source % a.width), (int)(source / a.width), (int)a.tx, (int)a.ty);
-  q.insert(Event{h0, list_of(source, a.lists), h0, source});
+  q.insert_or_update(Event{h0, list_of(source, a.lists), h0, source}, (int)list_of(source, a.lists));
   *a.st = AstarState{0xFFFFFFFFu, 0u, 0ull, 0ull, 0ull, 0u, 0u, {0xFFFFFFFFu, 0xFFFFFFFFu}, 0xFFFFFFFFu};
 }
 
@@ -450,6 +451,9 @@ gpq_status AstarModel::create(QueueBase* queue, const gpq_astar_config& c, Astar
   GPQ_CHECK_ARG(c.tile == 0 || c.tile == 16 || c.tile == 32, "A*: tile must be 0, 16 or 32");
   GPQ_CHECK_ARG(c.width >= 1 && c.height >= 1 && (uint64_t)c.width * c.height < (1ull << 31), "A*: bad grid size");
   GPQ_CHECK_ARG(c.sx < c.width && c.tx < c.width && c.sy < c.height && c.ty < c.height, "A*: endpoint outside grid");
+  GPQ_CHECK_ARG(!queue->cfg.update_ids || (queue->cfg.kind == GPQ_LAQ &&
+                                            (uint64_t)queue->cfg.update_ids >= (uint64_t)c.width * c.height),
+                "A*: a decrease-key LAQ needs update_ids >= the grid's cells");
   AstarModel* m = new AstarModel();
   m->cfg = c;
   m->q = queue;
@@ -506,6 +510,9 @@ gpq_status AstarModel::restart() {
   hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_blocks((cells + 31) / 32)), dim3(256), 0, q->stream, closed,
                      (cells + 31) / 32, 0u);
   GPQ_HIP(hipGetLastError());
+  // entries left by an earlier search are dropped as stale (their g is not the new g[cell]);
+  // in decrease-key mode they would hold their ids' marks, so the lists start empty
+  if (q->cfg.kind == GPQ_LAQ && q->cfg.update_ids) GPQ_TRY(static_cast<LaqQueue*>(q)->empty_all());
   const AstarDev d = view();
   const uint32_t source = cfg.sy * cfg.width + cfg.sx;
   GPQ_TRY(with_view(q, [&](auto qv) -> gpq_status {

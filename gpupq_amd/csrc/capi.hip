@@ -209,7 +209,7 @@ gpq_status gpq_queue_sync(gpq_queue* q) {
   return GPQ_OK;
 }
 
-gpq_status gpq_queue_insert_host(gpq_queue* q, const gpq_event* items, uint32_t n) {
+static gpq_status insert_host(gpq_queue* q, const gpq_event* items, uint32_t n, bool update) {
   GPQ_Q(q);
   if (n == 0) return GPQ_OK;
   if (!items) { set_error("null items"); return GPQ_ERR_INVALID; }
@@ -222,10 +222,18 @@ gpq_status gpq_queue_insert_host(gpq_queue* q, const gpq_event* items, uint32_t 
     set_error("copy of items failed");
     s = GPQ_ERR_HIP;
   }
-  if (s == GPQ_OK) s = gpq::launch_insert_items(q->impl, d, n);
+  if (s == GPQ_OK) s = gpq::launch_insert_items(q->impl, d, n, update);
   if (hipStreamSynchronize(q->impl->stream) != hipSuccess && s == GPQ_OK) s = GPQ_ERR_HIP;
   (void)hipFree(d);
   return s;
+}
+
+gpq_status gpq_queue_insert_host(gpq_queue* q, const gpq_event* items, uint32_t n) {
+  return insert_host(q, items, n, false);
+}
+
+gpq_status gpq_queue_insert_or_update_host(gpq_queue* q, const gpq_event* items, uint32_t n) {
+  return insert_host(q, items, n, true);
 }
 
 gpq_status gpq_queue_peek_host(gpq_queue* q, int set_done, gpq_event* items, int32_t* index, uint32_t max,

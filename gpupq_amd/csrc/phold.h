@@ -20,7 +20,8 @@ gpq_status with_view(QueueBase* q, F&& f) {
   }
 }
 
-gpq_status launch_insert_items(QueueBase* q, const Event* d_items, uint32_t n);
+// queue_insert (update = false) or queue_insert_or_update (true) of n device items.
+gpq_status launch_insert_items(QueueBase* q, const Event* d_items, uint32_t n, bool update = false);
 gpq_status launch_peek_items(QueueBase* q, int set_done, Event* d_out, int32_t* d_index, uint32_t* d_found,
                              uint32_t max);
 

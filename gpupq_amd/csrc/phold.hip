@@ -134,9 +134,11 @@ __global__ void phold_populate_kernel(Q q, PholdDev m, uint32_t j) {
 }
 
 template <class Q>
-__global__ void insert_items_kernel(Q q, const Event* items, uint32_t n) {
+__global__ void insert_items_kernel(Q q, const Event* items, uint32_t n, uint32_t update) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) q.insert(items[i]);
+  if (i >= n) return;
+  if (update) q.insert_or_update(items[i], (int)items[i].dst);
+  else q.insert(items[i]);
 }
 
 template <class Q>
@@ -200,9 +202,10 @@ uint64_t host_mix64(uint64_t x) {
 // ---------------------------------------------------------------------------------
 // generic launches over the queue kind
 // ---------------------------------------------------------------------------------
-gpq_status launch_insert_items(QueueBase* q, const Event* d_items, uint32_t n) {
+gpq_status launch_insert_items(QueueBase* q, const Event* d_items, uint32_t n, bool update) {
   return with_view(q, [&](auto v) -> gpq_status {
-    hipLaunchKernelGGL(insert_items_kernel<decltype(v)>, dim3(grid_for(n, 256)), dim3(256), 0, q->stream, v, d_items, n);
+    hipLaunchKernelGGL(insert_items_kernel<decltype(v)>, dim3(grid_for(n, 256)), dim3(256), 0, q->stream, v, d_items, n,
+                       update ? 1u : 0u);
     GPQ_HIP(hipGetLastError());
     return GPQ_OK;
   });

@@ -38,6 +38,14 @@ struct LaqView {
   // one buffer while the events it schedules land in the other.
   Event* stage_alt;
   uint32_t* ins_alt;
+  // Decrease-key mode (one-tier only; in_queue == nullptr = off): one bit per item id
+  // (Event.seq) marks an id with an entry in the queue, staged or in a ring; an
+  // insert_or_update of a marked id goes to the LP's update buffer, applied by queue_post.
+  uint32_t* in_queue;
+  Event* upd;                   // [L][upd_cap]
+  uint32_t* upd_n;
+  uint32_t upd_cap;
+  uint32_t update_ids;
   Event* far;                   // two-tier mode only (far_cap > 0)
   uint32_t* nfar;
   uint32_t* far_min;
@@ -63,6 +71,10 @@ struct LaqView {
 
   // queue_set_done: removes the head; the new head's ts is refreshed for the window scan.
   __device__ __forceinline__ void set_done(int index) const {
+    if (in_queue) {                       // the id leaves the queue
+      const uint32_t id = ring[index].seq;
+      atomicAnd(&in_queue[id >> 5], ~(1u << (id & 31u)));
+    }
     const uint32_t lp = (uint32_t)index >> cap_log2;
     const uint32_t mask = (1u << cap_log2) - 1u;
     const uint32_t r = (rd[lp] + 1u) & mask;
@@ -78,6 +90,13 @@ struct LaqView {
   __device__ __forceinline__ int length(int lp) const { return (int)len[lp]; }
   __device__ __forceinline__ void clear(int lp) const {
     const uint32_t mask = (1u << cap_log2) - 1u;
+    if (in_queue) {                       // every id of the LP leaves the queue
+      for (uint32_t i = 0; i < len[lp]; ++i) {
+        const uint32_t id = ring[((uint32_t)lp << cap_log2) | ((rd[lp] + i) & mask)].seq;
+        atomicAnd(&in_queue[id >> 5], ~(1u << (id & 31u)));
+      }
+      upd_n[lp] = 0;
+    }
     rd[lp] = (rd[lp] + len[lp]) & mask;
     len[lp] = 0;
     head[lp] = kInfBits;
@@ -86,9 +105,28 @@ struct LaqView {
       far_min[lp] = kInfBits;
     }
   }
-  // PHOLD events carry no identity for ==, so this is a plain insert (as in the reference
-  // without LOCAL_ARRAY_QUEUE_UPDATE_ELEMENTS, src/local_array_queue.cu:830-846).
-  __device__ __forceinline__ void insert_or_update(const Event& e, int) const { insert(e); }
+  // queue_insert_or_update (src/local_array_queue.cu:830-846).  Without decrease-key mode a
+  // plain insert (PHOLD events carry no identity).  With it: the first insert of an id marks
+  // it and is staged; a later one, while the id has an entry, goes to the update buffer of
+  // LP lp (= e.dst), and queue_post replaces the entry when the new key is smaller.
+  __device__ __forceinline__ void insert_or_update(const Event& e, int lp) const {
+    if (!in_queue) {
+      insert(e);
+      return;
+    }
+    if (e.seq >= update_ids) {
+      atomicOr(&ws->error, kErrBadState);
+      return;
+    }
+    const uint32_t bit = 1u << (e.seq & 31u);
+    if (atomicOr(&in_queue[e.seq >> 5], bit) & bit) {
+      const uint32_t slot = atomicAdd(&upd_n[lp], 1u);
+      if (slot < upd_cap) store_event(&upd[(size_t)lp * upd_cap + slot], e);
+      else atomicOr(&ws->error, kErrStageOverflow);
+    } else {
+      insert(e);
+    }
+  }
 };
 
 }  // namespace gpq

@@ -303,6 +303,69 @@ __global__ void laq_init_kernel(LaqView q) {
 
 __global__ void laq_clear_kernel(LaqView q, uint32_t lp) { q.clear((int)lp); }
 
+// Empties every LP and unmarks every id (decrease-key mode): the queue as after init.
+__global__ void laq_empty_kernel(LaqView q, uint32_t words) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < q.num_lps || i < words; i += stride) {
+    if (i < q.num_lps) {
+      q.len[i] = 0;
+      q.ins[i] = 0;
+      q.head[i] = kInfBits;
+      q.upd_n[i] = 0;
+    }
+    if (i < words) q.in_queue[i] = 0;
+  }
+}
+
+// Decrease-key pass of queue_post (src/local_array_queue.cu:350-416, execute_update), after the
+// staged inserts are merged: each LP's buffered updates in buffer order.  An update whose id
+// has an entry replaces it when its key is smaller and is dropped otherwise; an id whose entry
+// was removed after the update was buffered is inserted anew.  One thread per LP moves the
+// record down its sorted run; the block's partial minimum is lowered to the new head.
+__global__ void laq_update_kernel(LaqView q, uint32_t* partial) {
+  const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lp >= q.num_lps) return;
+  const uint32_t nu = q.upd_n[lp];
+  if (!nu) return;
+  const uint32_t C = 1u << q.cap_log2, mask = C - 1u;
+  Event* ring = q.ring + ((size_t)lp << q.cap_log2);
+  const uint32_t r = q.rd[lp];
+  uint32_t n = q.len[lp];
+  const uint32_t cnt = nu < q.upd_cap ? nu : q.upd_cap;
+  for (uint32_t u = 0; u < cnt; ++u) {
+    const Event e = load_event(&q.upd[(size_t)lp * q.upd_cap + u]);
+    uint32_t p = n;
+    for (uint32_t i = 0; i < n; ++i)
+      if (ring[(r + i) & mask].seq == e.seq) {
+        p = i;
+        break;
+      }
+    if (p < n) {
+      if (!key_less(e, load_event(&ring[(r + p) & mask]))) continue;   // not cheaper: dropped
+    } else {
+      if (n == C) {
+        atomicOr(&q.ws->error, kErrQueueOverflow);
+        continue;
+      }
+      atomicOr(&q.in_queue[e.seq >> 5], 1u << (e.seq & 31u));   // back in the queue
+      ++n;                                                     // slot p = old n is free
+    }
+    uint32_t j = p;   // larger records from j to p - 1 move up one slot
+    while (j > 0) {
+      const Event prev = load_event(&ring[(r + j - 1u) & mask]);
+      if (!key_less(e, prev)) break;
+      store_event(&ring[(r + j) & mask], prev);
+      --j;
+    }
+    store_event(&ring[(r + j) & mask], e);
+  }
+  q.len[lp] = n;
+  q.upd_n[lp] = 0;
+  const uint32_t h = n ? ring[r].ts : kInfBits;
+  q.head[lp] = h;
+  atomicMin(&partial[lp / kLaqBlock], h);
+}
+
 }  // namespace
 
 gpq_status LaqQueue::init() {
@@ -342,6 +405,18 @@ gpq_status LaqQueue::init() {
     GPQ_TRY(arena.alloc(&v_.nfar, L));
     GPQ_TRY(arena.alloc(&v_.far_min, L));
   }
+  if (cfg.update_ids) {
+    GPQ_CHECK_ARG(!cfg.far_per_lp, "LAQ decrease-key mode needs the one-tier LAQ (far_per_lp = 0)");
+    v_.update_ids = cfg.update_ids;
+    v_.upd_cap = cfg.update_per_lp ? cfg.update_per_lp : cfg.stage_per_lp;
+    GPQ_CHECK_ARG(v_.upd_cap <= 1024, "LAQ: update_per_lp must be <= 1024");
+    const size_t words = ((size_t)cfg.update_ids + 31) / 32;
+    GPQ_TRY(arena.alloc(&v_.in_queue, words));
+    GPQ_TRY(arena.alloc(&v_.upd, (size_t)L * v_.upd_cap));
+    GPQ_TRY(arena.alloc(&v_.upd_n, L));
+    GPQ_HIP(hipMemsetAsync(v_.in_queue, 0, sizeof(uint32_t) * words, stream));
+    GPQ_HIP(hipMemsetAsync(v_.upd_n, 0, sizeof(uint32_t) * L, stream));
+  }
   GPQ_TRY(arena.alloc(&partial, (v_.far_cap ? 3 : 1) * (size_t)partial_slots));
   hipLaunchKernelGGL(laq_init_kernel, dim3(ceil_div(L, 256)), dim3(256), 0, stream, v_);
   GPQ_HIP(hipGetLastError());
@@ -366,8 +441,22 @@ gpq_status LaqQueue::reset_stage_select() {
   return GPQ_OK;
 }
 
+gpq_status LaqQueue::empty_all() {
+  GPQ_CHECK_ARG(v_.in_queue, "LAQ: empty_all is for the decrease-key mode");
+  const uint32_t words = (v_.update_ids + 31) / 32;
+  const uint32_t n = std::max(cfg.num_lps, words);
+  hipLaunchKernelGGL(laq_empty_kernel, dim3(std::min<uint32_t>(ceil_div(n, 256), 4096)), dim3(256), 0, stream, v_,
+                     words);
+  GPQ_HIP(hipGetLastError());
+  return GPQ_OK;
+}
+
 gpq_status LaqQueue::merge_publish() {
   GPQ_TRY(merge_only());
+  if (v_.in_queue) {
+    hipLaunchKernelGGL(laq_update_kernel, dim3(ceil_div(cfg.num_lps, 256)), dim3(256), 0, stream, v_, partial);
+    GPQ_HIP(hipGetLastError());
+  }
   if (!v_.far_cap) return publish_window();
   if (profile_mark) GPQ_HIP(hipEventRecord(profile_mark, stream));
   hipLaunchKernelGGL(laq_far_publish_kernel, dim3(1), dim3(1024), 0, stream, partial, partial_slots, ws, cfg.lookahead,

@@ -40,6 +40,8 @@ class LaqQueue final : public QueueBase {
   // After a closing merge of such a step: both buffers are empty, the next step reads stage.
   gpq_status reset_stage_select();
   uint32_t group() const { return group_; }
+  // Decrease-key mode: every LP emptied, every id unmarked (a new search on the same queue).
+  gpq_status empty_all();
   float far_step() const { return far_step_; }
   // Two-tier mode, several windows: block b follows the near horizon of ws_array[win[b]].
   void set_block_windows(const uint32_t* win, const WindowState* ws_array) {

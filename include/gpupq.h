@@ -67,6 +67,12 @@ typedef struct {
                                 beyond the near horizon; the sorted ring keeps only events below it */
   float far_step;            /* LAQ two-tier: the near horizon moves to (minimum + lookahead + far_step)
                                 when a window reaches it (0 = 4 * lookahead) */
+  uint32_t update_ids;       /* LAQ decrease-key mode (0 = off; one-tier LAQ only): item ids are
+                                gpq_event.seq < update_ids (the A* cell); queue_insert_or_update of an
+                                id already in the queue replaces its entry when the new key is smaller
+                                (LOCAL_ARRAY_QUEUE_UPDATE_ELEMENTS, src/local_array_queue.cu:20-32,
+                                350-416, 830-846) */
+  uint32_t update_per_lp;    /* decrease-key mode: updates one LP may receive per step (0 = stage_per_lp) */
 } gpq_queue_config;
 
 typedef struct gpq_queue gpq_queue;
@@ -119,6 +125,9 @@ gpq_status gpq_queue_stats_get(gpq_queue* q, gpq_queue_stats* out);
  * src/test_laq.cu:115-145, src/test_gaq.cu:41-61): */
 /* queue_insert() for each of n host events, one device thread per event. */
 gpq_status gpq_queue_insert_host(gpq_queue* q, const gpq_event* items, uint32_t n);
+/* queue_insert_or_update(item, item.dst) (src/queue.cuh:66, src/local_array_queue.cu:830-846) for
+ * each of n host events; a plain insert unless the queue is a LAQ in decrease-key mode. */
+gpq_status gpq_queue_insert_or_update_host(gpq_queue* q, const gpq_event* items, uint32_t n);
 /* queue_peek() for every peek slot (LP for local queues, dequeue position for HQ/GAQ);
  * writes up to `max` found items and their indices, *found = count.  With set_done != 0
  * also calls queue_set_done() on each found item. */

@@ -48,7 +48,7 @@ def test_status_strings_without_device():
 def test_struct_layouts_match_header():
     from gpupq_amd import _lib
     assert ctypes.sizeof(_lib.Event) == 16
-    assert ctypes.sizeof(_lib.QueueConfig) == 56
+    assert ctypes.sizeof(_lib.QueueConfig) == 64
     assert ctypes.sizeof(_lib.PholdConfig) == 32
 
 
@@ -57,7 +57,7 @@ STRUCTS = {
     "gpq_des_report": "DesReport", "gpq_queue_stats": "QueueStats", "gpq_kernel_profile": "KernelProfile",
     "gpq_astar_config": "AstarConfig", "gpq_astar_report": "AstarReport", "gpq_query_spec": "QuerySpec",
     "gpq_batch_tuning": "BatchTuning", "gpq_serve_config": "ServeConfig", "gpq_request": "Request",
-    "gpq_serve_report": "ServeReport", "gpq_phold_instance": "PholdInstance",
+    "gpq_serve_report": "ServeReport", "gpq_phold_instance": "PholdInstance", "gpq_phold_tuning": "PholdTuning",
 }
 
 

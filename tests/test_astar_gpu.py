@@ -303,3 +303,33 @@ def test_path_image(gpq, tmp_path):
         data = f.read_bytes()
         assert data.startswith(b"P6\n48 32\n255\n") and len(data) == len(b"P6\n48 32\n255\n") + 3 * w * h
         a.model_finish()
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_single_query_laq_decrease_key(gpq, oracle, conn):
+    # open lists on a LAQ in decrease-key mode: one entry per cell, cheaper paths update it;
+    # exact costs, the same as the duplicate-insert lists, and a second search on the same queue
+    w, h = 97, 61
+    for seed in (1, 2, 3):
+        s, t = (3, 5), (90, 57)
+        with gpq.Queue("laq", num_lps=64, capacity_per_lp=128, stage_per_lp=32, lookahead=INF, horizon=INF,
+                       update_ids=w * h, collect_stats=True) as q:
+            a = gpq.Astar(q, w, h, connectivity=conn, source=s, target=t, seed=seed, threshold=0.25)
+            a.model_init()
+            ref_grid = oracle_grid(oracle, w, h, seed, 0, 0, thr(0.25), forced=(s, t))
+            cost, _ = oracle.astar(ref_grid, w, h, conn, s, t)
+            rep = a.run("graph", 8)
+            assert rep["cost"] == cost and rep["error_flags"] == 0
+            if cost >= 0:
+                check_path(a.path(), ref_grid, w, conn, s, t, cost)
+            a.model_init()                                   # the lists start empty again
+            rep2 = a.run("plain", 1)
+            assert rep2["cost"] == cost and rep2["error_flags"] == 0
+            a.model_finish()
+
+
+def test_single_query_decrease_key_needs_enough_ids(gpq):
+    with gpq.Queue("laq", num_lps=64, capacity_per_lp=128, stage_per_lp=32, lookahead=INF, horizon=INF,
+                   update_ids=100) as q:
+        with pytest.raises(gpq.GpqError):
+            gpq.Astar(q, 97, 61)

@@ -151,3 +151,45 @@ def test_clear_and_length(gpq, kind):
         assert q.queue_length(0) == 0 and q.queue_length(1) == 1
         q.queue_post()
         assert q.queue_get_min_ts() == 3.0
+
+
+def test_laq_decrease_key(gpq):
+    # LOCAL_ARRAY_QUEUE_UPDATE_ELEMENTS (src/local_array_queue.cu:20-32, 350-416, 830-846): an id
+    # (seq) already in the queue keeps one entry, which takes the smaller key at the next post
+    inf = float("inf")
+    with gpq.Queue("laq", num_lps=2, capacity_per_lp=8, stage_per_lp=4, lookahead=inf, horizon=inf,
+                   update_ids=64) as q:
+        def lp0():
+            return [(gpq.bits_f32(e["ts"]), int(e["seq"])) for e in q.lp_contents(0)]
+        q.insert_or_update(ev(gpq, [(10, 0, 0, 5), (20, 0, 0, 6), (4, 1, 0, 9)]))
+        q.queue_post()
+        assert lp0() == [(10.0, 5), (20.0, 6)]
+        q.insert_or_update(ev(gpq, [(8, 0, 0, 6)]))           # cheaper: id 6 moves to the front
+        q.queue_post()
+        assert lp0() == [(8.0, 6), (10.0, 5)] and q.queue_get_min_ts() == 4.0
+        q.insert_or_update(ev(gpq, [(12, 0, 0, 5)]))          # dearer: dropped
+        q.queue_post()
+        assert lp0() == [(8.0, 6), (10.0, 5)]
+        q.insert_or_update(ev(gpq, [(9, 0, 0, 7)]))           # new id: inserted
+        q.queue_post()
+        assert lp0() == [(8.0, 6), (9.0, 7), (10.0, 5)]
+        items, _ = q.peek(set_done=True)                      # pops id 6 (LP0) and id 9 (LP1)
+        assert sorted(int(s) for s in items["seq"]) == [6, 9]
+        q.queue_post()
+        q.insert_or_update(ev(gpq, [(30, 0, 0, 6)]))          # id 6 left the queue: a plain insert
+        q.queue_post()
+        assert lp0() == [(9.0, 7), (10.0, 5), (30.0, 6)]
+        q.insert_or_update(ev(gpq, [(7, 0, 0, 5), (6, 0, 0, 5)]))   # two updates of one id in one step
+        q.queue_post()
+        assert lp0() == [(6.0, 5), (9.0, 7), (30.0, 6)]
+        q.insert_or_update(ev(gpq, [(2, 0, 0, 5)]))           # buffered, then its entry is popped
+        q.peek(set_done=True)
+        q.queue_post()                                        # so it is inserted anew
+        assert lp0() == [(2.0, 5), (9.0, 7), (30.0, 6)]
+        assert q.queue_check_phold() == 3 and q.error_flags() == 0
+
+
+def test_laq_decrease_key_needs_one_tier(gpq):
+    with pytest.raises(gpq.GpqError):
+        gpq.Queue("laq", num_lps=4, capacity_per_lp=64, stage_per_lp=4, lookahead=0.1, horizon=1.0, far_per_lp=64,
+                  update_ids=16)
