@@ -26,7 +26,7 @@ class QueueConfig(ctypes.Structure):
                 ("stage_per_lp", c_u32), ("node_size", c_u32), ("heap_nodes", c_u32),
                 ("total_capacity", c_u64), ("lookahead", c_f32), ("horizon", c_f32),
                 ("block_threads", c_u32), ("collect_stats", c_u32), ("far_per_lp", c_u32), ("far_step", c_f32),
-                ("update_ids", c_u32), ("update_per_lp", c_u32)]
+                ("update_ids", c_u32), ("update_per_lp", c_u32), ("heaps", c_u32)]
 
 
 class PholdConfig(ctypes.Structure):

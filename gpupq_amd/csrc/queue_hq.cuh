@@ -34,27 +34,40 @@ struct HqState {
   Event* chunk_alt[kHqMaxChunks];
 };
 
+// Several heaps (HQ_NUM_QUEUES): heap h owns the LPs [h * lps_per_heap, (h + 1) * lps_per_heap),
+// its state st[h], its insert area enq + h * enq_cap and its dequeue slots deq + h * lps_per_heap.
 struct HqView {
   Event* enq;
   Event* deq;
   HqState* st;
   WindowState* ws;
-  uint32_t enq_cap;
+  uint32_t enq_cap;        // per heap
   uint32_t num_lps;
+  uint32_t heaps;
+  uint32_t lps_per_heap;
 
   __device__ __forceinline__ uint32_t slots() const { return num_lps; }
+  // heap_queue_insert (src/heap_queue.cu:1809-1826): into the insert area of the LP's heap.
   __device__ __forceinline__ bool insert(const Event& e) const {
-    const uint32_t pos = wave_append_slot(&st->m);
+    uint32_t h = 0, pos;
+    if (heaps == 1) {
+      pos = wave_append_slot(&st->m);
+    } else {
+      h = e.dst / lps_per_heap;
+      pos = atomicAdd(&st[h].m, 1u);
+    }
     if (pos >= enq_cap) {
       atomicOr(&ws->error, kErrStageOverflow);
       return false;
     }
-    store_event(enq + pos, e);
+    store_event(enq + (size_t)h * enq_cap + pos, e);
     return true;
   }
-  // heap_queue_peek(item, pos): dequeue-buffer slot pos (src/heap_queue.cu:1860-1896).
+  // heap_queue_peek(item, pos): dequeue slot pos, i.e. slot pos % lps_per_heap of heap
+  // pos / lps_per_heap (src/heap_queue.cu:1860-1896).
   __device__ __forceinline__ int peek(Event** item, int pos) const {
-    if ((uint32_t)pos >= st->deq_n) return -1;
+    const uint32_t h = (uint32_t)pos / lps_per_heap, i = (uint32_t)pos - h * lps_per_heap;
+    if (i >= st[h].deq_n) return -1;
     *item = deq + pos;
     return pos;
   }

@@ -66,21 +66,28 @@ __global__ void hq_plan_merge_root(HqState* st, const uint32_t* counts, const Ev
   st->bound = c0 ? nodes[c0 - 1] : Event{0u, 0u, 0u, 0u};
 }
 
-__global__ void hq_window_kernel(HqState* st, const Event* merged, WindowState* ws, float lookahead, float horizon,
-                                 uint32_t select) {
-  const uint32_t g = st->mcount ? merged[0].ts : kInfBits;
+// Step 2, window: the smallest record over every heap's merged buffer (heap h's at
+// merged + h * mstride) starts it; the reference takes the minimum over its heaps' dequeue
+// buffers (src/heap_queue.cu:1783-1806).
+__global__ void hq_window_kernel(HqState* st, uint32_t heaps, const Event* merged, size_t mstride, WindowState* ws,
+                                 float lookahead, float horizon, uint32_t select) {
+  uint32_t g = kInfBits;
+  for (uint32_t h = 0; h < heaps; ++h) {
+    const uint32_t v = st[h].mcount ? merged[h * mstride].ts : kInfBits;
+    g = v < g ? v : g;
+    st[h].select = select;
+    if (!select) {
+      st[h].prefix = 0;
+      st[h].deq_n = 0;
+      st[h].rest_n = 0;
+    }
+  }
   float wend = bits_to_f(g) + lookahead;
   if (horizon < wend) wend = horizon;
   ws->gmin = g;
   ws->wend = f_to_bits(wend);
   ws->done = (g == kInfBits || !(bits_to_f(g) < horizon)) ? 1u : 0u;
   ws->windows += 1;
-  st->select = select;
-  if (!select) {
-    st->prefix = 0;
-    st->deq_n = 0;
-    st->rest_n = 0;
-  }
 }
 
 // Step 3: R = (unselected part of the scanned prefix) ++ M[prefix:]; root <- R[0:r), carry <- rest.
@@ -281,8 +288,7 @@ __global__ void hq_plan_insert_level(HqState* st, uint32_t* counts, Event* nodes
   tasks[1] = c;
 }
 
-__global__ void hq_pending_kernel(const HqState* st, const uint32_t* counts, uint32_t num_nodes,
-                                  unsigned long long* out) {
+__global__ void hq_pending_kernel(const uint32_t* counts, uint32_t num_nodes, unsigned long long* out) {
   if (threadIdx.x || blockIdx.x) return;
   unsigned long long s = 0;
   for (uint32_t v = 0; v < num_nodes; ++v) s += counts[v];
@@ -299,69 +305,95 @@ uint32_t levels_with_children(uint32_t nodes) {
 
 gpq_status HqQueue::init() {
   GPQ_CHECK_ARG(cfg.node_size >= 1, "HQ: node_size (r) must be >= 1");
-  const uint64_t total = (uint64_t)cfg.num_lps * (cfg.capacity_per_lp ? cfg.capacity_per_lp : 1);
+  heaps_ = cfg.heaps ? cfg.heaps : 1u;
+  GPQ_CHECK_ARG(heaps_ <= cfg.num_lps && heaps_ <= 1024, "HQ: heaps must be in [1, min(num_lps, 1024)]");
+  lps_per_heap_ = (uint32_t)ceil_div(cfg.num_lps, heaps_);
+  const uint64_t total = (uint64_t)lps_per_heap_ * (cfg.capacity_per_lp ? cfg.capacity_per_lp : 1);
   const uint64_t nodes = cfg.heap_nodes ? cfg.heap_nodes : (total + cfg.node_size - 1) / cfg.node_size + 1;
   GPQ_CHECK_ARG(nodes >= 1 && nodes < (1u << 20), "HQ: bad heap_nodes");
   r_ = cfg.node_size;
   nodes_ = (uint32_t)nodes;
+  // every heap takes a whole step's inserts (the events of one step may all go to one heap)
   const uint64_t enq_cap = (uint64_t)cfg.num_lps * (cfg.stage_per_lp ? cfg.stage_per_lp : 1);
   GPQ_CHECK_ARG(enq_cap + r_ < (1ull << 31), "HQ: num_lps * stage_per_lp + node_size must be < 2^31");
   GPQ_CHECK_ARG(enq_cap <= (uint64_t)(kHqMaxChunks - 1) * r_, "HQ: num_lps * stage_per_lp must be <= 7 * node_size");
-  GPQ_CHECK_ARG((uint64_t)nodes_ * r_ < (1ull << 34), "HQ: heap too large");
+  GPQ_CHECK_ARG((uint64_t)heaps_ * nodes_ * r_ < (1ull << 34), "HQ: heap too large");
   v_.num_lps = cfg.num_lps;
   v_.enq_cap = (uint32_t)enq_cap;
+  v_.heaps = heaps_;
+  v_.lps_per_heap = lps_per_heap_;
   levels_ = levels_with_children(nodes_);
   slots_ = levels_ ? (1u << (levels_ - 1)) : 1u;
   depth_ = 0;
   while ((1u << (depth_ + 1)) - 1u < nodes_) ++depth_;   // deepest level index of any node
-  const uint64_t mcap = r_ + enq_cap;
-  GPQ_TRY(arena.alloc(&nodes_buf_, (size_t)nodes_ * r_));
-  GPQ_TRY(arena.alloc(&counts_, nodes_));
-  GPQ_TRY(arena.alloc(&dirty_, nodes_));
-  GPQ_TRY(arena.alloc(&v_.enq, enq_cap));
+  mcap_ = r_ + enq_cap;
+  GPQ_TRY(arena.alloc(&nodes_buf_, (size_t)heaps_ * nodes_ * r_));
+  GPQ_TRY(arena.alloc(&counts_, (size_t)heaps_ * nodes_));
+  GPQ_TRY(arena.alloc(&dirty_, (size_t)heaps_ * nodes_));
+  GPQ_TRY(arena.alloc(&v_.enq, (size_t)heaps_ * enq_cap));
   GPQ_TRY(arena.alloc(&sort_[0], enq_cap));
   GPQ_TRY(arena.alloc(&sort_[1], enq_cap));
-  GPQ_TRY(arena.alloc(&merged_, mcap));
-  GPQ_TRY(arena.alloc(&rest_, mcap));
-  GPQ_TRY(arena.alloc(&carry_, mcap));
+  GPQ_TRY(arena.alloc(&merged_, (size_t)heaps_ * mcap_));
+  GPQ_TRY(arena.alloc(&rest_, mcap_));
+  GPQ_TRY(arena.alloc(&carry_, mcap_));
   GPQ_TRY(arena.alloc(&chunk_buf_, (size_t)kHqMaxChunks * r_));
   GPQ_TRY(arena.alloc(&tmp2_, (size_t)slots_ * 2u * r_));
   GPQ_TRY(arena.alloc(&tmp1_, (size_t)slots_ * r_ + r_));
-  GPQ_TRY(arena.alloc(&v_.deq, cfg.num_lps));
-  GPQ_TRY(arena.alloc(&v_.st, 1));
+  GPQ_TRY(arena.alloc(&v_.deq, (size_t)heaps_ * lps_per_heap_));
+  GPQ_TRY(arena.alloc(&v_.st, heaps_));
   GPQ_TRY(arena.alloc(&first_, cfg.num_lps));
-  GPQ_TRY(arena.alloc(&scratch_, 2 * (size_t)ceil_div(mcap, 1024) + 2));
+  GPQ_TRY(arena.alloc(&scratch_, 2 * (size_t)ceil_div(mcap_, 1024) + 2));
   GPQ_TRY(arena.alloc(&tasks_, 3 * (size_t)slots_ + 8));
   GPQ_TRY(arena.alloc(&sum_, 1));
   v_.ws = ws;
   partial_slots = 1;
   GPQ_TRY(arena.alloc(&partial, 1));
   const uint32_t n = std::max(nodes_, cfg.num_lps);
-  hipLaunchKernelGGL(hq_init_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, stream, v_.st, counts_, dirty_, nodes_,
-                     first_, cfg.num_lps, merged_, ws);
+  for (uint32_t h = 0; h < heaps_; ++h) {
+    hipLaunchKernelGGL(hq_init_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, stream, v_.st + h,
+                       counts_ + (size_t)h * nodes_, dirty_ + (size_t)h * nodes_, nodes_, first_, cfg.num_lps,
+                       merged_ + (size_t)h * mcap_, ws);
+    GPQ_HIP(hipGetLastError());
+  }
+  return GPQ_OK;
+}
+
+// One post over every heap: the merges of inserts into the roots, the global window, then per
+// heap the selection and the heap maintenance.  Heaps run one after another on the stream and
+// share the sort, selection and task buffers.
+gpq_status HqQueue::step(bool select) {
+  MergeTask* t = tasks_;
+  for (uint32_t h = 0; h < heaps_; ++h) {
+    HqState* st = v_.st + h;
+    Event* sorted = nullptr;
+    GPQ_TRY(sort_records(v_.enq + (size_t)h * v_.enq_cap, sort_[0], sort_[1], &st->m, v_.enq_cap, stream, &sorted));
+    hipLaunchKernelGGL(hq_plan_merge_root, dim3(1), dim3(1), 0, stream, st, counts_ + (size_t)h * nodes_,
+                       nodes_buf_ + (size_t)h * nodes_ * r_, sorted, merged_ + (size_t)h * mcap_, v_.enq_cap, t);
+    GPQ_TRY(merge_tasks(t, 1, mcap_, stream));
+  }
+  hipLaunchKernelGGL(hq_window_kernel, dim3(1), dim3(1), 0, stream, v_.st, heaps_, merged_, (size_t)mcap_, ws,
+                     cfg.lookahead, cfg.horizon, select ? 1u : 0u);
+  for (uint32_t h = 0; h < heaps_; ++h) GPQ_TRY(step_heap(h, select));
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }
 
-gpq_status HqQueue::step(bool select) {
-  HqState* st = v_.st;
+gpq_status HqQueue::step_heap(uint32_t h, bool select) {
+  HqState* st = v_.st + h;
+  uint32_t* counts = counts_ + (size_t)h * nodes_;
+  uint8_t* dirty = dirty_ + (size_t)h * nodes_;
+  Event* nodes = nodes_buf_ + (size_t)h * nodes_ * r_;
+  Event* merged = merged_ + (size_t)h * mcap_;
   MergeTask* t = tasks_;
-  const uint64_t mcap = (uint64_t)r_ + v_.enq_cap;
-  Event* sorted = nullptr;
-  GPQ_TRY(sort_records(v_.enq, sort_[0], sort_[1], &st->m, v_.enq_cap, stream, &sorted));
-  hipLaunchKernelGGL(hq_plan_merge_root, dim3(1), dim3(1), 0, stream, st, counts_, nodes_buf_, sorted, merged_,
-                     v_.enq_cap, t);
-  GPQ_TRY(merge_tasks(t, 1, mcap, stream));
-  hipLaunchKernelGGL(hq_window_kernel, dim3(1), dim3(1), 0, stream, st, merged_, ws, cfg.lookahead, cfg.horizon,
-                     select ? 1u : 0u);
   if (select) {
     const SelectBufs b{first_, scratch_, &st->prefix, &st->deq_n, &st->rest_n};
-    GPQ_TRY(select_safe(&st->mptr, &st->mcount, (uint32_t)mcap, &ws->wend, &st->bound, &st->bound_valid, b, v_.deq,
-                        rest_, stream));
+    GPQ_TRY(select_safe(&st->mptr, &st->mcount, (uint32_t)mcap_, &ws->wend, &st->bound, &st->bound_valid, b,
+                        v_.deq + (size_t)h * lps_per_heap_, rest_, stream));
   }
-  hipLaunchKernelGGL(hq_plan_root, dim3(1), dim3(1), 0, stream, st, counts_, nodes_buf_, merged_, rest_, carry_, r_, t);
-  GPQ_TRY(merge_tasks(t, 1, mcap, stream));
-  hipLaunchKernelGGL(hq_plan_refill, dim3(1), dim3(1), 0, stream, st, counts_, nodes_buf_, dirty_, r_, tmp1_ + (size_t)slots_ * r_, t);
+  hipLaunchKernelGGL(hq_plan_root, dim3(1), dim3(1), 0, stream, st, counts, nodes, merged, rest_, carry_, r_, t);
+  GPQ_TRY(merge_tasks(t, 1, mcap_, stream));
+  hipLaunchKernelGGL(hq_plan_refill, dim3(1), dim3(1), 0, stream, st, counts, nodes, dirty, r_,
+                     tmp1_ + (size_t)slots_ * r_, t);
   GPQ_TRY(merge_tasks(t, 1, r_, stream));
   GPQ_TRY(merge_tasks(t + 1, 1, r_, stream));
   MergeTask* t1 = tasks_;
@@ -369,17 +401,17 @@ gpq_status HqQueue::step(bool select) {
   MergeTask* t3 = tasks_ + 2 * (size_t)slots_;
   for (uint32_t lv = 0; lv < levels_; ++lv) {
     const uint32_t ns = 1u << lv;
-    hipLaunchKernelGGL(hq_plan_level, dim3(ceil_div(ns, 256)), dim3(256), 0, stream, st, counts_, nodes_buf_, dirty_,
-                       r_, nodes_, lv, tmp2_, tmp1_, t1, t2, t3);
+    hipLaunchKernelGGL(hq_plan_level, dim3(ceil_div(ns, 256)), dim3(256), 0, stream, st, counts, nodes, dirty, r_,
+                       nodes_, lv, tmp2_, tmp1_, t1, t2, t3);
     GPQ_TRY(merge_tasks(t1, ns, 2ull * r_, stream));
     GPQ_TRY(merge_tasks(t2, ns, 3ull * r_, stream));
     GPQ_TRY(merge_tasks(t3, ns, r_, stream));
   }
-  hipLaunchKernelGGL(hq_plan_chunks, dim3(1), dim3(1), 0, stream, st, counts_, carry_, chunk_buf_, r_, nodes_, ws);
+  hipLaunchKernelGGL(hq_plan_chunks, dim3(1), dim3(1), 0, stream, st, counts, carry_, chunk_buf_, r_, nodes_, ws);
   const uint32_t chunks = std::min<uint32_t>(kHqMaxChunks, ceil_div(v_.enq_cap, r_) + 1);
   for (uint32_t c = 0; c < chunks; ++c) {
     for (uint32_t lv = 1; lv <= depth_; ++lv) {
-      hipLaunchKernelGGL(hq_plan_insert_level, dim3(1), dim3(1), 0, stream, st, counts_, nodes_buf_, r_, c, lv, tmp1_, t);
+      hipLaunchKernelGGL(hq_plan_insert_level, dim3(1), dim3(1), 0, stream, st, counts, nodes, r_, c, lv, tmp1_, t);
       GPQ_TRY(merge_tasks(t, 1, 2ull * r_, stream));
       GPQ_TRY(merge_tasks(t + 1, 1, 2ull * r_, stream));
     }
@@ -392,7 +424,7 @@ gpq_status HqQueue::post() { return step(true); }
 gpq_status HqQueue::post_init() { return step(false); }
 
 gpq_status HqQueue::pending(uint64_t* n) {
-  hipLaunchKernelGGL(hq_pending_kernel, dim3(1), dim3(1), 0, stream, v_.st, counts_, nodes_,
+  hipLaunchKernelGGL(hq_pending_kernel, dim3(1), dim3(1), 0, stream, counts_, heaps_ * nodes_,
                      reinterpret_cast<unsigned long long*>(sum_));
   GPQ_HIP(hipGetLastError());
   GPQ_HIP(hipMemcpyAsync(n, sum_, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
@@ -401,11 +433,12 @@ gpq_status HqQueue::pending(uint64_t* n) {
 }
 
 gpq_status HqQueue::snapshot(std::vector<Event>* out, std::vector<uint32_t>* counts) {
-  counts->resize(nodes_);
-  GPQ_HIP(hipMemcpyAsync(counts->data(), counts_, sizeof(uint32_t) * nodes_, hipMemcpyDeviceToHost, stream));
+  const uint32_t all = heaps_ * nodes_;   // node v of heap h at h * nodes_ + v, as in nodes_buf_
+  counts->resize(all);
+  GPQ_HIP(hipMemcpyAsync(counts->data(), counts_, sizeof(uint32_t) * all, hipMemcpyDeviceToHost, stream));
   GPQ_HIP(hipStreamSynchronize(stream));
   out->clear();
-  for (uint32_t v = 0; v < nodes_; ++v) {
+  for (uint32_t v = 0; v < all; ++v) {
     const uint32_t c = (*counts)[v];
     if (!c) continue;
     const size_t at = out->size();
@@ -418,7 +451,7 @@ gpq_status HqQueue::snapshot(std::vector<Event>* out, std::vector<uint32_t>* cou
 }
 
 gpq_status HqQueue::node_contents(uint32_t v, Event* out, uint32_t cap, uint32_t* n) {
-  GPQ_CHECK_ARG(v < nodes_, "node out of range");
+  GPQ_CHECK_ARG(v < heaps_ * nodes_, "node out of range");
   uint32_t c = 0;
   GPQ_HIP(hipMemcpyAsync(&c, counts_ + v, 4, hipMemcpyDeviceToHost, stream));
   GPQ_HIP(hipStreamSynchronize(stream));

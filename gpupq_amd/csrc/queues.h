@@ -115,9 +115,12 @@ class HqQueue final : public QueueBase {
 
  private:
   gpq_status step(bool select);
+  gpq_status step_heap(uint32_t h, bool select);
   gpq_status snapshot(std::vector<Event>* out, std::vector<uint32_t>* counts);
   HqView v_{};
   uint32_t r_ = 0, nodes_ = 0, levels_ = 0, slots_ = 1, depth_ = 0;
+  uint32_t heaps_ = 1, lps_per_heap_ = 0;   // nodes_ is per heap
+  uint64_t mcap_ = 0;                       // merged-buffer records per heap
   Event* nodes_buf_ = nullptr;
   uint32_t* counts_ = nullptr;
   uint8_t* dirty_ = nullptr;

@@ -73,6 +73,11 @@ typedef struct {
                                 (LOCAL_ARRAY_QUEUE_UPDATE_ELEMENTS, src/local_array_queue.cu:20-32,
                                 350-416, 830-846) */
   uint32_t update_per_lp;    /* decrease-key mode: updates one LP may receive per step (0 = stage_per_lp) */
+  uint32_t heaps;            /* HQ: independent heaps, HQ_NUM_QUEUES (src/config.h:87; 0 = 1).  Heap h
+                                holds the LPs [h * ceil(num_lps / heaps), (h + 1) * ceil(num_lps / heaps))
+                                (src/heap_queue.cu:1811); node_size and heap_nodes apply to each heap, and
+                                each heap can take num_lps * stage_per_lp inserts per step; the window is
+                                global (the minimum over all heaps) */
 } gpq_queue_config;
 
 typedef struct gpq_queue gpq_queue;
@@ -136,7 +141,8 @@ gpq_status gpq_queue_peek_host(gpq_queue* q, int set_done, gpq_event* items, int
 /* Contents owned by one LP in key order (local queues; HQ/GAQ: all pending items of that LP). */
 gpq_status gpq_queue_lp_contents(gpq_queue* q, uint32_t lp, gpq_event* out, uint32_t cap, uint32_t* n);
 /* Records of one node of the parallel heap (GPQ_HQ), in key order; node 0 is the root
- * (the reference's tests read heap_queue_data.fel per node, src/test_hq.cu:580-917). */
+ * (the reference's tests read heap_queue_data.fel per node, src/test_hq.cu:580-917).  With
+ * several heaps, node h * nodes_per_heap + v is node v of heap h. */
 gpq_status gpq_heap_node_contents(gpq_queue* q, uint32_t node, gpq_event* out, uint32_t cap, uint32_t* n);
 /* queue_length(lp) / queue_is_empty(lp) / queue_clear(lp) (src/queue.cuh:58-66). */
 gpq_status gpq_queue_length(gpq_queue* q, uint32_t lp, uint32_t* n);

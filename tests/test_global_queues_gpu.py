@@ -108,33 +108,40 @@ def test_hq_multi_insert(gpq):
         check_heap(gpq, q, 8)
 
 
-def check_heap(gpq, q, nodes):
-    """Every node sorted, parent maximum <= child minimum, counts add up."""
+def check_heap(gpq, q, nodes, heaps=1, lps_per_heap=None):
+    """Every node sorted, parent maximum <= child minimum, counts add up; with several heaps,
+    node v of heap h is h * nodes + v and holds only records of heap h's LPs."""
     key = lambda e: (int(e["ts"]), int(e["src"]), int(e["seq"]))
-    content = [q.heap_node(v) for v in range(nodes)]
     total = 0
-    for v, c in enumerate(content):
-        ks = [key(e) for e in c]
-        assert ks == sorted(ks), "node %d unsorted" % v
-        total += len(c)
-        for ch in (2 * v + 1, 2 * v + 2):
-            if ch < nodes and len(content[ch]) and len(c):
-                assert ks[-1] <= key(content[ch][0]), "heap order violated at %d -> %d" % (v, ch)
+    for h in range(heaps):
+        content = [q.heap_node(h * nodes + v) for v in range(nodes)]
+        for v, c in enumerate(content):
+            ks = [key(e) for e in c]
+            assert ks == sorted(ks), "heap %d node %d unsorted" % (h, v)
+            total += len(c)
+            if heaps > 1:
+                assert all(int(e["dst"]) // lps_per_heap == h for e in c), "record in the wrong heap"
+            for ch in (2 * v + 1, 2 * v + 2):
+                if ch < nodes and len(content[ch]) and len(c):
+                    assert ks[-1] <= key(content[ch][0]), "heap %d order violated at %d -> %d" % (h, v, ch)
     assert total == q.queue_check_phold()
     return total
 
 
-@pytest.mark.parametrize("kind,r", [("gaq", 0), ("hq", 4), ("hq", 16), ("hq", 64)])
-def test_random_rounds_against_model(gpq, kind, r):
-    rng = np.random.default_rng(r + 11)
+@pytest.mark.parametrize("kind,r,heaps", [("gaq", 0, 1), ("hq", 4, 1), ("hq", 16, 1), ("hq", 64, 1), ("hq", 8, 3),
+                                          ("hq", 16, 13)])
+def test_random_rounds_against_model(gpq, kind, r, heaps):
+    rng = np.random.default_rng(r + 11 + heaps)
     L, la = 13, 0.5
     nodes = 128
+    per_heap = -(-L // heaps)
+    stage = max(1, min(8, 7 * r // L)) if kind == "hq" else 8
     if kind == "gaq":
         q = gpq.Queue("gaq", num_lps=L, total_capacity=4096, stage_per_lp=8, lookahead=la, horizon=1e9)
     else:
-        q = gpq.Queue("hq", num_lps=L, node_size=r, heap_nodes=nodes, stage_per_lp=max(1, min(8, 7 * r // L)),
-                      lookahead=la, horizon=1e9)
-    enq_cap = L * (8 if kind == "gaq" else max(1, min(8, 7 * r // L)))
+        q = gpq.Queue("hq", num_lps=L, node_size=r, heap_nodes=nodes, stage_per_lp=stage, lookahead=la, horizon=1e9,
+                      heaps=heaps)
+    enq_cap = L * stage                                   # inserts per step (each heap takes them all)
     model = {}
     seq = 0
     now = 0.0
@@ -169,7 +176,7 @@ def test_random_rounds_against_model(gpq, kind, r):
                 for (t, s_, k, d) in got:
                     model[d].remove((t, s_, k))
             if kind == "hq":
-                check_heap(gpq, q, nodes)
+                check_heap(gpq, q, nodes, heaps, per_heap)
             now += 0.5
         assert q.queue_check_phold() == sum(len(v) for v in model.values())
         assert q.error_flags() == 0

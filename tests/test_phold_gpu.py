@@ -16,12 +16,16 @@ QUEUE_ARGS = {
     # the local array queue in two-tier mode: a 64-record sorted ring below the near horizon,
     # a 128-record unsorted far store above it
     "laq2": dict(capacity_per_lp=64, stage_per_lp=16, far_per_lp=128),
+    # four parallel heaps, each owning a quarter of the LPs (HQ_NUM_QUEUES = 4)
+    "hq4": dict(capacity_per_lp=40, stage_per_lp=1, heaps=4),
 }
-KIND = {"laq2": "laq"}
+KIND = {"laq2": "laq", "hq4": "hq"}
 
 
 def _over(kind, L):
     # the parallel heap's node size r: one node holds one safe event per LP (src/heap_queue.cu:1308)
+    if kind == "hq4":      # at most one heap per LP; a heap takes a whole step's inserts (<= L <= 7 r)
+        return {"node_size": max((L + 3) // 4, 32), "heaps": min(4, L)}
     return {"node_size": max(L, 32)} if kind == "hq" else {}
 
 
@@ -170,6 +174,8 @@ def test_phold_2p20_lps_matches_golden(gpq, golden, kind, drain):
     gd = golden.get("phold_2p20_k32_t50")
     if gd is None:
         pytest.skip("2^20-LP golden record not generated")
+    if kind == "hq4":
+        pytest.skip("the four-heap HQ is covered up to 65536 LPs (its steps launch four heaps' kernels)")
     g = run_gpu(gpq, kind, 1 << 20, 32, 50.0, checks=False, batch=64, drain=drain)
     assert g["flags"] == 0
     assert g["rep"]["events"] == gd["events"]
