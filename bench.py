@@ -55,11 +55,14 @@ FAR_STEP = 0.0     # LAQ two-tier: near-horizon step (0 = 4 x lookahead)
 
 
 def make(gpq, kind, wl, stats=False, checks=False):
+    heaps = 4 if kind == "hq4" else 1      # hq4: four heaps, each owning a quarter of the LPs
+    kind = "hq" if kind == "hq4" else kind
     args = dict(QUEUES[kind])
     if DRAIN:
         args["stage_per_lp"] = 32      # a drained window can send ~3 events to one LP on average
     if kind == "hq":
-        args["node_size"] = max(wl["num_lps"], 32)   # one node holds one safe event per LP
+        # one node holds one safe event per LP of its heap
+        args.update(node_size=max(-(-wl["num_lps"] // heaps), 32), heaps=heaps)
     if kind == "laq" and FAR:
         args.update(capacity_per_lp=64, far_per_lp=FAR, far_step=FAR_STEP)
     q = gpq.Queue(kind, num_lps=wl["num_lps"], lookahead=0.1, horizon=wl["end_time"], block_threads=256,
@@ -188,7 +191,8 @@ def per_queue_bench(gpq, wl, log):
     global DRAIN
     out = {}
     saved = DRAIN
-    runs = [("laq", False), ("lhq", True), ("lhq", False), ("lsq", True), ("gaq", False), ("hq", False)]
+    runs = [("laq", False), ("lhq", True), ("lhq", False), ("lsq", True), ("gaq", False), ("hq", False),
+            ("hq4", False)]
     if saved == "fused":
         runs.insert(0, ("laq", True))             # the two-tier LAQ with a separate post kernel
     for kind, drain in runs:

@@ -331,19 +331,30 @@ gpq_status HqQueue::init() {
   GPQ_TRY(arena.alloc(&counts_, (size_t)heaps_ * nodes_));
   GPQ_TRY(arena.alloc(&dirty_, (size_t)heaps_ * nodes_));
   GPQ_TRY(arena.alloc(&v_.enq, (size_t)heaps_ * enq_cap));
-  GPQ_TRY(arena.alloc(&sort_[0], enq_cap));
-  GPQ_TRY(arena.alloc(&sort_[1], enq_cap));
+  // working buffers, one set per heap so that the heaps' steps can run concurrently
+  scratch_n_ = 2 * (size_t)ceil_div(mcap_, 1024) + 2;
+  tasks_n_ = 3 * (size_t)slots_ + 8;
+  GPQ_TRY(arena.alloc(&sort_[0], (size_t)heaps_ * enq_cap));
+  GPQ_TRY(arena.alloc(&sort_[1], (size_t)heaps_ * enq_cap));
   GPQ_TRY(arena.alloc(&merged_, (size_t)heaps_ * mcap_));
-  GPQ_TRY(arena.alloc(&rest_, mcap_));
-  GPQ_TRY(arena.alloc(&carry_, mcap_));
-  GPQ_TRY(arena.alloc(&chunk_buf_, (size_t)kHqMaxChunks * r_));
-  GPQ_TRY(arena.alloc(&tmp2_, (size_t)slots_ * 2u * r_));
-  GPQ_TRY(arena.alloc(&tmp1_, (size_t)slots_ * r_ + r_));
+  GPQ_TRY(arena.alloc(&rest_, (size_t)heaps_ * mcap_));
+  GPQ_TRY(arena.alloc(&carry_, (size_t)heaps_ * mcap_));
+  GPQ_TRY(arena.alloc(&chunk_buf_, (size_t)heaps_ * kHqMaxChunks * r_));
+  GPQ_TRY(arena.alloc(&tmp2_, (size_t)heaps_ * slots_ * 2u * r_));
+  GPQ_TRY(arena.alloc(&tmp1_, (size_t)heaps_ * (slots_ * (size_t)r_ + r_)));
   GPQ_TRY(arena.alloc(&v_.deq, (size_t)heaps_ * lps_per_heap_));
   GPQ_TRY(arena.alloc(&v_.st, heaps_));
-  GPQ_TRY(arena.alloc(&first_, cfg.num_lps));
-  GPQ_TRY(arena.alloc(&scratch_, 2 * (size_t)ceil_div(mcap_, 1024) + 2));
-  GPQ_TRY(arena.alloc(&tasks_, 3 * (size_t)slots_ + 8));
+  GPQ_TRY(arena.alloc(&first_, cfg.num_lps));   // by LP: the heaps' LP ranges are disjoint
+  GPQ_TRY(arena.alloc(&scratch_, (size_t)heaps_ * scratch_n_));
+  GPQ_TRY(arena.alloc(&tasks_, (size_t)heaps_ * tasks_n_));
+  if (heaps_ > 1) {
+    // one stream per heap (default priority), forked from and joined back into the queue's
+    // stream twice per step; a hipGraph capture of the step records the branches
+    hs_.assign(heaps_, nullptr);
+    ev_.assign(2 * heaps_ + 2, nullptr);
+    for (auto& x : hs_) GPQ_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    for (auto& e : ev_) GPQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   GPQ_TRY(arena.alloc(&sum_, 1));
   v_.ws = ws;
   partial_slots = 1;
@@ -358,62 +369,95 @@ gpq_status HqQueue::init() {
   return GPQ_OK;
 }
 
+HqQueue::~HqQueue() {
+  if (stream) (void)hipStreamSynchronize(stream);   // the heap streams join it every step
+  for (auto e : ev_) if (e) (void)hipEventDestroy(e);
+  for (auto x : hs_) if (x) (void)hipStreamDestroy(x);
+}
+
 // One post over every heap: the merges of inserts into the roots, the global window, then per
-// heap the selection and the heap maintenance.  Heaps run one after another on the stream and
-// share the sort, selection and task buffers.
+// heap the selection and the heap maintenance.  With several heaps each heap's part runs on
+// its own stream (the reference runs one host pthread per heap, src/heap_queue.cu:1721-1757).
 gpq_status HqQueue::step(bool select) {
-  MergeTask* t = tasks_;
-  for (uint32_t h = 0; h < heaps_; ++h) {
-    HqState* st = v_.st + h;
-    Event* sorted = nullptr;
-    GPQ_TRY(sort_records(v_.enq + (size_t)h * v_.enq_cap, sort_[0], sort_[1], &st->m, v_.enq_cap, stream, &sorted));
-    hipLaunchKernelGGL(hq_plan_merge_root, dim3(1), dim3(1), 0, stream, st, counts_ + (size_t)h * nodes_,
-                       nodes_buf_ + (size_t)h * nodes_ * r_, sorted, merged_ + (size_t)h * mcap_, v_.enq_cap, t);
-    GPQ_TRY(merge_tasks(t, 1, mcap_, stream));
+  if (heaps_ == 1) {
+    GPQ_TRY(merge_root(0, stream));
+    hipLaunchKernelGGL(hq_window_kernel, dim3(1), dim3(1), 0, stream, v_.st, 1u, merged_, (size_t)mcap_, ws,
+                       cfg.lookahead, cfg.horizon, select ? 1u : 0u);
+    return step_heap(0, select, stream);
   }
-  hipLaunchKernelGGL(hq_window_kernel, dim3(1), dim3(1), 0, stream, v_.st, heaps_, merged_, (size_t)mcap_, ws,
-                     cfg.lookahead, cfg.horizon, select ? 1u : 0u);
-  for (uint32_t h = 0; h < heaps_; ++h) GPQ_TRY(step_heap(h, select));
+  for (int phase = 0; phase < 2; ++phase) {
+    hipEvent_t fork = ev_[2 * heaps_ + phase];
+    GPQ_HIP(hipEventRecord(fork, stream));
+    for (uint32_t h = 0; h < heaps_; ++h) {
+      GPQ_HIP(hipStreamWaitEvent(hs_[h], fork, 0));
+      GPQ_TRY(phase ? step_heap(h, select, hs_[h]) : merge_root(h, hs_[h]));
+      hipEvent_t join = ev_[phase * heaps_ + h];
+      GPQ_HIP(hipEventRecord(join, hs_[h]));
+      GPQ_HIP(hipStreamWaitEvent(stream, join, 0));
+    }
+    if (!phase)
+      hipLaunchKernelGGL(hq_window_kernel, dim3(1), dim3(1), 0, stream, v_.st, heaps_, merged_, (size_t)mcap_, ws,
+                         cfg.lookahead, cfg.horizon, select ? 1u : 0u);
+  }
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }
 
-gpq_status HqQueue::step_heap(uint32_t h, bool select) {
+// Step 1 for heap h: M = merge(root, sorted inserts).
+gpq_status HqQueue::merge_root(uint32_t h, hipStream_t s) {
+  HqState* st = v_.st + h;
+  MergeTask* t = tasks_ + (size_t)h * tasks_n_;
+  Event* sorted = nullptr;
+  GPQ_TRY(sort_records(v_.enq + (size_t)h * v_.enq_cap, sort_[0] + (size_t)h * v_.enq_cap,
+                       sort_[1] + (size_t)h * v_.enq_cap, &st->m, v_.enq_cap, s, &sorted));
+  hipLaunchKernelGGL(hq_plan_merge_root, dim3(1), dim3(1), 0, s, st, counts_ + (size_t)h * nodes_,
+                     nodes_buf_ + (size_t)h * nodes_ * r_, sorted, merged_ + (size_t)h * mcap_, v_.enq_cap, t);
+  GPQ_TRY(merge_tasks(t, 1, mcap_, s));
+  GPQ_HIP(hipGetLastError());
+  return GPQ_OK;
+}
+
+// Steps 2 (selection) to 6 for heap h, after the window is published.
+gpq_status HqQueue::step_heap(uint32_t h, bool select, hipStream_t s) {
   HqState* st = v_.st + h;
   uint32_t* counts = counts_ + (size_t)h * nodes_;
   uint8_t* dirty = dirty_ + (size_t)h * nodes_;
   Event* nodes = nodes_buf_ + (size_t)h * nodes_ * r_;
   Event* merged = merged_ + (size_t)h * mcap_;
-  MergeTask* t = tasks_;
+  Event* rest = rest_ + (size_t)h * mcap_;
+  Event* carry = carry_ + (size_t)h * mcap_;
+  Event* chunk_buf = chunk_buf_ + (size_t)h * kHqMaxChunks * r_;
+  Event* tmp2 = tmp2_ + (size_t)h * slots_ * 2u * r_;
+  Event* tmp1 = tmp1_ + (size_t)h * (slots_ * (size_t)r_ + r_);
+  MergeTask* t = tasks_ + (size_t)h * tasks_n_;
   if (select) {
-    const SelectBufs b{first_, scratch_, &st->prefix, &st->deq_n, &st->rest_n};
+    const SelectBufs b{first_, scratch_ + (size_t)h * scratch_n_, &st->prefix, &st->deq_n, &st->rest_n};
     GPQ_TRY(select_safe(&st->mptr, &st->mcount, (uint32_t)mcap_, &ws->wend, &st->bound, &st->bound_valid, b,
-                        v_.deq + (size_t)h * lps_per_heap_, rest_, stream));
+                        v_.deq + (size_t)h * lps_per_heap_, rest, s));
   }
-  hipLaunchKernelGGL(hq_plan_root, dim3(1), dim3(1), 0, stream, st, counts, nodes, merged, rest_, carry_, r_, t);
-  GPQ_TRY(merge_tasks(t, 1, mcap_, stream));
-  hipLaunchKernelGGL(hq_plan_refill, dim3(1), dim3(1), 0, stream, st, counts, nodes, dirty, r_,
-                     tmp1_ + (size_t)slots_ * r_, t);
-  GPQ_TRY(merge_tasks(t, 1, r_, stream));
-  GPQ_TRY(merge_tasks(t + 1, 1, r_, stream));
-  MergeTask* t1 = tasks_;
-  MergeTask* t2 = tasks_ + slots_;
-  MergeTask* t3 = tasks_ + 2 * (size_t)slots_;
+  hipLaunchKernelGGL(hq_plan_root, dim3(1), dim3(1), 0, s, st, counts, nodes, merged, rest, carry, r_, t);
+  GPQ_TRY(merge_tasks(t, 1, mcap_, s));
+  hipLaunchKernelGGL(hq_plan_refill, dim3(1), dim3(1), 0, s, st, counts, nodes, dirty, r_, tmp1 + (size_t)slots_ * r_, t);
+  GPQ_TRY(merge_tasks(t, 1, r_, s));
+  GPQ_TRY(merge_tasks(t + 1, 1, r_, s));
+  MergeTask* t1 = t;
+  MergeTask* t2 = t + slots_;
+  MergeTask* t3 = t + 2 * (size_t)slots_;
   for (uint32_t lv = 0; lv < levels_; ++lv) {
     const uint32_t ns = 1u << lv;
-    hipLaunchKernelGGL(hq_plan_level, dim3(ceil_div(ns, 256)), dim3(256), 0, stream, st, counts, nodes, dirty, r_,
-                       nodes_, lv, tmp2_, tmp1_, t1, t2, t3);
-    GPQ_TRY(merge_tasks(t1, ns, 2ull * r_, stream));
-    GPQ_TRY(merge_tasks(t2, ns, 3ull * r_, stream));
-    GPQ_TRY(merge_tasks(t3, ns, r_, stream));
+    hipLaunchKernelGGL(hq_plan_level, dim3(ceil_div(ns, 256)), dim3(256), 0, s, st, counts, nodes, dirty, r_, nodes_,
+                       lv, tmp2, tmp1, t1, t2, t3);
+    GPQ_TRY(merge_tasks(t1, ns, 2ull * r_, s));
+    GPQ_TRY(merge_tasks(t2, ns, 3ull * r_, s));
+    GPQ_TRY(merge_tasks(t3, ns, r_, s));
   }
-  hipLaunchKernelGGL(hq_plan_chunks, dim3(1), dim3(1), 0, stream, st, counts, carry_, chunk_buf_, r_, nodes_, ws);
+  hipLaunchKernelGGL(hq_plan_chunks, dim3(1), dim3(1), 0, s, st, counts, carry, chunk_buf, r_, nodes_, ws);
   const uint32_t chunks = std::min<uint32_t>(kHqMaxChunks, ceil_div(v_.enq_cap, r_) + 1);
   for (uint32_t c = 0; c < chunks; ++c) {
     for (uint32_t lv = 1; lv <= depth_; ++lv) {
-      hipLaunchKernelGGL(hq_plan_insert_level, dim3(1), dim3(1), 0, stream, st, counts, nodes, r_, c, lv, tmp1_, t);
-      GPQ_TRY(merge_tasks(t, 1, 2ull * r_, stream));
-      GPQ_TRY(merge_tasks(t + 1, 1, 2ull * r_, stream));
+      hipLaunchKernelGGL(hq_plan_insert_level, dim3(1), dim3(1), 0, s, st, counts, nodes, r_, c, lv, tmp1, t);
+      GPQ_TRY(merge_tasks(t, 1, 2ull * r_, s));
+      GPQ_TRY(merge_tasks(t + 1, 1, 2ull * r_, s));
     }
   }
   GPQ_HIP(hipGetLastError());

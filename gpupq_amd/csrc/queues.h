@@ -103,6 +103,7 @@ class GaqQueue final : public QueueBase {
 class HqQueue final : public QueueBase {
  public:
   explicit HqQueue(const gpq_queue_config& c) : QueueBase(c) {}
+  ~HqQueue() override;
   gpq_status init() override;
   gpq_status post() override;
   gpq_status post_init() override;
@@ -115,12 +116,16 @@ class HqQueue final : public QueueBase {
 
  private:
   gpq_status step(bool select);
-  gpq_status step_heap(uint32_t h, bool select);
+  gpq_status merge_root(uint32_t h, hipStream_t s);
+  gpq_status step_heap(uint32_t h, bool select, hipStream_t s);
   gpq_status snapshot(std::vector<Event>* out, std::vector<uint32_t>* counts);
   HqView v_{};
   uint32_t r_ = 0, nodes_ = 0, levels_ = 0, slots_ = 1, depth_ = 0;
   uint32_t heaps_ = 1, lps_per_heap_ = 0;   // nodes_ is per heap
   uint64_t mcap_ = 0;                       // merged-buffer records per heap
+  size_t scratch_n_ = 0, tasks_n_ = 0;      // per-heap sizes of scratch_ and tasks_
+  std::vector<hipStream_t> hs_;             // one stream per heap (several heaps only)
+  std::vector<hipEvent_t> ev_;              // joins of both phases, then the two forks
   Event* nodes_buf_ = nullptr;
   uint32_t* counts_ = nullptr;
   uint8_t* dirty_ = nullptr;
