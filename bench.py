@@ -186,8 +186,9 @@ def astar_bench(gpq, log, plain_n=16, cpu=True):
     return out
 
 
-def per_queue_bench(gpq, wl, log):
-    """The same workload through every queue kind behind the one API (one timed run each)."""
+def per_queue_bench(gpq, wl, log, drive="graph"):
+    """The same workload through every queue kind behind the one API (one timed run each);
+    drive "plain" launches every step eagerly (a host sync per step), for kernel tracing."""
     global DRAIN
     out = {}
     saved = DRAIN
@@ -200,7 +201,7 @@ def per_queue_bench(gpq, wl, log):
         q, m = make(gpq, kind, wl)
         q.sync()
         t0 = time.perf_counter()
-        rep = gpq.des_run(m, mode="graph", batch=64)
+        rep = gpq.des_run(m, mode=drive, batch=64)
         dt = time.perf_counter() - t0
         h = m.trace_hash()
         m.model_finish()
@@ -379,6 +380,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-queues", action="store_true", help="skip the same workload through the other queue kinds")
     ap.add_argument("--no-autotune", action="store_true", help="skip the PHOLD layout autotuner")
+    ap.add_argument("--per-queue-drive", choices=["graph", "plain"], default="graph",
+                    help="drive of the per-queue runs (plain: eager launches, for kernel tracing)")
     ap.add_argument("--astar-drive", default="graph", choices=("graph", "plain"),
                     help="single A* query: hipGraph replays, or one launch sequence per step (eager, for tracing)")
     ap.add_argument("--serve-rates", default="50,100,200,400", help="Poisson arrival rates (requests/s) to sweep")
@@ -478,7 +481,7 @@ def main():
     if not args.no_autotune and args.queue == "laq":
         line["phold_autotune"] = phold_autotune_bench(gpq, wl, log)
     if not args.no_queues:
-        line["per_queue"] = per_queue_bench(gpq, wl, log)
+        line["per_queue"] = per_queue_bench(gpq, wl, log, drive=args.per_queue_drive)
         hashes_q = {v["trace_hash"] for v in line["per_queue"].values()}
         line["per_queue_hash_equal"] = len(hashes_q) == 1 and (golden is None or hashes_q == {golden["trace_hash"]})
     if not args.no_batch:
