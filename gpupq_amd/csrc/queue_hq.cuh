@@ -47,14 +47,26 @@ struct HqView {
   uint32_t lps_per_heap;
 
   __device__ __forceinline__ uint32_t slots() const { return num_lps; }
-  // heap_queue_insert (src/heap_queue.cu:1809-1826): into the insert area of the LP's heap.
+  // heap_queue_insert (src/heap_queue.cu:1809-1826): into the insert area of the LP's heap,
+  // one counter atomic per wave and heap (every active lane of the wave takes part).
   __device__ __forceinline__ bool insert(const Event& e) const {
-    uint32_t h = 0, pos;
+    uint32_t h = 0, pos = 0;
     if (heaps == 1) {
       pos = wave_append_slot(&st->m);
     } else {
       h = e.dst / lps_per_heap;
-      pos = atomicAdd(&st[h].m, 1u);
+      const int lane = lane_id();
+      uint64_t pending = __ballot(1);
+      while (pending) {                                  // one round per heap present in the wave
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lh = (uint32_t)__shfl((int)h, leader, kWave);
+        const uint64_t same = __ballot(h == lh) & pending;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&st[lh].m, (uint32_t)__popcll(same));
+        base = (uint32_t)__shfl((int)base, leader, kWave);
+        if (h == lh) pos = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+      }
     }
     if (pos >= enq_cap) {
       atomicOr(&ws->error, kErrStageOverflow);
