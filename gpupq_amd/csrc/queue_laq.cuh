@@ -73,7 +73,7 @@ struct LaqView {
   __device__ __forceinline__ void set_done(int index) const {
     if (in_queue) {                       // the id leaves the queue
       const uint32_t id = ring[index].seq;
-      atomicAnd(&in_queue[id >> 5], ~(1u << (id & 31u)));
+      if (id < update_ids) atomicAnd(&in_queue[id >> 5], ~(1u << (id & 31u)));
     }
     const uint32_t lp = (uint32_t)index >> cap_log2;
     const uint32_t mask = (1u << cap_log2) - 1u;
@@ -93,7 +93,7 @@ struct LaqView {
     if (in_queue) {                       // every id of the LP leaves the queue
       for (uint32_t i = 0; i < len[lp]; ++i) {
         const uint32_t id = ring[((uint32_t)lp << cap_log2) | ((rd[lp] + i) & mask)].seq;
-        atomicAnd(&in_queue[id >> 5], ~(1u << (id & 31u)));
+        if (id < update_ids) atomicAnd(&in_queue[id >> 5], ~(1u << (id & 31u)));
       }
       upd_n[lp] = 0;
     }
