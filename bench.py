@@ -462,7 +462,7 @@ def main():
                    "laq_far_per_lp": FAR,
                    "trace_hash_vs_cpu": parity},
     }
-    if not args.no_roofline and args.queue in ("laq", "lhq", "lsq"):
+    def roofline_line():
         dom, kernels, counters = roofline(gpq, args.queue, wl, log)
         k = kernels[dom]
         kname = dom
@@ -470,27 +470,41 @@ def main():
             kname = "laq_post_far_kernel" if FAR else "laq_post_kernel"
         if dom == "merge_commit_step":
             kname = "phold_fused_kernel"
-        line["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS,
-                            "unit": "GB/s", "frac": k["gbps"] / HBM_PEAK_GBPS,
-                            "traffic": traffic_from_profiles(kname, args.workload),
-                            "alg_bytes_per_launch": k["alg_bytes"], "mean_launch_us": k["us"],
-                            "other_kernels": {n: {"achieved": v["gbps"], "mean_launch_us": v["us"]}
-                                              for n, v in kernels.items() if n != dom}}
+        return {"bound": "hbm", "kernel": dom, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": k["gbps"] / HBM_PEAK_GBPS,
+                "traffic": traffic_from_profiles(kname, args.workload),
+                "alg_bytes_per_launch": k["alg_bytes"], "mean_launch_us": k["us"],
+                "other_kernels": {n: {"achieved": v["gbps"], "mean_launch_us": v["us"]}
+                                  for n, v in kernels.items() if n != dom}}
+
+    def section(key, fn, *a, **kw):
+        # a failure of one part is reported in the line under its key, with the error text,
+        # instead of ending the run before the line is printed
+        try:
+            line[key] = fn(*a, **kw)
+        except Exception as e:   # noqa: BLE001
+            line[key] = {"error": "%s: %s" % (type(e).__name__, e)}
+            log("%s failed: %s" % (key, line[key]["error"]))
+
+    if not args.no_roofline and args.queue in ("laq", "lhq", "lsq"):
+        section("roofline", roofline_line)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(wl, log)
+        section("cpu_baseline", cpu_baseline, wl, log)
     if not args.no_autotune and args.queue == "laq":
-        line["phold_autotune"] = phold_autotune_bench(gpq, wl, log)
+        section("phold_autotune", phold_autotune_bench, gpq, wl, log)
     if not args.no_queues:
-        line["per_queue"] = per_queue_bench(gpq, wl, log, drive=args.per_queue_drive)
-        hashes_q = {v["trace_hash"] for v in line["per_queue"].values()}
-        line["per_queue_hash_equal"] = len(hashes_q) == 1 and (golden is None or hashes_q == {golden["trace_hash"]})
+        section("per_queue", per_queue_bench, gpq, wl, log, drive=args.per_queue_drive)
+        if "error" not in line["per_queue"]:
+            hashes_q = {v["trace_hash"] for v in line["per_queue"].values()}
+            line["per_queue_hash_equal"] = len(hashes_q) == 1 and (golden is None or
+                                                                    hashes_q == {golden["trace_hash"]})
     if not args.no_batch:
-        line["phold_batch"] = phold_batch_bench(gpq, log)
+        section("phold_batch", phold_batch_bench, gpq, log)
     if not args.no_astar:
-        line["astar_batch"] = astar_bench(gpq, log, cpu=not args.no_cpu)
-        line["astar_single"] = astar_single_bench(gpq, log, cpu=not args.no_cpu, drive=args.astar_drive)
+        section("astar_batch", astar_bench, gpq, log, cpu=not args.no_cpu)
+        section("astar_single", astar_single_bench, gpq, log, cpu=not args.no_cpu, drive=args.astar_drive)
     if not args.no_serving:
-        line["serving"] = serving_bench(gpq, log, rates=[float(r) for r in args.serve_rates.split(",")])
+        section("serving", serving_bench, gpq, log, rates=[float(r) for r in args.serve_rates.split(",")])
     print(json.dumps(line), flush=True)
 
 
