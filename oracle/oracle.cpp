@@ -3,7 +3,7 @@
 // checker; never linked into or called by the product library.
 //
 // Each function cites the reference file:line it follows (paths relative to the
-// reference root).  The BASELINE workloads redefine the reference's numerics
+// reference root).  The benchmark workloads redefine the reference's numerics
 // (float32 timestamps, Philox streams, total-order tie-break, k events per LP);
 // the algorithmic structure below follows the reference.
 #include "oracle.h"
