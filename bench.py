@@ -393,7 +393,7 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not provided "
+        print("bench.py: the benchmark runs on one GPU; --gpus %d is not provided "
               "(the multi-GPU sharding is a design note only)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

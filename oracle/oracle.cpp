@@ -465,7 +465,7 @@ int64_t best_first(const uint8_t* blocked, int w, int h, int conn, int sx, int s
 }  // namespace
 
 // Exact integer A* (the reference's expansion, src/a_star.cu:269-593, keeps f in fp16 and
-// pops up to 8 entries per step, so it is not guaranteed optimal; the build requires the optimum).
+// pops up to 8 entries per step, so it is not guaranteed optimal; this restatement returns the optimum).
 extern "C" int64_t orc_astar_grid(const uint8_t* blocked, int w, int h, int conn, int sx, int sy,
                                   int tx, int ty, uint64_t* expanded) {
   return best_first(blocked, w, h, conn, sx, sy, tx, ty, true, expanded);
