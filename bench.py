@@ -230,9 +230,10 @@ def phold_autotune_bench(gpq, wl, log, sample_end_time=5.0):
     return out
 
 
-def phold_batch_bench(gpq, log, instances=32, seed=5):
+def phold_batch_bench(gpq, log, instances=32, seed=5, plain=True):
     """Instances of uneven size (LPs 2^U[12,18], 16 events per LP, end time 20) advanced together
-    in one pool vs each alone through its own queue with a host sync per step."""
+    in one pool vs each alone through its own queue with a host sync per step (plain=False: the
+    pool only, for kernel tracing)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     inst = [(int(round(2 ** rng.uniform(12, 18))), 16, int(rng.integers(1, 2 ** 31)), 20.0) for _ in range(instances)]
@@ -241,11 +242,16 @@ def phold_batch_bench(gpq, log, instances=32, seed=5):
         b.run("graph", batch=16)                  # warm up (graph capture)
         tg = b.run("graph", batch=16)
         hg, eg = b.results()
-        tp = b.run("plain")
-        hp, ep = b.results()
-    if list(hg) != list(hp) or list(eg) != list(ep):
+        if plain:
+            tp = b.run("plain")
+            hp, ep = b.results()
+    if plain and (list(hg) != list(hp) or list(eg) != list(ep)):
         raise RuntimeError("batched PHOLD: pool and plain results differ")
     ev = int(eg.sum())
+    if not plain:
+        out = {"instances": instances, "total_lps": sum(i[0] for i in inst), "events": ev, "events_per_s": ev / tg}
+        log("phold_batch (pool only): %s" % json.dumps(out))
+        return out
     out = {"instances": instances, "lps": "2^U[12,18] x 16, end time 20", "total_lps": sum(i[0] for i in inst),
            "laq_far_per_lp": FAR,
            "events": ev, "events_per_s": ev / tg, "plain_events_per_s": ev / tp, "batched_vs_plain": tp / tg,
@@ -380,6 +386,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-queues", action="store_true", help="skip the same workload through the other queue kinds")
     ap.add_argument("--no-autotune", action="store_true", help="skip the PHOLD layout autotuner")
+    ap.add_argument("--batch-pool-only", action="store_true",
+                    help="batched PHOLD: time the pool only, not the plain per-instance path (kernel tracing)")
     ap.add_argument("--per-queue-drive", choices=["graph", "plain"], default="graph",
                     help="drive of the per-queue runs (plain: eager launches, for kernel tracing)")
     ap.add_argument("--astar-drive", default="graph", choices=("graph", "plain"),
@@ -419,7 +427,7 @@ def main():
     if args.skip_phold:
         line = {"metric": METRIC, "value": None, "unit": "events/s", "n_gpus": 1, "note": "PHOLD part skipped"}
         if not args.no_batch:
-            line["phold_batch"] = phold_batch_bench(gpq, log)
+            line["phold_batch"] = phold_batch_bench(gpq, log, plain=not args.batch_pool_only)
         if not args.no_astar:
             line["astar_batch"] = astar_bench(gpq, log, cpu=not args.no_cpu)
             line["astar_single"] = astar_single_bench(gpq, log, cpu=not args.no_cpu, drive=args.astar_drive)
