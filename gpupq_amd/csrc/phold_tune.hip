@@ -69,6 +69,8 @@ extern "C" gpq_status gpq_phold_autotune(gpq_phold_config* model, gpq_queue_conf
       qc.far_step = c.far_step_la * queue->lookahead;
       qc.horizon = t;
       qc.collect_stats = 0;
+      qc.update_ids = 0;           // PHOLD records carry no ids: no decrease-key mode
+      qc.update_per_lp = 0;
       gpq_phold_config pc = *model;
       pc.end_time = t;
       pc.drain = c.drain;
@@ -105,6 +107,8 @@ extern "C" gpq_status gpq_phold_autotune(gpq_phold_config* model, gpq_queue_conf
   queue->stage_per_lp = pick->stage;
   queue->far_per_lp = pick->far;
   queue->far_step = pick->far_step_la * queue->lookahead;
+  queue->update_ids = 0;
+  queue->update_per_lp = 0;
   model->drain = pick->drain;
   return GPQ_OK;
 }

@@ -187,8 +187,8 @@ typedef struct {
  * staging slots, far store, near-horizon step and step kind -- on the model's own workload cut
  * to end time sample_end_time (0 = 5, at most model->end_time), each in a fresh queue driven
  * by GPQ_DRIVE_GRAPH, and writes the fastest (a later candidate must beat the pick so far by
- * more than 2%) into *queue (kind = GPQ_LAQ, capacity_per_lp,
- * stage_per_lp, far_per_lp, far_step; the other fields are kept) and model->drain.  results
+ * more than 2%) into *queue (kind = GPQ_LAQ, capacity_per_lp, stage_per_lp, far_per_lp, far_step;
+ * update_ids and update_per_lp cleared; the other fields are kept) and model->drain.  results
  * (may be NULL) receives up to max_results candidates in the order timed, *n_results (may be
  * NULL) their count.  GPQ_ERR_CAPACITY when no layout holds the workload. */
 gpq_status gpq_phold_autotune(gpq_phold_config* model, gpq_queue_config* queue, float sample_end_time,
