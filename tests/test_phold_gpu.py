@@ -216,3 +216,22 @@ def test_phold_autotune_rejects_mismatched_sizes(gpq):
         import ctypes
         gpq._lib.check(gpq.lib.gpq_phold_autotune(ctypes.byref(p), ctypes.byref(q), 0.0, None, 0, None),
                        "phold_autotune")
+
+
+def test_phold_autotune_skips_layouts_that_do_not_fit(gpq, oracle):
+    # 100 initial events per LP fit only the 128-record one-tier ring: the other layouts are
+    # reported as not fitting and the pick is that ring
+    L, k, T = 512, 100, 1.0
+    kw, drain, cands = gpq.phold_autotune(L, init_per_lp=k, seed=42, end_time=T, sample_end_time=0.5)
+    assert [c["status"] for c in cands].count("ran") == 1
+    assert all(c["status"] == "does not fit" for c in cands if c["capacity_per_lp"] < k)
+    assert (kw["capacity_per_lp"], kw["far_per_lp"], drain) == (128, 0, True)
+    res, cnt, hsh = oracle.phold("seq", L, k, T)
+    q = gpq.Queue("laq", num_lps=L, lookahead=0.1, horizon=T, **kw)
+    m = gpq.Phold(q, init_per_lp=k, seed=42, end_time=T, drain=drain)
+    m.model_init()
+    gpq.des_run(m, mode="graph", batch=16)
+    c, h = m.lp_state()
+    assert np.array_equal(c, cnt) and np.array_equal(h, hsh)
+    m.model_finish()
+    q.queue_finish()
