@@ -219,9 +219,9 @@ def test_phold_autotune_rejects_mismatched_sizes(gpq):
 
 
 def test_phold_autotune_skips_layouts_that_do_not_fit(gpq, oracle):
-    # 100 initial events per LP fit only the 128-record one-tier ring: the other layouts are
+    # 80 initial events per LP fit only the 128-record one-tier ring: the other layouts are
     # reported as not fitting and the pick is that ring
-    L, k, T = 512, 100, 1.0
+    L, k, T = 512, 80, 1.0
     kw, drain, cands = gpq.phold_autotune(L, init_per_lp=k, seed=42, end_time=T, sample_end_time=0.5)
     assert [c["status"] for c in cands].count("ran") == 1
     assert all(c["status"] == "does not fit" for c in cands if c["capacity_per_lp"] < k)
