@@ -35,6 +35,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+ASTAR_SEED = 2024
+ASTAR_QUERIES = 512
 RECORD = 16          # bytes per event record
 METRIC = "PHOLD events/s @2^20 LPs; batched A* queries/s + p99 latency vs plain"
 
@@ -75,7 +77,14 @@ def make(gpq, kind, wl, stats=False, checks=False):
 def roofline(gpq, kind, wl, log):
     """Mean launch durations of the model step kernel and the queue post kernel from a
     HIP-event-instrumented run, and their algorithmic bytes from the device work
-    counters of a second (identical, deterministic) run."""
+    counters of a second (identical, deterministic) run.
+
+    The end-to-end PHOLD bytes are structure-independent: per committed event the record's
+    lifecycle (written to staging, read from staging, written into the queue, read at
+    dequeue: 4 x 16 B) and the LP's state (12 B read + 12 B write), 88 B in all, plus 4 B per
+    LP and step for the window scan.  Bytes a kernel moves beyond that because of the queue's
+    structure (records displaced by a sorted insert, far-store moves, LP words beyond the
+    head) are reported as overhead, not as achieved bandwidth."""
     q, m = make(gpq, kind, wl)
     prof = m.profile(batch=64)
     m.model_finish(); q.queue_finish()
@@ -85,29 +94,38 @@ def roofline(gpq, kind, wl, log):
     after = q.stats()
     m.model_finish(); q.queue_finish()
     d = {k: after[k] - before[k] for k in after}
-    launches = max(d["post_launches"], 1)
     L = wl["num_lps"]
-    # queue post kernel: sorted insert of k records displacing d (16*(2k+2d)), one head word
-    # per LP for the window scan, one partial minimum per 256-LP block
-    scan = 12 if FAR else 4          # head (+ far minimum and far count in two-tier mode) per LP
-    post_bytes = ((RECORD * (2 * d["inserted"] + 2 * d["displaced"]) + scan * d["scanned_lps"] + d["aux_bytes"])
-                  / launches + scan * ((L + 255) // 256))
-    # model step: one head word per LP, and per committed event the head record (16), the
-    # per-LP state read+write (count 8, hash 16, ring indices 16, head 4 + next head 4),
-    # the staging counter (4) and the staged record (16)
-    events_per_step = rep["events"] / max(d["post_launches"], 1)
-    step_bytes = 4 * L + 88 * events_per_step
+    events = rep["events"]
+    if prof["events"] != events:
+        raise RuntimeError("roofline: profiled run committed %d events, counted run %d" % (prof["events"], events))
+    steps = prof["steps"]                     # launches of the model step kernel in the profiled run
+    launches = max(d["post_launches"], 1)     # post (merge) launches of the counted run
+    scan = 12 if FAR else 4                   # LP words the merge scans: head (+ far minimum and count)
+    # structure-dependent bytes per launch, from the kernels' own counters
+    overhead = {"displaced_records": RECORD * 2 * d["displaced"] / launches,
+                "far_store_moves": d["aux_bytes"] / launches,
+                "lp_words_beyond_head": (scan - 4) * d["scanned_lps"] / launches}
+    path_bytes = 88.0 * events + 4.0 * L * steps       # over all `steps` launches
     if DRAIN == "fused":
-        # one kernel does the queue post's merge and the model step: the work of both
+        # one kernel merges the staged events and commits the window: the whole path's bytes
+        # over the same launch set whose durations were measured
         kernels = {
-            "merge_commit_step": {"us": prof["step_kernel_us"], "alg_bytes": post_bytes + step_bytes},
-            "window_publish": {"us": prof["window_kernel_us"], "alg_bytes": 80},
+            "merge_commit_step": {"us": prof["step_kernel_us"], "alg_bytes": path_bytes / steps,
+                                  "launches": steps, "overhead_bytes": overhead},
+            "window_publish": {"us": prof["window_kernel_us"], "alg_bytes": 24.0, "launches": steps},
         }
     else:
+        # queue post: the sorted insert of k records displacing d (16*(2k+2d)), 4 B per scanned
+        # head and per block partial; model step: 88 B per committed event + 4 B per LP
+        post_bytes = (RECORD * (2 * d["inserted"] + 2 * d["displaced"]) + 4 * d["scanned_lps"]) / launches \
+            + 4 * ((L + 255) // 256)
         kernels = {
-            "queue_post": {"us": prof["post_kernel_us"], "alg_bytes": post_bytes},
-            "model_step": {"us": prof["step_kernel_us"], "alg_bytes": step_bytes},
-            "window_publish": {"us": prof["window_kernel_us"], "alg_bytes": 4 * ((L + 255) // 256) + 24},
+            "queue_post": {"us": prof["post_kernel_us"], "alg_bytes": post_bytes, "launches": launches,
+                           "overhead_bytes": {"far_store_moves": overhead["far_store_moves"],
+                                              "lp_words_beyond_head": overhead["lp_words_beyond_head"]}},
+            "model_step": {"us": prof["step_kernel_us"], "alg_bytes": path_bytes / steps, "launches": steps},
+            "window_publish": {"us": prof["window_kernel_us"], "alg_bytes": 4 * ((L + 255) // 256) + 24,
+                               "launches": steps},
         }
     for k in kernels.values():
         k["gbps"] = k["alg_bytes"] / (k["us"] * 1e-6) / 1e9 if k["us"] > 0 else 0.0
@@ -116,19 +134,46 @@ def roofline(gpq, kind, wl, log):
     return dom, kernels, d
 
 
+def host_cpu():
+    """Model name and logical core count of the host CPU (the lscpu "Model name" field)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model, os.cpu_count(), avail
+
+
 def cpu_baseline(wl, log):
+    """The oracle on the host: the sequential std::priority_queue simulation on one core
+    (the value), and the windowed drain on every core this process may use (at most 16,
+    the share of one GPU's host cores), each on a bounded prefix of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
+    o = Oracle()
     sample_T = 0.5
-    res, _, _ = Oracle().phold("seq", wl["num_lps"], wl["init_per_lp"], sample_T)
+    res, _, _ = o.phold("seq", wl["num_lps"], wl["init_per_lp"], sample_T)
     log("cpu_baseline: %d events in %.2fs" % (res.events, res.seconds))
-    return {"value": res.events / res.seconds, "unit": "events/s", "cores": 1, "kind": "port",
-            "sample": "sequential std::priority_queue oracle, %d LPs x %d events, end time %.2f "
-                      "(%d events), event loop only" % (wl["num_lps"], wl["init_per_lp"], sample_T, res.events)}
+    model, ncpu, avail = host_cpu()
+    out = {"value": res.events / res.seconds, "unit": "events/s", "cores": 1, "kind": "port",
+           "sample": "sequential std::priority_queue oracle, %d LPs x %d events, end time %.2f "
+                     "(%d events), event loop only" % (wl["num_lps"], wl["init_per_lp"], sample_T, res.events),
+           "cpu_model": model, "host_logical_cpus": ncpu}
+    threads = max(1, min(16, avail))
+    par_T = 2.0
+    pres, _, _ = o.phold("parallel", wl["num_lps"], wl["init_per_lp"], par_T, threads=threads)
+    log("cpu_baseline all cores: %d events in %.2fs on %d threads" % (pres.events, pres.seconds, threads))
+    out["all_cores"] = {"value": pres.events / pres.seconds, "unit": "events/s", "cores": threads,
+                        "sample": "windowed drain over %d host threads, end time %.1f (%d events), event loop only"
+                                  % (threads, par_T, pres.events)}
+    return out
 
 
-ASTAR_SEED = 2024
-ASTAR_QUERIES = 512
 
 
 def pct(v, p):
@@ -137,7 +182,7 @@ def pct(v, p):
     return v[k]
 
 
-def astar_bench(gpq, log, plain_n=16, cpu=True):
+def astar_bench(gpq, log, plain_n=ASTAR_QUERIES, cpu=True):
     specs = [gpq.query_synthetic(ASTAR_SEED, q) for q in range(ASTAR_QUERIES)]
     t0 = time.perf_counter()
     tuning, tuned_s = gpq.autotune_batch(specs, sample=64)      # threads per query x staging, on 64 queries
@@ -149,6 +194,8 @@ def astar_bench(gpq, log, plain_n=16, cpu=True):
         t = b.run("graph")
         start, finish = b.timing()
         costs, expanded = b.results()
+    # the plain one-query-per-launch path on the same queries (every query, one at a time, a
+    # host sync per expansion step), and the batched launch on exactly those queries
     with gpq.AstarBatch(specs[:plain_n], **tuning) as b2:
         tp = b2.run("plain")
         cp, _ = b2.results()
@@ -156,13 +203,16 @@ def astar_bench(gpq, log, plain_n=16, cpu=True):
         tb = b2.run("graph")
     if list(cp) != list(costs[:plain_n]):
         raise RuntimeError("A*: plain and batched costs differ")
+    log("astar plain: %d queries in %.2fs; batched on the same queries %.4fs" % (plain_n, tp, tb))
     out = {"queries": ASTAR_QUERIES, "seed": ASTAR_SEED, "threads_per_query": threads,
            "tuning": dict(tuning, autotune_seconds=tune_wall),
            "queries_per_s": ASTAR_QUERIES / t, "seconds": t,
            "expanded_per_s": float(expanded.sum()) / t,
            "p50_completion_ms": pct(finish, 50) / 1000.0, "p99_completion_ms": pct(finish, 99) / 1000.0,
-           "plain_subset": plain_n, "plain_queries_per_s": plain_n / tp, "batched_subset_queries_per_s": plain_n / tb,
-           "batched_vs_plain": (ASTAR_QUERIES / t) / (plain_n / tp), "unreachable": int((costs < 0).sum())}
+           "plain_queries": plain_n, "plain_queries_per_s": plain_n / tp,
+           "batched_same_queries_per_s": plain_n / tb,
+           "batched_vs_plain": tp / tb, "batched_vs_plain_basis": "the same %d queries, both timed" % plain_n,
+           "unreachable": int((costs < 0).sum())}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_lib import Oracle
@@ -318,7 +368,13 @@ def serving_bench(gpq, log, requests=64, rates=(50.0, 100.0, 200.0, 400.0)):
     kw = dict(seed=7, phold_fraction=0.5, hbm_budget=8 << 30)
     rep_p, plain = gpq.serve_run(requests, mode="plain", rate=0.0, **kw)
     res = [r["result"] for r in plain]
-    keys = ("requests_per_second", "p50_ms", "p99_ms", "phold_p99_ms", "astar_p99_ms", "peak_inflight", "peak_bytes")
+    # the plain path with every PHOLD request's queue and model created before the clock, as the
+    # served pool is: the like-for-like comparison (no allocation inside either clock)
+    rep_pa, plain_a = gpq.serve_run(requests, mode="plain", rate=0.0, preallocated=True, **kw)
+    if [r["result"] for r in plain_a] != res:
+        raise RuntimeError("serving: plain preallocated and plain results differ")
+    keys = ("requests_per_second", "p50_ms", "p99_ms", "phold_p99_ms", "astar_p99_ms", "peak_inflight", "peak_bytes",
+            "phold_peak_bytes")
     swept = []
     for rate in rates:
         rep_s, served = gpq.serve_run(requests, mode="graph", rate=rate, **kw)
@@ -332,7 +388,11 @@ def serving_bench(gpq, log, requests=64, rates=(50.0, 100.0, 200.0, 400.0)):
            "served_at_rate": swept,
            "served_saturated": {k: rep_x[k] for k in keys},
            "plain": {k: rep_p[k] for k in ("requests_per_second", "p50_ms", "p99_ms")},
+           "plain_preallocated": {k: rep_pa[k] for k in ("requests_per_second", "p50_ms", "p99_ms")},
            "served_vs_plain_throughput": rep_x["requests_per_second"] / rep_p["requests_per_second"],
+           "served_vs_plain_preallocated_throughput":
+               rep_x["requests_per_second"] / rep_pa["requests_per_second"],
+           "pool_bytes": rep_x["pool_bytes"], "hbm_budget": kw["hbm_budget"],
            "results_served_vs_plain": "match"}
     log("serving: %s" % json.dumps(out))
     return out
@@ -478,12 +538,17 @@ def main():
             kname = "laq_post_far_kernel" if FAR else "laq_post_kernel"
         if dom == "merge_commit_step":
             kname = "phold_fused_kernel"
-        return {"bound": "hbm", "kernel": dom, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": k["gbps"] / HBM_PEAK_GBPS,
-                "traffic": traffic_from_profiles(kname, args.workload),
-                "alg_bytes_per_launch": k["alg_bytes"], "mean_launch_us": k["us"],
-                "other_kernels": {n: {"achieved": v["gbps"], "mean_launch_us": v["us"]}
-                                  for n, v in kernels.items() if n != dom}}
+        out = {"bound": "hbm", "kernel": dom, "kernel_symbol": kname, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS,
+               "unit": "GB/s", "frac": k["gbps"] / HBM_PEAK_GBPS,
+               "traffic": traffic_from_profiles(kname, args.workload),
+               "alg_bytes_per_launch": k["alg_bytes"], "mean_launch_us": k["us"], "launches": k["launches"],
+               "alg_bytes_model": ("88 B per committed event + 4 B per LP and step" if dom != "queue_post" else
+                                   "sorted insert 16*(2k+2d) + 4 B per scanned LP head and per block partial"),
+               "other_kernels": {n: {"achieved": v["gbps"], "mean_launch_us": v["us"]}
+                                 for n, v in kernels.items() if n != dom}}
+        if "overhead_bytes" in k:
+            out["overhead_bytes_per_launch"] = k["overhead_bytes"]
+        return out
 
     def section(key, fn, *a, **kw):
         # a failure of one part is reported in the line under its key, with the error text,

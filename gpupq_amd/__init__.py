@@ -448,9 +448,9 @@ class PholdBatch:
 # serving
 # ---------------------------------------------------------------------------------------
 def _serve_config(requests, seed=1, rate=0.0, phold_fraction=0.5, hbm_budget=0, phold_log2=(12, 18),
-                  phold_init_per_lp=16, phold_end_time=20.0, phold_batch=16, astar_lanes=4):
+                  phold_init_per_lp=16, phold_end_time=20.0, phold_batch=16, astar_lanes=4, preallocated=False):
     return _lib.ServeConfig(requests, seed, rate, phold_fraction, hbm_budget, phold_log2[0], phold_log2[1],
-                            phold_init_per_lp, phold_end_time, phold_batch, astar_lanes)
+                            phold_init_per_lp, phold_end_time, phold_batch, astar_lanes, 1 if preallocated else 0)
 
 
 def _requests_dict(arr):
@@ -466,7 +466,8 @@ def serve_trace(requests, **kw):
 
 
 def serve_run(requests, mode="graph", **kw):
-    """Serves the trace; mode "graph" = continuous batching, "plain" = one request at a time.
+    """Serves the trace; mode "graph" = continuous batching, "plain" = one request at a time
+    (preallocated=True: the plain drive's PHOLD queues and models are created before the clock).
     Returns (report dict, list of per-request dicts)."""
     cfg = _serve_config(requests, **kw)
     rep = _lib.ServeReport()

@@ -85,7 +85,7 @@ class ServeConfig(ctypes.Structure):
     _fields_ = [("requests", c_u32), ("seed", c_u32), ("rate", ctypes.c_double), ("phold_fraction", ctypes.c_double),
                 ("hbm_budget", c_u64), ("phold_min_log2", c_u32), ("phold_max_log2", c_u32),
                 ("phold_init_per_lp", c_u32), ("phold_end_time", c_f32), ("phold_batch", c_u32),
-                ("astar_lanes", c_u32)]
+                ("astar_lanes", c_u32), ("plain_preallocated", c_u32)]
 
 
 class Request(ctypes.Structure):
@@ -97,7 +97,7 @@ class ServeReport(ctypes.Structure):
     _fields_ = [("completed", c_u32), ("peak_inflight", c_u32), ("seconds", ctypes.c_double),
                 ("requests_per_second", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("phold_p99_ms", ctypes.c_double), ("astar_p99_ms", ctypes.c_double),
-                ("peak_bytes", c_u64)]
+                ("peak_bytes", c_u64), ("phold_peak_bytes", c_u64), ("pool_bytes", c_u64)]
 
 
 # (name, restype, argtypes) for every entry point of include/gpupq.h

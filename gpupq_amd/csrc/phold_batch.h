@@ -52,6 +52,8 @@ class PholdPool {
   uint32_t error_flags_host() const;
   gpq_status errors(uint32_t* flags);
 
+  // HBM the pool holds: its queue's arena and the pool's own tables and model state.
+  uint64_t device_bytes() const;
   QueueBase* queue() { return q_; }
   uint32_t capacity() const { return capacity_; }
   uint32_t free_lps() const;

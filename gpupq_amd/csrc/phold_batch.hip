@@ -240,6 +240,8 @@ uint64_t PholdPool::bytes_per_lp(uint32_t capacity_per_lp, uint32_t stage_per_lp
          (far_per_lp ? 8ull : 0ull);
 }
 
+uint64_t PholdPool::device_bytes() const { return (q_ ? q_->arena.bytes() : 0) + arena_.bytes(); }
+
 gpq_status PholdPool::create(uint32_t capacity_lps, uint32_t capacity_per_lp, uint32_t stage_per_lp,
                              uint32_t max_instances, PholdPool** out, uint32_t far_per_lp) {
   GPQ_CHECK_ARG(capacity_lps >= 1 && max_instances >= 1, "pool: empty");

@@ -96,8 +96,9 @@ struct Instance {
   }
 };
 
+// Creates a PHOLD request's queue and model; populates it (model_init) when `populate` is set.
 gpq_status start_phold(const gpq_serve_config& c, const gpq_request& r, uint32_t seed, uint32_t batch,
-                       std::unique_ptr<Instance>* out) {
+                       std::unique_ptr<Instance>* out, bool populate = true) {
   auto inst = std::make_unique<Instance>();
   const gpq_queue_config qc = phold_queue_config(r.size, c.phold_end_time);
   inst->q.reset(make_queue(qc));
@@ -107,7 +108,7 @@ gpq_status start_phold(const gpq_serve_config& c, const gpq_request& r, uint32_t
   PholdModel* m = nullptr;
   GPQ_TRY(PholdModel::create(inst->q.get(), phold_model_config(r.size, c, seed), &m));
   inst->m.reset(m);
-  GPQ_TRY(m->init());
+  if (populate) GPQ_TRY(m->init());
   if (batch) {
     GPQ_TRY(m->prepare_graph(batch));
     GPQ_HIP(hipEventCreateWithFlags(&inst->ev, hipEventDisableTiming));
@@ -203,6 +204,19 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
       astar_req.push_back(i);
       specs.push_back(s);
     }
+  // Every PHOLD request must fit the HBM budget on its own (the pool's per-LP layout, whole
+  // 256-LP blocks); a request that cannot is rejected before anything runs.
+  const uint64_t per_lp_bytes = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage, kPholdFar);
+  for (uint32_t i = 0; i < n; ++i)
+    if (req[i].kind == 0) {
+      const uint64_t need = ((uint64_t)req[i].size + 255) / 256 * 256 * per_lp_bytes + (1ull << 20);
+      if (need > c.hbm_budget) {
+        set_error("serve: PHOLD request " + std::to_string(i) + " (" + std::to_string(req[i].size) +
+                  " LPs) needs " + std::to_string(need) + " B of HBM, above the budget of " +
+                  std::to_string(c.hbm_budget) + " B");
+        return GPQ_ERR_CAPACITY;
+      }
+    }
   std::unique_ptr<AstarBatch> batch;
   const uint32_t lanes_n = mode == GPQ_DRIVE_GRAPH ? std::max<uint32_t>(1, c.astar_lanes) : 1;
   if (!specs.empty()) {
@@ -255,10 +269,15 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
     for (const auto& r : req)
       if (r.kind == 0) { max_lps = std::max(max_lps, r.size); ++phold_n; }
     if (phold_n) {
-      const uint64_t per_lp = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage, kPholdFar);
-      uint64_t cap = c.hbm_budget / per_lp;
-      cap = std::max<uint64_t>(cap, ((uint64_t)max_lps + 255) / 256 * 256);
+      // as many whole 256-LP blocks as the budget holds beside 1 MiB of fixed tables (instance
+      // table, windows, block maps, partial minima); every request fits (checked above)
+      const uint64_t fixed = 1ull << 20;
+      uint64_t cap = c.hbm_budget > fixed ? (c.hbm_budget - fixed) / per_lp_bytes / 256 * 256 : 0;
       cap = std::min<uint64_t>(cap, ((1ull << 31) / kPholdFar) - 256);
+      if (cap < ((uint64_t)max_lps + 255) / 256 * 256) {
+        set_error("serve: the PHOLD pool's LP limit is below the largest request");
+        return GPQ_ERR_CAPACITY;
+      }
       PholdPool* raw = nullptr;
       GPQ_TRY(PholdPool::create((uint32_t)cap, kPholdCapacity, kPholdStage, std::min<uint32_t>(phold_n, 1024), &raw,
                                 kPholdFar));
@@ -268,7 +287,21 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
     }
   }
 
-  uint64_t inflight_bytes = 0, peak_bytes = 0;
+  // Plain drive with preallocation: every PHOLD request's queue and model exist before the
+  // clock starts; inside it a request is populated, run and read back.
+  std::vector<std::unique_ptr<Instance>> pre(mode == GPQ_DRIVE_PLAIN && c.plain_preallocated ? n : 0);
+  uint64_t pre_bytes = 0;
+  for (uint32_t i = 0; i < (uint32_t)pre.size(); ++i)
+    if (req[i].kind == 0) {
+      GPQ_TRY(start_phold(c, req[i], seeds[i], 0, &pre[i], false));
+      pre_bytes += pre[i]->bytes;
+    }
+  if (!pre.empty()) {
+    for (auto& p : pre)
+      if (p) GPQ_HIP(hipStreamSynchronize(p->q->stream));
+  }
+
+  uint64_t inflight_bytes = 0, peak_bytes = 0, phold_bytes_now = 0, phold_peak = 0;
   uint32_t inflight = 0, peak_inflight = 0, completed = 0;
   const auto t0 = std::chrono::steady_clock::now();
 
@@ -276,13 +309,22 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
     uint32_t* h_done = nullptr;
     GPQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_done), sizeof(uint32_t), hipHostMallocDefault));
     std::unique_ptr<uint32_t, void (*)(uint32_t*)> done_guard(h_done, [](uint32_t* p) { (void)hipHostFree(p); });
+    if (!pre.empty()) phold_peak = peak_bytes = pre_bytes;     // all of them are held throughout
     for (uint32_t i = 0; i < n; ++i) {
       while (now_ms(t0) < req[i].arrival_ms) std::this_thread::sleep_for(std::chrono::microseconds(50));
       req[i].start_ms = now_ms(t0);
       if (req[i].kind == 0) {
-        std::unique_ptr<Instance> inst;
-        GPQ_TRY(start_phold(c, req[i], seeds[i], 0, &inst));
-        peak_bytes = std::max(peak_bytes, inst->bytes);
+        std::unique_ptr<Instance> own;
+        Instance* inst = nullptr;
+        if (!pre.empty()) {
+          inst = pre[i].get();
+          GPQ_TRY(inst->m->init());
+        } else {
+          GPQ_TRY(start_phold(c, req[i], seeds[i], 0, &own));
+          inst = own.get();
+          peak_bytes = std::max(peak_bytes, inst->bytes);
+          phold_peak = std::max(phold_peak, inst->bytes);
+        }
         gpq_des_report dr;
         GPQ_TRY(inst->m->run(GPQ_DRIVE_PLAIN, 1, &dr));
         GPQ_TRY(finish_phold(*inst, &req[i]));
@@ -301,7 +343,6 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
     std::vector<uint32_t> slot_req(pool ? 1024 : 0, 0xFFFFFFFFu);
     std::vector<uint32_t> fin;
     std::vector<uint64_t> fin_hash, fin_events;
-    const uint64_t per_lp_bytes = PholdPool::bytes_per_lp(kPholdCapacity, kPholdStage, kPholdFar);
     std::deque<uint32_t> waiting;          // arrived, not admitted
     std::vector<uint32_t> astar_ready;     // admitted A* queries not yet launched
     uint32_t next_arrival = 0;
@@ -327,6 +368,7 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
           req[i].work = fin_events[k];
           req[i].finish_ms = done_ms;
           inflight_bytes -= req[i].bytes;
+          phold_bytes_now -= req[i].bytes;
           --inflight;
           ++completed;
           progress = true;
@@ -361,6 +403,8 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
         req[i].start_ms = now_ms(t0);
         req[i].bytes = ((uint64_t)req[i].size + 255) / 256 * 256 * per_lp_bytes;
         inflight_bytes += req[i].bytes;
+        phold_bytes_now += req[i].bytes;
+        phold_peak = std::max(phold_peak, phold_bytes_now);
         ++inflight;
         it = waiting.erase(it);
         progress = true;
@@ -438,6 +482,8 @@ gpq_status serve_run(const gpq_serve_config& cin, gpq_drive_mode mode, gpq_serve
     rep->phold_p99_ms = percentile(lat_p, 0.99);
     rep->astar_p99_ms = percentile(lat_a, 0.99);
     rep->peak_bytes = peak_bytes;
+    rep->phold_peak_bytes = phold_peak;
+    rep->pool_bytes = pool ? pool->device_bytes() : 0;
     rep->peak_inflight = peak_inflight;
   }
   if (out) std::copy(req.begin(), req.end(), out);

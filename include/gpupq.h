@@ -383,6 +383,10 @@ typedef struct {
   float phold_end_time;              /* 0: 20 */
   uint32_t phold_batch;              /* steps per graph replay of a served PHOLD request (0: 16) */
   uint32_t astar_lanes;              /* concurrent A* launches when served (0: 4) */
+  uint32_t plain_preallocated;       /* GPQ_DRIVE_PLAIN only: 1 = every PHOLD request's queue and model
+                                        are created before the clock starts and populated inside it,
+                                        so neither drive allocates inside the clock; 0 = each request
+                                        creates and destroys its own inside the clock (process per run) */
 } gpq_serve_config;
 
 typedef struct {
@@ -401,15 +405,19 @@ typedef struct {
   double requests_per_second;
   double p50_ms, p99_ms, mean_ms;    /* latency = completion - arrival */
   double phold_p99_ms, astar_p99_ms;
-  uint64_t peak_bytes;               /* most HBM held by admitted requests at once */
+  uint64_t peak_bytes;               /* most HBM held by admitted requests at once (PHOLD + A*) */
+  uint64_t phold_peak_bytes;         /* most HBM held by admitted PHOLD requests at once (<= hbm_budget) */
+  uint64_t pool_bytes;               /* served: HBM of the shared PHOLD pool (<= hbm_budget) */
 } gpq_serve_report;
 
 /* The trace alone (kind, size, arrival_ms of each request). */
 gpq_status gpq_serve_trace(const gpq_serve_config* cfg, gpq_request* out);
 /* Serves the trace.  GPQ_DRIVE_GRAPH: continuous batching -- admission in arrival order under
- * the HBM budget, each PHOLD request on its own stream by graph replays, A* queries gathered
+ * the HBM budget into one shared PHOLD pool advanced by graph replays, A* queries gathered
  * into persistent-kernel launches on astar_lanes streams.  GPQ_DRIVE_PLAIN: one request at a
- * time, a host synchronization per step.  out (may be NULL) receives every request. */
+ * time, a host synchronization per step.  A PHOLD request whose queue alone exceeds hbm_budget
+ * is rejected with GPQ_ERR_CAPACITY before anything runs.  out (may be NULL) receives every
+ * request. */
 gpq_status gpq_serve_run(const gpq_serve_config* cfg, gpq_drive_mode mode, gpq_serve_report* rep, gpq_request* out);
 
 /* ---- self-test hooks: the device random-number path evaluated in bulk ------------------- */

@@ -13,7 +13,9 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <atomic>
 #include <queue>
+#include <thread>
 #include <vector>
 
 // ------------------------------------------------------------------------------
@@ -280,6 +282,121 @@ extern "C" int orc_phold_windowed(const orc_phold_cfg* c, uint32_t* lp_count, ui
   finish(c, st, lp_count, lp_hash, res, now_s() - t0, pending);
   res->steps = steps;
   res->max_queue_len = max_len;
+  return 0;
+}
+
+// Windowed drain on `threads` host threads: every LP commits all of its events below
+// wend = min(gmin + lookahead, T) in key order, then the events created in the window are
+// delivered.  An event created in a window lies at or beyond its end ((ts + lookahead) + Exp
+// >= fl(gmin + lookahead) for ts >= gmin), so each LP commits the sequential trace.  Thread t
+// owns the LPs [t*L/threads, (t+1)*L/threads) and hands events for other threads' LPs over
+// through one outbox per (source thread, destination thread) pair.  This is the all-cores CPU
+// baseline; the trace equals orc_phold_seq's.
+namespace {
+struct SpinBarrier {
+  std::atomic<uint32_t> arrived{0}, phase{0};
+  uint32_t n;
+  explicit SpinBarrier(uint32_t n_) : n(n_) {}
+  void wait() {
+    const uint32_t p = phase.load(std::memory_order_acquire);
+    if (arrived.fetch_add(1, std::memory_order_acq_rel) + 1 == n) {
+      arrived.store(0, std::memory_order_relaxed);
+      phase.store(p + 1, std::memory_order_release);
+    } else {
+      while (phase.load(std::memory_order_acquire) == p) std::this_thread::yield();
+    }
+  }
+};
+}  // namespace
+
+extern "C" int orc_phold_parallel(const orc_phold_cfg* c, uint32_t threads, uint32_t* lp_count, uint64_t* lp_hash,
+                                  orc_phold_result* res) {
+  if (!c || !res || c->num_lps == 0 || threads == 0) return -1;
+  std::memset(res, 0, sizeof(*res));
+  const uint32_t L = c->num_lps;
+  if (threads > L) threads = L;
+  std::vector<Ev> init;
+  populate(c, init);
+  using PQ = std::priority_queue<Ev, std::vector<Ev>, KeyGreater>;
+  std::vector<PQ> q(L);
+  for (const Ev& e : init) q[e.dst].push(e);
+  init.clear();
+  init.shrink_to_fit();
+  LpState st;
+  init_state(c, st);
+  auto lo = [&](uint32_t t) { return (uint32_t)((uint64_t)L * t / threads); };
+  auto owner = [&](uint32_t lp) {   // largest t with lo(t) <= lp
+    uint32_t t = (uint32_t)((uint64_t)lp * threads / L);
+    while (t + 1 < threads && lo(t + 1) <= lp) ++t;
+    while (lo(t) > lp) --t;
+    return t;
+  };
+  std::vector<std::vector<Ev>> box((size_t)threads * threads);
+  std::vector<uint32_t> tmin(threads, 0xFFFFFFFFu);
+  std::vector<uint8_t> mono(threads, 1);
+  std::vector<uint64_t> tsteps(threads, 0);
+  SpinBarrier bar(threads);
+  const uint32_t t_end = fbits(c->end_time);
+  auto work = [&](uint32_t t) {
+    const uint32_t a = lo(t), b = lo(t + 1);
+    uint32_t m = 0xFFFFFFFFu;
+    for (uint32_t lp = a; lp < b; ++lp)
+      if (!q[lp].empty()) m = std::min(m, q[lp].top().ts);
+    tmin[t] = m;
+    bar.wait();
+    uint64_t steps = 0;
+    for (;;) {
+      uint32_t gmin = 0xFFFFFFFFu;
+      for (uint32_t u = 0; u < threads; ++u) gmin = std::min(gmin, tmin[u]);
+      if (gmin == 0xFFFFFFFFu || !(gmin < t_end)) break;
+      float wend = bitsf(gmin) + c->lookahead;
+      if (c->end_time < wend) wend = c->end_time;
+      const uint32_t wbits = fbits(wend);
+      ++steps;
+      for (uint32_t lp = a; lp < b; ++lp) {
+        PQ& h = q[lp];
+        while (!h.empty() && h.top().ts < wbits) {
+          const Ev e = h.top();
+          h.pop();
+          const uint32_t n = st.count[lp]++;
+          if (e.ts < st.last_ts[lp]) mono[t] = 0;
+          st.last_ts[lp] = e.ts;
+          st.hash[lp] = orc_trace_step(st.hash[lp], e.ts, e.src, e.seq);
+          uint32_t r[4];
+          draw(c->seed, n, lp, 0u, r);
+          const float ex = orc_exp_from_u32(r[0]);
+          const uint32_t dst = (r[1] < c->p_self_u32) ? lp : (uint32_t)(((uint64_t)r[2] * L) >> 32);
+          const Ev ne{fbits((bitsf(e.ts) + c->lookahead) + ex), dst, lp, c->init_per_lp + n};
+          if (dst >= a && dst < b) box[(size_t)t * threads + t].push_back(ne);
+          else box[(size_t)t * threads + owner(dst)].push_back(ne);
+        }
+      }
+      bar.wait();                                  // every outbox of this window is complete
+      for (uint32_t u = 0; u < threads; ++u) {
+        std::vector<Ev>& in = box[(size_t)u * threads + t];
+        for (const Ev& e : in) q[e.dst].push(e);
+        in.clear();
+      }
+      m = 0xFFFFFFFFu;
+      for (uint32_t lp = a; lp < b; ++lp)
+        if (!q[lp].empty()) m = std::min(m, q[lp].top().ts);
+      bar.wait();                                  // every thread read the old minima
+      tmin[t] = m;
+      bar.wait();                                  // the new minima are published
+    }
+    tsteps[t] = steps;
+  };
+  const double t0 = now_s();
+  std::vector<std::thread> pool;
+  for (uint32_t t = 1; t < threads; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+  const double secs = now_s() - t0;
+  size_t pending = 0;
+  for (uint32_t lp = 0; lp < L; ++lp) pending += q[lp].size();
+  for (uint32_t t = 0; t < threads; ++t) st.monotone = st.monotone && mono[t];
+  finish(c, st, lp_count, lp_hash, res, secs, pending);
+  res->steps = tsteps[0];
   return 0;
 }
 

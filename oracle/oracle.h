@@ -69,6 +69,10 @@ int orc_phold_seq(const orc_phold_cfg* cfg, uint32_t* lp_count, uint64_t* lp_has
 int orc_phold_brute(const orc_phold_cfg* cfg, uint32_t* lp_count, uint64_t* lp_hash, orc_phold_result* res);
 /* Windowed: per step, every LP whose head has ts < min(gmin + lookahead, T) processes exactly its head. */
 int orc_phold_windowed(const orc_phold_cfg* cfg, uint32_t* lp_count, uint64_t* lp_hash, orc_phold_result* res);
+/* Windowed drain on `threads` host threads (LPs split in contiguous ranges): the all-cores CPU
+ * baseline; commits the same trace as orc_phold_seq. */
+int orc_phold_parallel(const orc_phold_cfg* cfg, uint32_t threads, uint32_t* lp_count, uint64_t* lp_hash,
+                       orc_phold_result* res);
 uint64_t orc_phold_combine(const uint32_t* lp_count, const uint64_t* lp_hash, uint32_t num_lps);
 /* Per-LP chained trace hash step for one committed event (ts bits, src LP, seq). */
 uint64_t orc_trace_step(uint64_t h, uint32_t ts_bits, uint32_t src, uint32_t seq);

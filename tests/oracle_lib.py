@@ -82,11 +82,19 @@ class Oracle:
         return out
 
     # -- PHOLD
-    def phold(self, variant, num_lps, init_per_lp, end_time, seed=42, lookahead=0.1, p_self_u32=0x80000000):
+    def phold(self, variant, num_lps, init_per_lp, end_time, seed=42, lookahead=0.1, p_self_u32=0x80000000,
+              threads=1):
+        """variant: "seq", "brute", "windowed" or "parallel" (windowed drain on `threads` host threads)."""
         cfg = PholdCfg(num_lps, init_per_lp, seed, np.float32(end_time), np.float32(lookahead), p_self_u32)
         res = PholdRes()
         cnt = np.zeros(num_lps, dtype=np.uint32)
         hsh = np.zeros(num_lps, dtype=np.uint64)
+        if variant == "parallel":
+            rc = self.lib.orc_phold_parallel(ctypes.byref(cfg), ctypes.c_uint32(threads), _ptr(cnt), _ptr(hsh),
+                                             ctypes.byref(res))
+            if rc != 0:
+                raise RuntimeError("oracle phold failed: %d" % rc)
+            return res, cnt, hsh
         fn = {"seq": self.lib.orc_phold_seq, "brute": self.lib.orc_phold_brute,
               "windowed": self.lib.orc_phold_windowed}[variant]
         rc = fn(ctypes.byref(cfg), _ptr(cnt), _ptr(hsh), ctypes.byref(res))

@@ -82,6 +82,18 @@ def test_phold_windowed_matches_at_scale(oracle):
     assert b[0].steps > 100
 
 
+@pytest.mark.parametrize("threads", [1, 2, 3, 8])
+def test_phold_parallel_cpu_matches_sequential(oracle, threads):
+    # the all-cores CPU baseline (windowed drain, LPs split over host threads) commits the
+    # sequential trace, including with more threads than a small LP count divides evenly
+    for L, k, T in [(5, 2, 6.0), (1000, 8, 10.0)]:
+        a = oracle.phold("seq", L, k, T)
+        b = oracle.phold("parallel", L, k, T, threads=threads)
+        assert b[0].events == a[0].events and b[0].trace_hash == a[0].trace_hash
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+        assert b[0].in_flight_end == L * k and b[0].monotone_ok == 1
+
+
 def test_phold_rate_matches_model(oracle):
     # steady state: L*k events in flight, each lives lookahead + Exp(1) = 1.1 on average
     L, k, T = 1024, 16, 100.0

@@ -174,8 +174,6 @@ def test_phold_2p20_lps_matches_golden(gpq, golden, kind, drain):
     gd = golden.get("phold_2p20_k32_t50")
     if gd is None:
         pytest.skip("2^20-LP golden record not generated")
-    if kind == "hq4":
-        pytest.skip("the four-heap HQ is covered up to 65536 LPs (its steps launch four heaps' kernels)")
     g = run_gpu(gpq, kind, 1 << 20, 32, 50.0, checks=False, batch=64, drain=drain)
     assert g["flags"] == 0
     assert g["rep"]["events"] == gd["events"]
