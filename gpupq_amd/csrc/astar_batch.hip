@@ -158,7 +158,7 @@ __device__ bool search_query(const BatchDev& d, Shared& sh, uint32_t* cnt, uint6
                              uint32_t qidx, uint32_t slot, uint32_t max_iters, bool resume) {
   const uint32_t tid = threadIdx.x, K = blockDim.x, S = d.stage_cap;
   const BatchQuery Q = d.q[qidx];
-  const uint32_t W = Q.side, cells = Q.side * Q.side;
+  const uint32_t W = Q.side;
   const uint32_t* bits = d.bits + Q.bit_off;
   uint32_t* g = d.g + (uint64_t)slot * d.g_stride;
   const ThreadHeap hp{lds_rows + tid, d.heap + (uint64_t)slot * d.heap_cap * K + tid, K, d.lds_rows};
@@ -172,7 +172,9 @@ __device__ bool search_query(const BatchDev& d, Shared& sh, uint32_t* cnt, uint6
     if (tid == 0) d.clk[2 * qidx] = wall_clock64();
     const uint32_t prev = d.slot[slot].tag;
     if (prev == 0) {                                   // tags exhausted: clear the slot once
-      for (uint32_t c = tid; c < cells; c += K) g[c] = 0xFFFFFFFFu;
+      // the whole slot, not only this query's cells: a later, larger query in the slot would
+      // otherwise meet entries of older queries whose tags are below its own
+      for (uint64_t c = tid; c < d.g_stride; c += K) g[c] = 0xFFFFFFFFu;
     }
     tag = prev == 0 ? kTagNever - 1 : prev - 1;
     hn = 0;

@@ -333,3 +333,22 @@ def test_single_query_decrease_key_needs_enough_ids(gpq):
                    update_ids=100) as q:
         with pytest.raises(gpq.GpqError):
             gpq.Astar(q, 97, 61)
+
+
+def test_batch_slot_reuse_past_tag_exhaustion(gpq, oracle):
+    # one slot serves 66 queries: the 64th exhausts the slot's generation tags and clears it;
+    # it is smaller than the queries before and after it, so the clear must cover the whole
+    # slot and not only its own cells (the 65th and 66th read cells the 64th never touched)
+    S = gpq._lib.QuerySpec
+    specs = []
+    for q in range(66):
+        side = 8 if q == 63 else 128
+        specs.append(S(side, thr(0.2), 0, 0, side - 1, side - 1, 8, 9, 1, q))
+    with gpq.AstarBatch(specs, threads_per_query=64, max_resident=1) as b:
+        b.run("graph")
+        costs, _ = b.results()
+        for q in (0, 62, 63, 64, 65):
+            s = specs[q]
+            grid = b.grid(q).reshape(-1)
+            c, _ = oracle.astar(grid, s.side, s.side, 8, (s.sx, s.sy), (s.tx, s.ty))
+            assert costs[q] == c, "query %d" % q
