@@ -54,6 +54,8 @@ QUEUES = {
 DRAIN = False
 FAR = 0            # LAQ two-tier mode: far-store records per LP (0 = one sorted ring per LP)
 FAR_STEP = 0.0     # LAQ two-tier: near-horizon step (0 = 4 x lookahead)
+CAL = 0            # LAQ calendar mode: buckets (windows) per LP (0 = off)
+CAL_SLOTS = 16     # LAQ calendar mode: records per bucket
 
 
 def make(gpq, kind, wl, stats=False, checks=False):
@@ -67,6 +69,8 @@ def make(gpq, kind, wl, stats=False, checks=False):
         args.update(node_size=max(-(-wl["num_lps"] // heaps), 32), heaps=heaps)
     if kind == "laq" and FAR:
         args.update(capacity_per_lp=64, far_per_lp=FAR, far_step=FAR_STEP)
+    if kind == "laq" and CAL and DRAIN == "fused":
+        args.update(capacity_per_lp=CAL_SLOTS, stage_per_lp=32, far_per_lp=FAR or 128, calendar_windows=CAL)
     q = gpq.Queue(kind, num_lps=wl["num_lps"], lookahead=0.1, horizon=wl["end_time"], block_threads=256,
                   collect_stats=stats, **args)
     m = gpq.Phold(q, init_per_lp=wl["init_per_lp"], seed=42, end_time=wl["end_time"], checks=checks, drain=DRAIN)
@@ -439,6 +443,9 @@ def main():
     ap.add_argument("--far", type=int, default=128,
                     help="LAQ two-tier mode: far-store records per LP (0 = single sorted ring)")
     ap.add_argument("--far-step", type=float, default=0.0, help="LAQ two-tier: near-horizon step (0 = 4 x lookahead)")
+    ap.add_argument("--calendar", type=int, default=0,
+                    help="LAQ calendar mode for the merge-and-commit step: buckets (windows) per LP (0 = off)")
+    ap.add_argument("--cal-slots", type=int, default=16, help="LAQ calendar mode: records per bucket")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-astar", action="store_true")
@@ -476,8 +483,9 @@ def main():
     DRAIN = False if step == "one-event" or args.queue not in ("laq", "lhq", "lsq") else \
         ("fused" if step == "fused" else True)
     FAR = args.far
-    global FAR_STEP
+    global FAR_STEP, CAL, CAL_SLOTS
     FAR_STEP = args.far_step
+    CAL, CAL_SLOTS = args.calendar, args.cal_slots
     wl = WORKLOADS[args.workload]
     golden = None
     gpath = os.path.join(ROOT, "tests", "golden", "phold_golden.json")
@@ -528,6 +536,8 @@ def main():
                               if DRAIN == "fused" else
                               "drain each LP's safe window" if DRAIN else "one event per LP per window"),
                    "laq_far_per_lp": FAR,
+                   "laq_calendar": ({"windows": CAL, "slots_per_bucket": CAL_SLOTS} if CAL and DRAIN == "fused"
+                                    else None),
                    "trace_hash_vs_cpu": parity},
     }
     def roofline_line():

@@ -46,12 +46,13 @@ class Queue:
 
     def __init__(self, kind="laq", num_lps=1024, capacity_per_lp=64, stage_per_lp=16, node_size=0,
                  heap_nodes=0, total_capacity=0, lookahead=0.1, horizon=float("inf"), block_threads=256,
-                 collect_stats=False, far_per_lp=0, far_step=0.0, update_ids=0, update_per_lp=0, heaps=0):
+                 collect_stats=False, far_per_lp=0, far_step=0.0, update_ids=0, update_per_lp=0, heaps=0,
+                 calendar_windows=0):
         self.kind = kind
         self.num_lps = num_lps
         cfg = QueueConfig(_lib.QUEUE_KINDS[kind], num_lps, capacity_per_lp, stage_per_lp, node_size,
                           heap_nodes, total_capacity, lookahead, horizon, block_threads, 1 if collect_stats else 0,
-                          far_per_lp, far_step, update_ids, update_per_lp, heaps)
+                          far_per_lp, far_step, update_ids, update_per_lp, heaps, calendar_windows)
         self.cfg = cfg
         h = ctypes.c_void_p()
         check(lib.gpq_queue_create(ctypes.byref(cfg), ctypes.byref(h)), "queue_init")

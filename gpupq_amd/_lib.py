@@ -26,7 +26,7 @@ class QueueConfig(ctypes.Structure):
                 ("stage_per_lp", c_u32), ("node_size", c_u32), ("heap_nodes", c_u32),
                 ("total_capacity", c_u64), ("lookahead", c_f32), ("horizon", c_f32),
                 ("block_threads", c_u32), ("collect_stats", c_u32), ("far_per_lp", c_u32), ("far_step", c_f32),
-                ("update_ids", c_u32), ("update_per_lp", c_u32), ("heaps", c_u32)]
+                ("update_ids", c_u32), ("update_per_lp", c_u32), ("heaps", c_u32), ("calendar_windows", c_u32)]
 
 
 class PholdConfig(ctypes.Structure):
@@ -164,6 +164,8 @@ SIGNATURES = [
     ("gpq_selftest_select", ctypes.c_int, [c_void_p, c_u32, c_u32, c_f32, c_void_p, c_void_p, ctypes.POINTER(c_u32),
                                            c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     ("gpq_selftest_philox", ctypes.c_int, [c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_void_p]),
+    ("gpq_queue_device_view", ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_int), c_void_p, c_u64]),
+    ("gpq_queue_stream", ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
 ]
 
 _lib = None

@@ -59,5 +59,22 @@ def build(verbose=True):
     return OUT
 
 
+def build_model(src, out, verbose=True):
+    """Compiles a caller's model kernels (`src`, HIP) against the public headers under include/
+    into the shared library `out`.  Nothing of libgpupq is linked: the model reaches the queue
+    through gpq_queue_device_view at run time, and its kernels inline the device verbs of
+    include/gpupq_device.h."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *FLAGS, "-I" + inc, "-shared", src, "-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), res.stderr))
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build()

@@ -289,6 +289,40 @@ gpq_status gpq_queue_length(gpq_queue* q, uint32_t lp, uint32_t* n) {
 
 gpq_status gpq_queue_clear_lp(gpq_queue* q, uint32_t lp) { GPQ_Q(q); return q->impl->clear_lp(lp); }
 
+gpq_status gpq_queue_device_view(gpq_queue* q, gpq_queue_kind* kind, void* view, uint64_t view_bytes) {
+  GPQ_Q(q);
+  if (!view) { set_error("null view"); return GPQ_ERR_INVALID; }
+  if (q->impl->cfg.kind == GPQ_LAQ) {
+    const gpq::LaqView v = static_cast<gpq::LaqQueue*>(q->impl)->view();
+    if (v.wlog) {
+      set_error("device_view: a calendar-mode LAQ serves the library's PHOLD step only");
+      return GPQ_ERR_STATE;
+    }
+    if (v.stage_alt) {
+      set_error("device_view: the queue is driven by a PHOLD model that merges and commits in one kernel; "
+                "its staging buffers alternate per step");
+      return GPQ_ERR_STATE;
+    }
+  }
+  return gpq::with_view(q->impl, [&](const auto& v) -> gpq_status {
+    if (view_bytes != sizeof(v)) {
+      set_error("device_view: view_bytes " + std::to_string(view_bytes) + " differs from the view size " +
+                std::to_string(sizeof(v)) + " of the queue's kind");
+      return GPQ_ERR_INVALID;
+    }
+    std::memcpy(view, &v, sizeof(v));
+    if (kind) *kind = q->impl->cfg.kind;
+    return GPQ_OK;
+  });
+}
+
+gpq_status gpq_queue_stream(gpq_queue* q, void** stream) {
+  GPQ_Q(q);
+  if (!stream) { set_error("null out"); return GPQ_ERR_INVALID; }
+  *stream = static_cast<void*>(q->impl->stream);
+  return GPQ_OK;
+}
+
 gpq_status gpq_phold_create(gpq_queue* q, const gpq_phold_config* cfg, gpq_phold** out) {
   GPQ_Q(q);
   if (!cfg || !out) { set_error("null argument"); return GPQ_ERR_INVALID; }

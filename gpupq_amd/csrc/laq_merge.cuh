@@ -4,7 +4,8 @@
 // src/sort.cu:329-380) and the heap-backed variant's near/far split (:515-681).
 #pragma once
 
-#include "queue_laq.cuh"
+#include "../../include/gpupq_device/queue_laq.cuh"
+#include "gpq_device.h"
 
 namespace gpq {
 namespace laq {

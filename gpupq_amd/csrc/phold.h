@@ -43,6 +43,7 @@ class PholdModel {
   // and the closing fold of the last step's staged events (a no-op in the other modes)
   gpq_status launch_fused(uint32_t settle);
   gpq_status settle();
+  bool is_calendar() const;     // the queue is a calendar-mode LAQ
 
   gpq_phold_config cfg{};
   QueueBase* q = nullptr;

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cstring>
 
+#include "laq_calendar.cuh"
 #include "phold.h"
 #include "phold_fused.cuh"
 
@@ -124,6 +125,13 @@ __global__ __launch_bounds__(kFusedBlock, 4) void phold_fused_kernel(LaqView q, 
   fused_block<G, SL>(q, fm, settle);
 }
 
+// Commit part of the step on a calendar-mode LAQ (laq_calendar.cuh), after the queue grouped the
+// window's log by LP: the whole queue is one instance.
+__global__ __launch_bounds__(cal::kBlock) void phold_calendar_kernel(LaqView q, PholdDev m, float end_time) {
+  const FusedModel fm{q.ws, m.count, m.hash, m.last_ts, 0u, m.num_lps, m.init_per_lp, m.seed, m.p_self, m.lookahead};
+  cal::commit_block(q, fm, end_time);
+}
+
 // phold_populate (src/phold.cu:63-79): initial event j of every LP.
 template <class Q>
 __global__ void phold_populate_kernel(Q q, PholdDev m, uint32_t j) {
@@ -231,7 +239,9 @@ gpq_status PholdModel::create(QueueBase* queue, const gpq_phold_config& c, Phold
   GPQ_CHECK_ARG(!c.drain || queue->cfg.kind == GPQ_LAQ || queue->cfg.kind == GPQ_LHQ || queue->cfg.kind == GPQ_LSQ,
                 "phold drain mode needs a per-LP queue (peek by LP)");
   GPQ_CHECK_ARG(c.drain <= 2, "phold drain must be 0, 1 or 2");
-  if (c.drain == 2) {
+  const bool calendar = queue->cfg.kind == GPQ_LAQ && static_cast<LaqQueue*>(queue)->calendar();
+  GPQ_CHECK_ARG(!calendar || c.drain == 2, "a calendar-mode LAQ serves the merge-and-commit step (drain 2) only");
+  if (c.drain == 2 && !calendar) {
     GPQ_CHECK_ARG(queue->cfg.kind == GPQ_LAQ && queue->cfg.far_per_lp > 0 && queue->cfg.capacity_per_lp <= 64,
                   "phold drain mode 2 (merge and commit in one step) needs the two-tier LAQ with capacity_per_lp <= 64");
     GPQ_TRY(static_cast<LaqQueue*>(queue)->enable_alt_stage());
@@ -276,6 +286,7 @@ gpq_status PholdModel::init() {
     }));
     GPQ_TRY(q->post_init());
   }
+  if (is_calendar()) GPQ_TRY(static_cast<LaqQueue*>(q)->calendar_start());
   GPQ_HIP(hipStreamSynchronize(q->stream));
   uint32_t flags = 0;
   GPQ_TRY(q->error_flags(&flags));
@@ -286,8 +297,20 @@ gpq_status PholdModel::init() {
   return GPQ_OK;
 }
 
+bool PholdModel::is_calendar() const {
+  return q->cfg.kind == GPQ_LAQ && static_cast<const LaqQueue*>(q)->calendar();
+}
+
 gpq_status PholdModel::launch_fused(uint32_t settle) {
   auto* laq = static_cast<LaqQueue*>(q);
+  if (laq->calendar()) {     // nothing is staged between calendar steps: no closing fold
+    if (settle) return GPQ_OK;
+    GPQ_TRY(laq->calendar_group());
+    hipLaunchKernelGGL(phold_calendar_kernel, dim3(ceil_div(cfg.num_lps, cal::kBlock)), dim3(cal::kBlock), 0,
+                       q->stream, laq->view(), dev_of(this), cfg.end_time);
+    GPQ_HIP(hipGetLastError());
+    return GPQ_OK;
+  }
   hipLaunchKernelGGL((phold_fused_kernel<8, laq::kSlots>), dim3(laq->partial_slots), dim3(kFusedBlock), 0, q->stream,
                      laq->view(), dev_of(this), settle);
   GPQ_HIP(hipGetLastError());
@@ -297,7 +320,7 @@ gpq_status PholdModel::launch_fused(uint32_t settle) {
 // Folds the last step's staged events into the rings and far stores, so the queue holds
 // every pending event as after a queue_post.
 gpq_status PholdModel::settle() {
-  if (cfg.drain != 2) return GPQ_OK;
+  if (cfg.drain != 2 || is_calendar()) return GPQ_OK;
   GPQ_TRY(launch_fused(1));
   return static_cast<LaqQueue*>(q)->reset_stage_select();
 }

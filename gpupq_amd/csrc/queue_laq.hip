@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "laq_calendar.cuh"
 #include "laq_merge.cuh"
 #include "phold_fused.cuh"
 #include "queues.h"
@@ -366,9 +367,164 @@ __global__ void laq_update_kernel(LaqView q, uint32_t* partial) {
   atomicMin(&partial[lp / kLaqBlock], h);
 }
 
+// ---- calendar mode (laq_calendar.cuh) -------------------------------------------------
+
+// queue_post_init: the staged initial events go to the beyond-the-calendar list of parity 0
+// (the window ends, and so the logs, are known only once every initial event is in); the block
+// minima of their timestamps feed cal_init_publish_kernel.
+__global__ __launch_bounds__(kLaqBlock) void cal_post_init_kernel(LaqView q, uint32_t* partial) {
+  const uint32_t lp = blockIdx.x * kLaqBlock + threadIdx.x;
+  uint32_t m = kInfBits;
+  if (lp < q.num_lps) {
+    const uint32_t k0 = q.ins[lp];
+    const uint32_t k = k0 < q.stage_cap ? k0 : q.stage_cap;
+    for (uint32_t i = 0; i < k; ++i) {
+      const Event e = load_event(&q.stage[(size_t)lp * q.stage_cap + i]);
+      m = e.ts < m ? e.ts : m;
+      const uint32_t pos = wave_append_slot(&q.over_n[0]);
+      if (pos < q.over_cap) store_event(&q.over[pos], e);
+      else atomicOr(&q.ws->error, kErrQueueOverflow);
+    }
+    if (k0) q.ins[lp] = 0;
+  }
+  block_min_to_partial(m, partial);
+}
+
+// Window publication after a post_init chunk: gmin = the smaller of the chunk's minimum and
+// the minimum of the chunks before, the window end one lookahead above it (capped).
+__global__ void cal_init_publish_kernel(const uint32_t* partial, uint32_t count, WindowState* ws, float lookahead,
+                                        float horizon) {
+  __shared__ uint32_t s_w[16];
+  uint32_t m = kInfBits;
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) m = partial[i] < m ? partial[i] : m;
+  m = wave_min_u32(m);
+  if (lane_id() == 0) s_w[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t g = ws->gmin;
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) g = s_w[i] < g ? s_w[i] : g;
+    float wend = bits_to_f(g) + lookahead;
+    if (horizon < wend) wend = horizon;
+    ws->gmin = g;
+    ws->wend = f_to_bits(wend);
+    ws->done = (g == kInfBits || !(bits_to_f(g) < horizon)) ? 1u : 0u;
+    ws->windows += 1;
+  }
+}
+
+// The window ends of windows 0 .. B - 1 from the first one (published from the initial
+// events), each one lookahead after the previous in float arithmetic, capped at the end time.
+__global__ void cal_start_kernel(LaqView q, float lookahead, float end_time) {
+  CalState* st = q.cal_st;
+  const uint32_t B = q.cal_windows;
+  st->cur = 0;
+  uint32_t w = q.ws->wend;
+  st->w[0] = w;
+  for (uint32_t i = 1; i < B; ++i) {
+    float nw = bits_to_f(w) + lookahead;
+    if (end_time < nw) nw = end_time;
+    w = f_to_bits(nw);
+    st->w[i] = w;
+  }
+  q.ws->sel = 0;
+}
+
+// Files every initial event (the list of parity 0) in the log of its window (windows 0 .. B - 1),
+// in the beyond-the-calendar list of parity 1 (read by the first step), or with the parked ones;
+// appends are aggregated per block and route.
+__global__ __launch_bounds__(cal::kBlock) void cal_fill_kernel(LaqView q, float end_time) {
+  __shared__ uint32_t sw[kCalMaxWindows];
+  __shared__ cal::BlockAppend ba;
+  const uint32_t cur = cal::load_windows(q, sw);
+  __syncthreads();
+  const uint32_t B = q.cal_windows, end_bits = f_to_bits(end_time);
+  const uint32_t n0 = q.over_n[0], n = n0 < q.over_cap ? n0 : q.over_cap;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {   // block-uniform trip count
+    const uint32_t i = base + threadIdx.x;
+    const bool has = i < n;
+    const Event e = has ? load_event(&q.over[i]) : Event{};
+    const uint32_t r = has ? cal::route_of(sw, B, e.ts, end_bits, 0u) : cal::Route::kNone;
+    cal::block_append(q, cur, 1u, r, e, ba);    // parity 1: the list the first step reads
+  }
+}
+
+__global__ void cal_fill_done_kernel(LaqView q) { q.over_n[0] = 0; }
+
+// Step part 1: the log of the window to commit, grouped by LP into the staging area.
+__global__ __launch_bounds__(cal::kBlock) void cal_group_kernel(LaqView q, float end_time) {
+  cal::group_block(q, end_time);
+}
+
+// queue_post after a calendar step: the window advances by the lookahead (capped at the end
+// time) and the calendar by one window, whose end is appended to the ring; the committed
+// window's log and the beyond-the-calendar list the step read are emptied; the window that
+// reached the end time ends the run.
+__global__ void cal_publish_kernel(LaqView q, float lookahead, float end_time) {
+  WindowState w = *q.ws;
+  if (w.done) return;
+  CalState* st = q.cal_st;
+  const uint32_t B = q.cal_windows;
+  const uint32_t cur = st->cur;
+  q.wlog_n[cur & (B - 1u)] = 0;
+  q.over_n[(cur & 1u) ^ 1u] = 0;
+  w.windows += 1;
+  w.gmin = w.wend;                                 // a lower bound of the pending minimum
+  if (!(bits_to_f(w.wend) < end_time)) {
+    w.done = 1;
+  } else {
+    const uint32_t nc = cur + 1u;
+    st->cur = nc;
+    float nw = bits_to_f(st->w[(nc + B - 2u) & (B - 1u)]) + lookahead;
+    if (end_time < nw) nw = end_time;
+    st->w[(nc + B - 1u) & (B - 1u)] = f_to_bits(nw);
+    w.wend = st->w[nc & (B - 1u)];
+  }
+  *q.ws = w;
+}
+
+// Pending records: every window log, both beyond-the-calendar lists, the parked ones and the
+// staging area (one thread; the counters are few).
+__global__ void cal_count_kernel(LaqView q, unsigned long long* sum) {
+  unsigned long long n = 0;
+  for (uint32_t b = 0; b < q.cal_windows; ++b) n += min(q.wlog_n[b], q.wlog_cap);
+  for (uint32_t p = 0; p < 2; ++p) n += min(q.over_n[p], q.over_cap);
+  n += min(*q.parked_n, q.parked_cap);
+  *sum = n;
+}
+
+// Every pending record owned by LP lp (unsorted) into out[0 .. *n) (at most cap), grid-stride
+// over every list.
+__device__ __forceinline__ void gather_one(const Event* e, uint32_t lp, Event* out, uint32_t cap, uint32_t* n) {
+  if (e->dst == lp) {
+    const uint32_t k = atomicAdd(n, 1u);
+    if (k < cap) out[k] = *e;
+  }
+}
+
+__global__ void cal_gather_kernel(LaqView q, uint32_t lp, Event* out, uint32_t cap, uint32_t* n) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t b = 0; b < q.cal_windows; ++b) {
+    const uint32_t c = min(q.wlog_n[b], q.wlog_cap);
+    for (uint32_t i = t; i < c; i += stride) gather_one(&q.wlog[(size_t)b * q.wlog_cap + i], lp, out, cap, n);
+  }
+  for (uint32_t p = 0; p < 2; ++p) {
+    const uint32_t c = min(q.over_n[p], q.over_cap);
+    for (uint32_t i = t; i < c; i += stride) gather_one(&q.over[(size_t)p * q.over_cap + i], lp, out, cap, n);
+  }
+  const uint32_t c = min(*q.parked_n, q.parked_cap);
+  for (uint32_t i = t; i < c; i += stride) gather_one(&q.parked[i], lp, out, cap, n);
+  if (t == 0) {
+    const uint32_t k = min(q.ins[lp], q.stage_cap);
+    for (uint32_t i = 0; i < k; ++i) gather_one(&q.stage[(size_t)lp * q.stage_cap + i], lp, out, cap, n);
+  }
+}
+
 }  // namespace
 
 gpq_status LaqQueue::init() {
+  if (cfg.calendar_windows) return init_calendar();
   GPQ_CHECK_ARG(is_pow2(cfg.capacity_per_lp) && cfg.capacity_per_lp >= 2,
                 "LAQ: capacity_per_lp must be a power of two >= 2");
   GPQ_CHECK_ARG(cfg.stage_per_lp >= 1 && cfg.stage_per_lp <= 64, "LAQ: stage_per_lp must be in [1, 64]");
@@ -423,7 +579,76 @@ gpq_status LaqQueue::init() {
   return GPQ_OK;
 }
 
+gpq_status LaqQueue::init_calendar() {
+  const uint32_t L = cfg.num_lps, B = cfg.calendar_windows;
+  GPQ_CHECK_ARG(is_pow2(B) && B >= 8 && B <= kCalMaxWindows, "LAQ calendar: calendar_windows must be a power of "
+                "two in [8, 256]");
+  GPQ_CHECK_ARG(cfg.capacity_per_lp >= 1 && cfg.capacity_per_lp <= 64,
+                "LAQ calendar: capacity_per_lp (a window's records per LP, on average over the LPs) must be in [1, 64]");
+  GPQ_CHECK_ARG(cfg.far_per_lp >= 1 && cfg.far_per_lp <= 4096, "LAQ calendar: far_per_lp must be in [1, 4096]");
+  GPQ_CHECK_ARG(cfg.stage_per_lp >= 1 && cfg.stage_per_lp <= 64, "LAQ calendar: stage_per_lp must be in [1, 64]");
+  GPQ_CHECK_ARG(std::isfinite(cfg.lookahead) && cfg.lookahead > 0.0f && std::isfinite(cfg.horizon),
+                "LAQ calendar: needs a finite lookahead > 0 and a finite horizon");
+  GPQ_CHECK_ARG(!cfg.update_ids, "LAQ calendar: no decrease-key mode");
+  GPQ_CHECK_ARG((uint64_t)L * cfg.capacity_per_lp < (1ull << 32) && (uint64_t)L * cfg.far_per_lp < (1ull << 32),
+                "LAQ calendar: num_lps too large for the window logs or the lists");
+  v_.num_lps = L;
+  v_.cap_log2 = 1;                    // no sorted run in this mode: a two-slot ring keeps the words valid
+  v_.stage_cap = cfg.stage_per_lp;
+  v_.cal_windows = B;
+  v_.wlog_cap = L * cfg.capacity_per_lp;
+  v_.over_cap = L * cfg.far_per_lp;
+  v_.parked_cap = L * cfg.far_per_lp;
+  far_step_ = 0.0f;
+  GPQ_TRY(arena.alloc(&v_.ring, (size_t)L << v_.cap_log2));
+  GPQ_TRY(arena.alloc(&v_.stage, (size_t)L * v_.stage_cap));
+  GPQ_TRY(arena.alloc(&v_.rd, L));
+  GPQ_TRY(arena.alloc(&v_.len, L));
+  GPQ_TRY(arena.alloc(&v_.ins, L));
+  GPQ_TRY(arena.alloc(&v_.head, L));
+  GPQ_TRY(arena.alloc(&sum_, 1));
+  GPQ_TRY(arena.alloc(&v_.wlog, (size_t)B * v_.wlog_cap));
+  GPQ_TRY(arena.alloc(&v_.wlog_n, B + 2));        // + over_n[2]
+  v_.over_n = v_.wlog_n + B;
+  GPQ_TRY(arena.alloc(&v_.over, 2 * (size_t)v_.over_cap));
+  GPQ_TRY(arena.alloc(&v_.parked, (size_t)v_.parked_cap));
+  GPQ_TRY(arena.alloc(&v_.parked_n, 1));
+  GPQ_TRY(arena.alloc(&v_.cal_st, 1));
+  GPQ_TRY(arena.alloc(&gather_, kGatherCap + 1));
+  GPQ_HIP(hipMemsetAsync(v_.wlog_n, 0, sizeof(uint32_t) * (B + 2), stream));
+  GPQ_HIP(hipMemsetAsync(v_.parked_n, 0, sizeof(uint32_t), stream));
+  GPQ_HIP(hipMemsetAsync(v_.cal_st, 0, sizeof(CalState), stream));
+  v_.ws = ws;
+  v_.stats = stats;
+  partial_slots = ceil_div(L, kLaqBlock);
+  GPQ_TRY(arena.alloc(&partial, (size_t)partial_slots));
+  hipLaunchKernelGGL(laq_init_kernel, dim3(ceil_div(L, 256)), dim3(256), 0, stream, v_);
+  GPQ_HIP(hipGetLastError());
+  return GPQ_OK;
+}
+
+gpq_status LaqQueue::calendar_start() {
+  GPQ_CHECK_ARG(v_.wlog, "LAQ: calendar_start needs calendar mode");
+  hipLaunchKernelGGL(cal_start_kernel, dim3(1), dim3(1), 0, stream, v_, cfg.lookahead, cfg.horizon);
+  hipLaunchKernelGGL(cal_fill_kernel, dim3(cal::kGroupBlocks), dim3(cal::kBlock), 0, stream, v_, cfg.horizon);
+  hipLaunchKernelGGL(cal_fill_done_kernel, dim3(1), dim3(1), 0, stream, v_);
+  GPQ_HIP(hipGetLastError());
+  return GPQ_OK;
+}
+
+gpq_status LaqQueue::calendar_group() {
+  hipLaunchKernelGGL(cal_group_kernel, dim3(cal::kGroupBlocks), dim3(cal::kBlock), 0, stream, v_, cfg.horizon);
+  GPQ_HIP(hipGetLastError());
+  return GPQ_OK;
+}
+
 gpq_status LaqQueue::post() {
+  if (v_.wlog) {             // the calendar step committed its window: the window and calendar move
+    if (profile_mark) GPQ_HIP(hipEventRecord(profile_mark, stream));
+    hipLaunchKernelGGL(cal_publish_kernel, dim3(1), dim3(1), 0, stream, v_, cfg.lookahead, cfg.horizon);
+    GPQ_HIP(hipGetLastError());
+    return GPQ_OK;
+  }
   if (v_.stage_alt) {        // the model step merged and committed: only the window moves
     if (profile_mark) GPQ_HIP(hipEventRecord(profile_mark, stream));
     hipLaunchKernelGGL(laq_step_publish_kernel, dim3(1), dim3(1), 0, stream, ws, cfg.lookahead, cfg.horizon,
@@ -447,6 +672,15 @@ gpq_status LaqQueue::empty_all() {
   const uint32_t n = std::max(cfg.num_lps, words);
   hipLaunchKernelGGL(laq_empty_kernel, dim3(std::min<uint32_t>(ceil_div(n, 256), 4096)), dim3(256), 0, stream, v_,
                      words);
+  GPQ_HIP(hipGetLastError());
+  return GPQ_OK;
+}
+
+gpq_status LaqQueue::post_init() {
+  if (!v_.wlog) return merge_publish();
+  hipLaunchKernelGGL(cal_post_init_kernel, dim3(partial_slots), dim3(kLaqBlock), 0, stream, v_, partial);
+  hipLaunchKernelGGL(cal_init_publish_kernel, dim3(1), dim3(1024), 0, stream, partial, partial_slots, ws,
+                     cfg.lookahead, cfg.horizon);
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }
@@ -497,6 +731,13 @@ gpq_status LaqQueue::enable_alt_stage() {
 }
 
 gpq_status LaqQueue::pending(uint64_t* n) {
+  if (v_.wlog) {
+    hipLaunchKernelGGL(cal_count_kernel, dim3(1), dim3(1), 0, stream, v_, reinterpret_cast<unsigned long long*>(sum_));
+    GPQ_HIP(hipGetLastError());
+    GPQ_HIP(hipMemcpyAsync(n, sum_, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    GPQ_HIP(hipStreamSynchronize(stream));
+    return GPQ_OK;
+  }
   GPQ_TRY(sum_u32(v_.len, cfg.num_lps, sum_, n));
   uint64_t x = 0;
   if (v_.far_cap) {
@@ -512,8 +753,35 @@ gpq_status LaqQueue::pending(uint64_t* n) {
   return GPQ_OK;
 }
 
+gpq_status LaqQueue::calendar_gather(uint32_t lp, std::vector<Event>* out) {
+  uint32_t* d_n = reinterpret_cast<uint32_t*>(gather_ + kGatherCap);
+  GPQ_HIP(hipMemsetAsync(d_n, 0, sizeof(uint32_t), stream));
+  hipLaunchKernelGGL(cal_gather_kernel, dim3(1024), dim3(256), 0, stream, v_, lp, gather_, kGatherCap, d_n);
+  GPQ_HIP(hipGetLastError());
+  uint32_t n = 0;
+  GPQ_HIP(hipMemcpyAsync(&n, d_n, 4, hipMemcpyDeviceToHost, stream));
+  GPQ_HIP(hipStreamSynchronize(stream));
+  if (n > kGatherCap) {
+    set_error("LAQ calendar: LP " + std::to_string(lp) + " holds more than " + std::to_string(kGatherCap) +
+              " records");
+    return GPQ_ERR_CAPACITY;
+  }
+  out->resize(n);
+  if (n) {
+    GPQ_HIP(hipMemcpyAsync(out->data(), gather_, sizeof(Event) * n, hipMemcpyDeviceToHost, stream));
+    GPQ_HIP(hipStreamSynchronize(stream));
+  }
+  return GPQ_OK;
+}
+
 gpq_status LaqQueue::length(uint32_t lp, uint32_t* n) {
   GPQ_CHECK_ARG(lp < cfg.num_lps, "lp out of range");
+  if (v_.wlog) {
+    std::vector<Event> all;
+    GPQ_TRY(calendar_gather(lp, &all));
+    *n = (uint32_t)all.size();
+    return GPQ_OK;
+  }
   uint32_t f = 0;
   GPQ_HIP(hipMemcpyAsync(n, v_.len + lp, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
   if (v_.far_cap) GPQ_HIP(hipMemcpyAsync(&f, v_.nfar + lp, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
@@ -524,6 +792,10 @@ gpq_status LaqQueue::length(uint32_t lp, uint32_t* n) {
 
 gpq_status LaqQueue::clear_lp(uint32_t lp) {
   GPQ_CHECK_ARG(lp < cfg.num_lps, "lp out of range");
+  if (v_.wlog) {
+    set_error("LAQ calendar mode: per-LP clear is not provided");
+    return GPQ_ERR_UNSUPPORTED;
+  }
   hipLaunchKernelGGL(laq_clear_kernel, dim3(1), dim3(1), 0, stream, v_, lp);
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
@@ -531,6 +803,18 @@ gpq_status LaqQueue::clear_lp(uint32_t lp) {
 
 gpq_status LaqQueue::lp_contents(uint32_t lp, Event* out, uint32_t cap, uint32_t* n) {
   GPQ_CHECK_ARG(lp < cfg.num_lps, "lp out of range");
+  if (v_.wlog) {
+    std::vector<Event> all;
+    GPQ_TRY(calendar_gather(lp, &all));
+    std::sort(all.begin(), all.end(), [](const Event& a, const Event& b) {
+      if (a.ts != b.ts) return a.ts < b.ts;
+      if (a.src != b.src) return a.src < b.src;
+      return a.seq < b.seq;
+    });
+    *n = (uint32_t)all.size();
+    for (uint32_t i = 0; i < *n && i < cap; ++i) out[i] = all[i];
+    return GPQ_OK;
+  }
   uint32_t rd = 0, len = 0, nf = 0;
   const uint32_t C = 1u << v_.cap_log2;
   std::vector<Event> ring(C), far(v_.far_cap);

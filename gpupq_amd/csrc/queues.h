@@ -3,11 +3,12 @@
 #pragma once
 
 #include "gpq_host.h"
-#include "queue_laq.cuh"
-#include "queue_lhq.cuh"
-#include "queue_lsq.cuh"
-#include "queue_gaq.cuh"
-#include "queue_hq.cuh"
+#include "../../include/gpupq_device/queue_laq.cuh"
+#include "../../include/gpupq_device/queue_lhq.cuh"
+#include "../../include/gpupq_device/queue_lsq.cuh"
+#include "../../include/gpupq_device/queue_gaq.cuh"
+#include "../../include/gpupq_device/queue_hq.cuh"
+#include "sortprim.h"
 
 namespace gpq {
 
@@ -18,7 +19,7 @@ class LaqQueue final : public QueueBase {
   explicit LaqQueue(const gpq_queue_config& c) : QueueBase(c) {}
   gpq_status init() override;
   gpq_status post() override;
-  gpq_status post_init() override { return merge_publish(); }
+  gpq_status post_init() override;
   gpq_status lp_contents(uint32_t lp, Event* out, uint32_t cap, uint32_t* n) override;
   gpq_status pending(uint64_t* n) override;
   gpq_status length(uint32_t lp, uint32_t* n) override;
@@ -43,6 +44,14 @@ class LaqQueue final : public QueueBase {
   // Decrease-key mode: every LP emptied, every id unmarked (a new search on the same queue).
   gpq_status empty_all();
   float far_step() const { return far_step_; }
+  // Calendar mode: whether it is on; and, once every initial event is in (post_init), the
+  // ends of the first windows and the filing of the initial events in their buckets.
+  bool calendar() const { return v_.wlog != nullptr; }
+  gpq_status calendar_start();
+  // Calendar mode, step part 1: the log of the window the next step commits, grouped by LP
+  // into the staging area (part 2 is the model's commit kernel).
+  gpq_status calendar_group();
+  static constexpr uint32_t kGatherCap = 1u << 16;   // records one LP may hold for lp_contents
   // Two-tier mode, several windows: block b follows the near horizon of ws_array[win[b]].
   void set_block_windows(const uint32_t* win, const WindowState* ws_array) {
     v_.block_win = win;
@@ -50,8 +59,11 @@ class LaqQueue final : public QueueBase {
   }
 
  private:
+  gpq_status init_calendar();
+  gpq_status calendar_gather(uint32_t lp, std::vector<Event>* out);
   LaqView v_{};
   uint64_t* sum_ = nullptr;
+  Event* gather_ = nullptr;    // calendar mode: one LP's records for lp_contents / length (+ a count word)
   float far_step_ = 0.0f;
   uint32_t group_ = 16;   // lanes per LP in the post kernel
 };

@@ -56,16 +56,4 @@ gpq_status select_safe(const Event* const* d_s, const uint32_t* d_n, uint32_t ca
 // Copies records: dst[i] = src[i] for i < *d_n (bounded by cap).
 gpq_status copy_records(const Event* src, Event* dst, const uint32_t* d_n, uint32_t cap, hipStream_t stream);
 
-// Reserves one slot of a shared append counter per calling lane with one atomic per wave
-// (every active lane of the wave that reaches the call takes part).
-__device__ __forceinline__ uint32_t wave_append_slot(uint32_t* counter) {
-  const uint64_t mask = __ballot(1);
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
-  uint32_t base = 0;
-  if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = (uint32_t)__shfl((int)base, leader, kWave);
-  return base + rank;
-}
-
 }  // namespace gpq

@@ -13,7 +13,8 @@
  * reference entry point maps to one function below (cited per function).
  * The device-side verbs of src/queue.cuh:46-70 (insert / peek / set_done /
  * is_empty / length / clear / insert_or_update) are the header-only device API
- * in gpupq_amd/csrc/queue_*.cuh, compiled into the model kernels.
+ * of include/gpupq_device.h, compiled into the caller's model kernels; a queue's
+ * view for them comes from gpq_queue_device_view.
  */
 #ifndef GPUPQ_H
 #define GPUPQ_H
@@ -78,6 +79,14 @@ typedef struct {
                                 (src/heap_queue.cu:1811); node_size and heap_nodes apply to each heap, and
                                 each heap can take num_lps * stage_per_lp inserts per step; the window is
                                 global (the minimum over all heaps) */
+  uint32_t calendar_windows;  /* LAQ calendar mode (0 = off; a power of two in [8, 256]; needs far_per_lp):
+                                for the PHOLD merge-and-commit step (drain 2) only, whose windows end one
+                                lookahead apart: each LP keeps an unsorted bucket of capacity_per_lp
+                                records (<= 64) for each of the next calendar_windows windows, and a step
+                                reads only the bucket of the window it commits.  Events beyond the
+                                calendar, at or past the horizon, or for a full bucket pass through a
+                                per-LP inbox of stage_per_lp records into the far store.  Same
+                                committed trace. */
 } gpq_queue_config;
 
 typedef struct gpq_queue gpq_queue;
@@ -144,6 +153,19 @@ gpq_status gpq_queue_lp_contents(gpq_queue* q, uint32_t lp, gpq_event* out, uint
  * (the reference's tests read heap_queue_data.fel per node, src/test_hq.cu:580-917).  With
  * several heaps, node h * nodes_per_heap + v is node v of heap h. */
 gpq_status gpq_heap_node_contents(gpq_queue* q, uint32_t node, gpq_event* out, uint32_t cap, uint32_t* n);
+/* The device operator API for model kernels compiled outside the library (src/queue.cuh:46-70:
+ * queue_insert / queue_peek / queue_set_done / queue_is_empty / queue_length / queue_clear /
+ * queue_insert_or_update, which the reference links into its model kernels with nvcc -rdc,
+ * src/Makefile:21).  Copies q's device view -- the struct of its kind declared in
+ * include/gpupq_device.h (gpq::LaqView, LhqView, LsqView, GaqView or HqView), whose device
+ * methods are those verbs -- to *view; view_bytes must equal the size of that struct.  *kind (may
+ * be NULL) receives the kind.  The caller passes the struct by value to its kernels and launches
+ * them on the queue's stream between gpq_queue_pre and gpq_queue_post, as des_run orders
+ * queue_pre, model_handle_next and queue_post (src/des.cu:154-214).  GPQ_ERR_STATE for a LAQ bound
+ * to a PHOLD model in merge-and-commit mode (drain 2), whose staging buffers alternate per step. */
+gpq_status gpq_queue_device_view(gpq_queue* q, gpq_queue_kind* kind, void* view, uint64_t view_bytes);
+/* The HIP stream (hipStream_t) on which the queue orders its device work. */
+gpq_status gpq_queue_stream(gpq_queue* q, void** stream);
 /* queue_length(lp) / queue_is_empty(lp) / queue_clear(lp) (src/queue.cuh:58-66). */
 gpq_status gpq_queue_length(gpq_queue* q, uint32_t lp, uint32_t* n);
 gpq_status gpq_queue_clear_lp(gpq_queue* q, uint32_t lp);

@@ -1,6 +1,8 @@
-// gpq_device.h -- device-side vocabulary shared by every queue and model kernel:
-// the event record and its total order, the counter-based random stream, the
-// Exp(1) sampler, the trace hash and the wave64 helpers.  gfx950 only.
+// gpupq_device/core.cuh -- device-side vocabulary shared by every queue view and by model
+// kernels written against them: the event record and its total order, the counter-based
+// random stream and Exp(1) sampler of the PHOLD workloads, the trace hash, the wave64
+// helpers, the window state a queue publishes, and the staging insert of the per-LP queues.
+// Header-only HIP for gfx950.
 //
 // Reference counterparts: src/queue_item.h:11-32 (event, operator<),
 // src/phold.cu:29-34 (rand_exponential), src/util.cuh:13-16 (get_lp).
@@ -164,25 +166,16 @@ __device__ __forceinline__ bool stage_insert(Event* stage, uint32_t* ins, uint32
   return true;
 }
 
-// Block-wide min of one value per thread, written to partial[blockIdx.x].  No fence:
-// a device-scope fence on gfx950 writes back the XCD's L2 (buffer_wbl2), far too
-// costly per block; the partials are folded by window_publish_kernel, launched
-// after the producing kernel on the same stream.  blockDim.x: multiple of 64, <= 1024.
-__device__ __forceinline__ void block_min_to_partial(uint32_t v, uint32_t* partial) {
-  __shared__ uint32_t s_wmin[16];
-  v = wave_min_u32(v);
-  if (lane_id() == 0) s_wmin[threadIdx.x / kWave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t m = s_wmin[0];
-    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) m = s_wmin[i] < m ? s_wmin[i] : m;
-    partial[blockIdx.x] = m;
-  }
+// Reserves one slot of a shared append counter per calling lane with one atomic per wave
+// (every active lane of the wave that reaches the call takes part).
+__device__ __forceinline__ uint32_t wave_append_slot(uint32_t* counter) {
+  const uint64_t mask = __ballot(1);
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = (uint32_t)__shfl((int)base, leader, kWave);
+  return base + rank;
 }
-
-// One block: folds the per-block minima and publishes gmin, the window end
-// min(gmin + lookahead, horizon) and the done flag.
-__global__ void window_publish_kernel(const uint32_t* partial, uint32_t count, WindowState* ws, float lookahead,
-                                      float horizon);
 
 }  // namespace gpq

@@ -9,8 +9,7 @@
 // selected records are compacted out and the sorted inserts are merged in (one pass).
 #pragma once
 
-#include "gpq_device.h"
-#include "sortprim.h"
+#include "core.cuh"
 
 namespace gpq {
 

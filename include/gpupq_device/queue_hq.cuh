@@ -8,8 +8,7 @@
 // appended to enq; queue_post leaves the next step's safe events in deq (peek slot = position).
 #pragma once
 
-#include "gpq_device.h"
-#include "sortprim.h"
+#include "core.cuh"
 
 namespace gpq {
 

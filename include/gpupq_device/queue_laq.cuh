@@ -16,9 +16,20 @@
 // (src/local_array_queue.cu:486-513, src/sort.cu:329-356).
 #pragma once
 
-#include "gpq_device.h"
+#include "core.cuh"
 
 namespace gpq {
+
+constexpr uint32_t kCalMaxWindows = 256;
+
+// Calendar mode: the window index `cur` of the step that commits next, and the window ends
+// w[(cur + o) mod B] of windows cur .. cur + B - 1 (float bits; every window after the one
+// that reaches the end time ends at the end time).
+struct CalState {
+  uint32_t cur;
+  uint32_t pad[3];
+  uint32_t w[kCalMaxWindows];
+};
 
 struct LaqView {
   Event* ring;
@@ -46,6 +57,22 @@ struct LaqView {
   uint32_t* upd_n;
   uint32_t upd_cap;
   uint32_t update_ids;
+  // Calendar mode (cal_windows > 0; the PHOLD merge-and-commit step only): one append log per
+  // upcoming window, wlog[(cur + o) mod cal_windows] for window cur + o, count wlog_n[.]; events
+  // beyond the calendar in over[p] (p = parity of the step that appended them), count over_n[p];
+  // events at or past the end time in parked, count *parked_n.  A step groups the log of its
+  // window by LP into the staging area (stage, ins) and commits from there.
+  Event* wlog;
+  uint32_t* wlog_n;
+  Event* over;
+  uint32_t* over_n;
+  Event* parked;
+  uint32_t* parked_n;
+  CalState* cal_st;
+  uint32_t cal_windows;
+  uint32_t wlog_cap;
+  uint32_t over_cap;
+  uint32_t parked_cap;
   Event* far;                   // two-tier mode only (far_cap > 0)
   uint32_t* nfar;
   uint32_t* far_min;
@@ -87,7 +114,9 @@ struct LaqView {
   __device__ __forceinline__ bool is_empty(int lp) const {
     return len[lp] == 0 && ins[lp] == 0 && (far_cap == 0 || nfar[lp] == 0);
   }
-  __device__ __forceinline__ int length(int lp) const { return (int)len[lp]; }
+  // Pending events of LP lp: the sorted ring plus, in two-tier mode, the far store (staged
+  // inserts of the current step are not counted until queue_post folds them in).
+  __device__ __forceinline__ int length(int lp) const { return (int)(len[lp] + (far_cap ? nfar[lp] : 0u)); }
   __device__ __forceinline__ void clear(int lp) const {
     const uint32_t mask = (1u << cap_log2) - 1u;
     if (in_queue) {                       // every id of the LP leaves the queue

@@ -9,7 +9,7 @@
 // head[lp] caches the root's ts bits for the coalesced window scan.
 #pragma once
 
-#include "gpq_device.h"
+#include "core.cuh"
 
 namespace gpq {
 

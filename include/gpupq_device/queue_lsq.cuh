@@ -10,7 +10,7 @@
 // free slots form a list threaded through `left`, head in freelist[lp].
 #pragma once
 
-#include "gpq_device.h"
+#include "core.cuh"
 
 namespace gpq {
 

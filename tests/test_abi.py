@@ -113,3 +113,19 @@ def test_serve_trace_on_host():
     assert min(ph) >= 1 << 12 and max(ph) <= 1 << 18
     zero = gpq.serve_trace(10, seed=5, rate=0.0)
     assert all(r["arrival_ms"] == 0.0 for r in zero)
+
+
+def test_external_model_built_against_public_device_header():
+    # tests/ext/phold_ext.hip is compiled against include/gpupq_device.h alone; it loads without a
+    # GPU and knows the view struct of every queue kind
+    path = os.path.join(ROOT, "tests", "ext", "libphold_ext.so")
+    lib = ctypes.CDLL(path)
+    lib.ext_view_bytes.restype = ctypes.c_uint64
+    lib.ext_view_bytes.argtypes = [ctypes.c_int]
+    sizes = [lib.ext_view_bytes(k) for k in range(5)]
+    assert all(s >= 32 for s in sizes) and lib.ext_view_bytes(9) == 0
+    for name in ("ext_phold_create", "ext_phold_populate", "ext_phold_handle_next", "ext_phold_state", "ext_lp_verbs"):
+        assert hasattr(lib, name)
+    from gpupq_amd import _lib
+    libq = ctypes.CDLL(_lib.LIB_PATH)
+    assert hasattr(libq, "gpq_queue_device_view") and hasattr(libq, "gpq_queue_stream")
