@@ -54,8 +54,8 @@ QUEUES = {
 DRAIN = False
 FAR = 0            # LAQ two-tier mode: far-store records per LP (0 = one sorted ring per LP)
 FAR_STEP = 0.0     # LAQ two-tier: near-horizon step (0 = 4 x lookahead)
-CAL = 0            # LAQ calendar mode: buckets (windows) per LP (0 = off)
-CAL_SLOTS = 16     # LAQ calendar mode: records per bucket
+CAL = 0            # LAQ calendar mode: windows in the calendar (0 = off)
+CAL_SLOTS = 8      # LAQ calendar mode: a window's log records per LP
 
 
 def make(gpq, kind, wl, stats=False, checks=False):
@@ -243,15 +243,18 @@ def astar_bench(gpq, log, plain_n=ASTAR_QUERIES, cpu=True):
 def per_queue_bench(gpq, wl, log, drive="graph"):
     """The same workload through every queue kind behind the one API (one timed run each);
     drive "plain" launches every step eagerly (a host sync per step), for kernel tracing."""
-    global DRAIN
+    global DRAIN, CAL
     out = {}
-    saved = DRAIN
+    saved, saved_cal = DRAIN, CAL
     runs = [("laq", False), ("lhq", True), ("lhq", False), ("lsq", True), ("gaq", False), ("hq", False),
             ("hq4", False)]
     if saved == "fused":
         runs.insert(0, ("laq", True))             # the two-tier LAQ with a separate post kernel
+        if saved_cal:
+            runs.insert(0, ("laq", "fused"))      # the two-tier sorted ring with the merge-and-commit step
     for kind, drain in runs:
         DRAIN = drain
+        CAL = saved_cal if drain == "fused" and not saved_cal else 0
         q, m = make(gpq, kind, wl)
         q.sync()
         t0 = time.perf_counter()
@@ -260,11 +263,11 @@ def per_queue_bench(gpq, wl, log, drive="graph"):
         h = m.trace_hash()
         m.model_finish()
         q.queue_finish()
-        key = kind + ("_drain" if drain else "_one_event")
+        key = kind + ("_ring_fused" if drain == "fused" else "_drain" if drain else "_one_event")
         out[key] = {"events_per_s": rep["events"] / dt, "seconds": dt, "windows": rep["windows"],
                     "trace_hash": "%016x" % h}
         log("per_queue %s: %.3e events/s, %d windows, hash %016x" % (key, rep["events"] / dt, rep["windows"], h))
-    DRAIN = saved
+    DRAIN, CAL = saved, saved_cal
     return out
 
 
@@ -275,10 +278,13 @@ def phold_autotune_bench(gpq, wl, log, sample_end_time=5.0):
     kw, drain, cands = gpq.phold_autotune(wl["num_lps"], init_per_lp=wl["init_per_lp"], seed=42,
                                           end_time=wl["end_time"], sample_end_time=sample_end_time)
     dt = time.perf_counter() - t0
-    bench_layout = (64, 32, FAR, DRAIN) if FAR else (QUEUES["laq"]["capacity_per_lp"], 32, 0, DRAIN)
-    pick = (kw["capacity_per_lp"], kw["stage_per_lp"], kw["far_per_lp"], drain)
+    if CAL and DRAIN == "fused":
+        bench_layout = (CAL_SLOTS, 32, FAR or 128, DRAIN, CAL)
+    else:
+        bench_layout = ((64, 32, FAR, DRAIN) if FAR else (QUEUES["laq"]["capacity_per_lp"], 32, 0, DRAIN)) + (0,)
+    pick = (kw["capacity_per_lp"], kw["stage_per_lp"], kw["far_per_lp"], drain, kw["calendar_windows"])
     out = {"sample_end_time": sample_end_time, "seconds": dt, "pick": dict(kw, drain=drain),
-           "benchmarked_layout_is_pick": bench_layout == pick and (kw["far_step"] == FAR_STEP or not FAR),
+           "benchmarked_layout_is_pick": bench_layout == pick and (kw["far_step"] == FAR_STEP or not FAR or CAL),
            "candidates": cands}
     log("phold_autotune: %s" % json.dumps(out))
     return out
@@ -443,9 +449,11 @@ def main():
     ap.add_argument("--far", type=int, default=128,
                     help="LAQ two-tier mode: far-store records per LP (0 = single sorted ring)")
     ap.add_argument("--far-step", type=float, default=0.0, help="LAQ two-tier: near-horizon step (0 = 4 x lookahead)")
-    ap.add_argument("--calendar", type=int, default=0,
-                    help="LAQ calendar mode for the merge-and-commit step: buckets (windows) per LP (0 = off)")
-    ap.add_argument("--cal-slots", type=int, default=16, help="LAQ calendar mode: records per bucket")
+    ap.add_argument("--calendar", type=int, default=64,
+                    help="LAQ calendar mode for the merge-and-commit step: windows in the calendar (0 = off: the "
+                         "two-tier sorted ring)")
+    ap.add_argument("--cal-slots", type=int, default=8,
+                    help="LAQ calendar mode: a window's log records per LP (capacity_per_lp)")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-astar", action="store_true")
@@ -547,7 +555,7 @@ def main():
         if args.queue == "laq" and dom == "queue_post":
             kname = "laq_post_far_kernel" if FAR else "laq_post_kernel"
         if dom == "merge_commit_step":
-            kname = "phold_fused_kernel"
+            kname = "phold_calendar_kernel" if CAL else "phold_fused_kernel"
         out = {"bound": "hbm", "kernel": dom, "kernel_symbol": kname, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS,
                "unit": "GB/s", "frac": k["gbps"] / HBM_PEAK_GBPS,
                "traffic": traffic_from_profiles(kname, args.workload),

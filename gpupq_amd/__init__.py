@@ -172,10 +172,10 @@ def phold_autotune(num_lps, init_per_lp=32, seed=42, end_time=50.0, lookahead=0.
     check(lib.gpq_phold_autotune(ctypes.byref(pc), ctypes.byref(qc), sample_end_time, res, 16, ctypes.byref(n)),
           "phold_autotune")
     kw = dict(capacity_per_lp=qc.capacity_per_lp, stage_per_lp=qc.stage_per_lp, far_per_lp=qc.far_per_lp,
-              far_step=qc.far_step)
+              far_step=qc.far_step, calendar_windows=qc.calendar_windows)
     drain = {0: False, 1: True, 2: "fused"}[pc.drain]
     cands = [dict(capacity_per_lp=r.capacity_per_lp, stage_per_lp=r.stage_per_lp, far_per_lp=r.far_per_lp,
-                  far_step=r.far_step, drain={0: False, 1: True, 2: "fused"}[r.drain],
+                  far_step=r.far_step, calendar_windows=r.calendar_windows, drain={0: False, 1: True, 2: "fused"}[r.drain],
                   status={0: "ran", 1: "does not fit", 2: "trace differs"}[r.status],
                   events_per_s=r.events_per_second) for r in res[:n.value]]
     return kw, drain, cands

@@ -36,7 +36,8 @@ class PholdConfig(ctypes.Structure):
 
 class PholdTuning(ctypes.Structure):
     _fields_ = [("capacity_per_lp", c_u32), ("stage_per_lp", c_u32), ("far_per_lp", c_u32), ("far_step", c_f32),
-                ("drain", c_u32), ("status", c_u32), ("events_per_second", ctypes.c_double)]
+                ("drain", c_u32), ("status", c_u32), ("events_per_second", ctypes.c_double),
+                ("calendar_windows", c_u32)]
 
 
 class DesReport(ctypes.Structure):

@@ -1,6 +1,6 @@
-// phold_tune.hip -- the PHOLD layout autotuner: ring capacity per LP, staging slots, far
-// store, near-horizon step and step kind, each candidate timed on a prefix of the run itself
-// (same LPs, same seed, end time cut short) through the LAQ.
+// phold_tune.hip -- the PHOLD layout autotuner: calendar or sorted ring, capacity per LP,
+// staging slots, far store, near-horizon step and step kind, each candidate timed on a prefix of
+// the run itself (same LPs, same seed, end time cut short) through the LAQ.
 #include <memory>
 
 #include "phold.h"
@@ -12,13 +12,16 @@ struct Candidate {
   uint32_t cap, stage, far;
   float far_step_la;   // near-horizon step in lookaheads (0 = the queue's default, 4)
   uint32_t drain;
+  uint32_t cal;        // calendar mode: windows in the calendar (0 = off); cap = a window's log
+                       // records per LP, far = beyond-the-calendar / parked records per LP
 };
 
-// Two-tier layouts with the merge-and-commit step first, then the post kernel + drain step,
-// then the one-tier ring.
+// The calendar layouts with the merge-and-commit step first, then the two-tier layouts with the
+// same step, then the post kernel + drain step, then the one-tier ring.
 constexpr Candidate kCandidates[] = {
-    {64, 32, 128, 0.0f, 2}, {64, 32, 128, 6.0f, 2}, {64, 32, 256, 0.0f, 2},
-    {64, 16, 128, 0.0f, 2}, {64, 32, 128, 0.0f, 1}, {128, 32, 0, 0.0f, 1},
+    {8, 32, 128, 0.0f, 2, 64}, {8, 32, 128, 0.0f, 2, 32},
+    {64, 32, 128, 0.0f, 2, 0}, {64, 32, 128, 6.0f, 2, 0}, {64, 32, 256, 0.0f, 2, 0},
+    {64, 16, 128, 0.0f, 2, 0}, {64, 32, 128, 0.0f, 1, 0}, {128, 32, 0, 0.0f, 1, 0},
 };
 
 constexpr double kMargin = 1.02;
@@ -60,13 +63,14 @@ extern "C" gpq_status gpq_phold_autotune(gpq_phold_config* model, gpq_queue_conf
     double rate = 0.0;
     uint64_t hash = 0;
     uint32_t status = 1;
-    if (model->init_per_lp <= c.cap) {
+    if (c.cal ? model->init_per_lp <= c.far : model->init_per_lp <= c.cap) {
       gpq_queue_config qc = *queue;
       qc.kind = GPQ_LAQ;
       qc.capacity_per_lp = c.cap;
       qc.stage_per_lp = c.stage;
       qc.far_per_lp = c.far;
       qc.far_step = c.far_step_la * queue->lookahead;
+      qc.calendar_windows = c.cal;
       qc.horizon = t;
       qc.collect_stats = 0;
       qc.update_ids = 0;           // PHOLD records carry no ids: no decrease-key mode
@@ -88,7 +92,8 @@ extern "C" gpq_status gpq_phold_autotune(gpq_phold_config* model, gpq_queue_conf
       if (status) rate = 0.0;
     }
     if (results && n < max_results)
-      results[n] = gpq_phold_tuning{c.cap, c.stage, c.far, c.far_step_la * queue->lookahead, c.drain, status, rate};
+      results[n] = gpq_phold_tuning{c.cap, c.stage, c.far, c.far_step_la * queue->lookahead, c.drain, status, rate,
+                                    c.cal};
     ++n;
     // candidates are listed in order of preference: a later one must be faster by more than
     // the run-to-run spread of the timing (2%) to replace the pick
@@ -107,6 +112,7 @@ extern "C" gpq_status gpq_phold_autotune(gpq_phold_config* model, gpq_queue_conf
   queue->stage_per_lp = pick->stage;
   queue->far_per_lp = pick->far;
   queue->far_step = pick->far_step_la * queue->lookahead;
+  queue->calendar_windows = pick->cal;
   queue->update_ids = 0;
   queue->update_per_lp = 0;
   model->drain = pick->drain;

@@ -369,7 +369,7 @@ __global__ void laq_update_kernel(LaqView q, uint32_t* partial) {
 
 // ---- calendar mode (laq_calendar.cuh) -------------------------------------------------
 
-// queue_post_init: the staged initial events go to the beyond-the-calendar list of parity 0
+// queue_post_init: the staged initial events go to the list over[1]
 // (the window ends, and so the logs, are known only once every initial event is in); the block
 // minima of their timestamps feed cal_init_publish_kernel.
 __global__ __launch_bounds__(kLaqBlock) void cal_post_init_kernel(LaqView q, uint32_t* partial) {
@@ -381,8 +381,8 @@ __global__ __launch_bounds__(kLaqBlock) void cal_post_init_kernel(LaqView q, uin
     for (uint32_t i = 0; i < k; ++i) {
       const Event e = load_event(&q.stage[(size_t)lp * q.stage_cap + i]);
       m = e.ts < m ? e.ts : m;
-      const uint32_t pos = wave_append_slot(&q.over_n[0]);
-      if (pos < q.over_cap) store_event(&q.over[pos], e);
+      const uint32_t pos = wave_append_slot(&q.over_n[1]);
+      if (pos < q.over_cap) store_event(&q.over[q.over_cap + pos], e);
       else atomicOr(&q.ws->error, kErrQueueOverflow);
     }
     if (k0) q.ins[lp] = 0;
@@ -412,15 +412,15 @@ __global__ void cal_init_publish_kernel(const uint32_t* partial, uint32_t count,
   }
 }
 
-// The window ends of windows 0 .. B - 1 from the first one (published from the initial
-// events), each one lookahead after the previous in float arithmetic, capped at the end time.
+// The ends of windows 0 .. B from the first one (published from the initial events), each one
+// lookahead after the previous in float arithmetic, capped at the end time.
 __global__ void cal_start_kernel(LaqView q, float lookahead, float end_time) {
   CalState* st = q.cal_st;
   const uint32_t B = q.cal_windows;
   st->cur = 0;
   uint32_t w = q.ws->wend;
   st->w[0] = w;
-  for (uint32_t i = 1; i < B; ++i) {
+  for (uint32_t i = 1; i <= B; ++i) {
     float nw = bits_to_f(w) + lookahead;
     if (end_time < nw) nw = end_time;
     w = f_to_bits(nw);
@@ -429,68 +429,87 @@ __global__ void cal_start_kernel(LaqView q, float lookahead, float end_time) {
   q.ws->sel = 0;
 }
 
-// Files every initial event (the list of parity 0) in the log of its window (windows 0 .. B - 1),
-// in the beyond-the-calendar list of parity 1 (read by the first step), or with the parked ones;
-// appends are aggregated per block and route.
+// Files every initial event (the list over[1]) in the log of its window (windows 0 .. B - 1),
+// in over[0] (read when window 0 is staged), or with the parked ones; appends are aggregated per
+// block and route.
 __global__ __launch_bounds__(cal::kBlock) void cal_fill_kernel(LaqView q, float end_time) {
   __shared__ uint32_t sw[kCalMaxWindows];
   __shared__ cal::BlockAppend ba;
-  const uint32_t cur = cal::load_windows(q, sw);
+  cal::load_windows(q, 0u, sw);
   __syncthreads();
   const uint32_t B = q.cal_windows, end_bits = f_to_bits(end_time);
-  const uint32_t n0 = q.over_n[0], n = n0 < q.over_cap ? n0 : q.over_cap;
+  const uint32_t n1 = q.over_n[1], n = n1 < q.over_cap ? n1 : q.over_cap;
+  const Event* src = q.over + q.over_cap;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {   // block-uniform trip count
     const uint32_t i = base + threadIdx.x;
     const bool has = i < n;
-    const Event e = has ? load_event(&q.over[i]) : Event{};
+    const Event e = has ? load_event(&src[i]) : Event{};
     const uint32_t r = has ? cal::route_of(sw, B, e.ts, end_bits, 0u) : cal::Route::kNone;
-    cal::block_append(q, cur, 1u, r, e, ba);    // parity 1: the list the first step reads
+    cal::block_append(q, 0u, 0u, r, e, ba);
   }
 }
 
-__global__ void cal_fill_done_kernel(LaqView q) { q.over_n[0] = 0; }
+__global__ void cal_fill_done_kernel(LaqView q) { q.over_n[1] = 0; }
 
-// Step part 1: the log of the window to commit, grouped by LP into the staging area.
+// Stages window 0 (before the first step; later windows are staged by the step before theirs).
 __global__ __launch_bounds__(cal::kBlock) void cal_group_kernel(LaqView q, float end_time) {
-  cal::group_block(q, end_time);
+  cal::group_block(q, end_time, 0u, blockIdx.x, gridDim.x);
 }
 
-// queue_post after a calendar step: the window advances by the lookahead (capped at the end
-// time) and the calendar by one window, whose end is appended to the ring; the committed
-// window's log and the beyond-the-calendar list the step read are emptied; the window that
-// reached the end time ends the run.
+// After window 0 was staged: its log and the list it read are empty.
+__global__ void cal_group_done_kernel(LaqView q) {
+  q.wlog_n[0] = 0;
+  q.over_n[0] = 0;
+}
+
+// queue_post after a calendar step (window c committed, window c + 1 staged): the log of window
+// c + 1 and the list the staging read are emptied; the window advances by the lookahead (capped
+// at the end time) and the end of window c + 1 + B joins the ring; the window that reached the
+// end time ends the run.
 __global__ void cal_publish_kernel(LaqView q, float lookahead, float end_time) {
   WindowState w = *q.ws;
   if (w.done) return;
   CalState* st = q.cal_st;
-  const uint32_t B = q.cal_windows;
-  const uint32_t cur = st->cur;
-  q.wlog_n[cur & (B - 1u)] = 0;
-  q.over_n[(cur & 1u) ^ 1u] = 0;
+  const uint32_t B = q.cal_windows, M = 2u * B - 1u;
+  const uint32_t c = st->cur;
+  q.wlog_n[(c + 1u) & (B - 1u)] = 0;
+  q.over_n[(c + 1u) & 1u] = 0;
   w.windows += 1;
   w.gmin = w.wend;                                 // a lower bound of the pending minimum
   if (!(bits_to_f(w.wend) < end_time)) {
     w.done = 1;
   } else {
-    const uint32_t nc = cur + 1u;
+    const uint32_t nc = c + 1u;
     st->cur = nc;
-    float nw = bits_to_f(st->w[(nc + B - 2u) & (B - 1u)]) + lookahead;
+    float nw = bits_to_f(st->w[(nc + B - 1u) & M]) + lookahead;
     if (end_time < nw) nw = end_time;
-    st->w[(nc + B - 1u) & (B - 1u)] = f_to_bits(nw);
-    w.wend = st->w[nc & (B - 1u)];
+    st->w[(nc + B) & M] = f_to_bits(nw);
+    w.wend = st->w[nc & M];
   }
   *q.ws = w;
 }
 
-// Pending records: every window log, both beyond-the-calendar lists, the parked ones and the
-// staging area (one thread; the counters are few).
+// Pending records: every window log, both beyond-the-calendar lists and the parked ones (by
+// the first thread), and both staging areas (all threads, one LP each).
 __global__ void cal_count_kernel(LaqView q, unsigned long long* sum) {
+  __shared__ unsigned long long s[16];
+  const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long n = 0;
-  for (uint32_t b = 0; b < q.cal_windows; ++b) n += min(q.wlog_n[b], q.wlog_cap);
-  for (uint32_t p = 0; p < 2; ++p) n += min(q.over_n[p], q.over_cap);
-  n += min(*q.parked_n, q.parked_cap);
-  *sum = n;
+  if (lp < q.num_lps) n = min(q.ins[lp], q.stage_cap) + min(q.ins_alt[lp], q.stage_cap);
+  if (lp == 0) {
+    for (uint32_t b = 0; b < q.cal_windows; ++b) n += min(q.wlog_n[b], q.wlog_cap);
+    for (uint32_t p = 0; p < 2; ++p) n += min(q.over_n[p], q.over_cap);
+    n += min(*q.parked_n, q.parked_cap);
+  }
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, kWave);
+  if (lane_id() == 0) s[threadIdx.x / kWave] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s[w];
+    if (t) atomicAdd(sum, t);
+  }
 }
 
 // Every pending record owned by LP lp (unsorted) into out[0 .. *n) (at most cap), grid-stride
@@ -515,9 +534,10 @@ __global__ void cal_gather_kernel(LaqView q, uint32_t lp, Event* out, uint32_t c
   }
   const uint32_t c = min(*q.parked_n, q.parked_cap);
   for (uint32_t i = t; i < c; i += stride) gather_one(&q.parked[i], lp, out, cap, n);
-  if (t == 0) {
-    const uint32_t k = min(q.ins[lp], q.stage_cap);
-    for (uint32_t i = 0; i < k; ++i) gather_one(&q.stage[(size_t)lp * q.stage_cap + i], lp, out, cap, n);
+  if (t == 0) {                         // both staging areas (slot-major between steps)
+    const uint32_t k0 = min(q.ins[lp], q.stage_cap), k1 = min(q.ins_alt[lp], q.stage_cap);
+    for (uint32_t i = 0; i < k0; ++i) gather_one(&q.stage[(size_t)i * q.num_lps + lp], lp, out, cap, n);
+    for (uint32_t i = 0; i < k1; ++i) gather_one(&q.stage_alt[(size_t)i * q.num_lps + lp], lp, out, cap, n);
   }
 }
 
@@ -614,7 +634,10 @@ gpq_status LaqQueue::init_calendar() {
   GPQ_TRY(arena.alloc(&v_.parked, (size_t)v_.parked_cap));
   GPQ_TRY(arena.alloc(&v_.parked_n, 1));
   GPQ_TRY(arena.alloc(&v_.cal_st, 1));
+  GPQ_TRY(arena.alloc(&v_.stage_alt, (size_t)L * v_.stage_cap));
+  GPQ_TRY(arena.alloc(&v_.ins_alt, L));
   GPQ_TRY(arena.alloc(&gather_, kGatherCap + 1));
+  GPQ_HIP(hipMemsetAsync(v_.ins_alt, 0, sizeof(uint32_t) * L, stream));
   GPQ_HIP(hipMemsetAsync(v_.wlog_n, 0, sizeof(uint32_t) * (B + 2), stream));
   GPQ_HIP(hipMemsetAsync(v_.parked_n, 0, sizeof(uint32_t), stream));
   GPQ_HIP(hipMemsetAsync(v_.cal_st, 0, sizeof(CalState), stream));
@@ -632,12 +655,8 @@ gpq_status LaqQueue::calendar_start() {
   hipLaunchKernelGGL(cal_start_kernel, dim3(1), dim3(1), 0, stream, v_, cfg.lookahead, cfg.horizon);
   hipLaunchKernelGGL(cal_fill_kernel, dim3(cal::kGroupBlocks), dim3(cal::kBlock), 0, stream, v_, cfg.horizon);
   hipLaunchKernelGGL(cal_fill_done_kernel, dim3(1), dim3(1), 0, stream, v_);
-  GPQ_HIP(hipGetLastError());
-  return GPQ_OK;
-}
-
-gpq_status LaqQueue::calendar_group() {
   hipLaunchKernelGGL(cal_group_kernel, dim3(cal::kGroupBlocks), dim3(cal::kBlock), 0, stream, v_, cfg.horizon);
+  hipLaunchKernelGGL(cal_group_done_kernel, dim3(1), dim3(1), 0, stream, v_);
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }
@@ -732,7 +751,9 @@ gpq_status LaqQueue::enable_alt_stage() {
 
 gpq_status LaqQueue::pending(uint64_t* n) {
   if (v_.wlog) {
-    hipLaunchKernelGGL(cal_count_kernel, dim3(1), dim3(1), 0, stream, v_, reinterpret_cast<unsigned long long*>(sum_));
+    GPQ_HIP(hipMemsetAsync(sum_, 0, sizeof(uint64_t), stream));
+    hipLaunchKernelGGL(cal_count_kernel, dim3(ceil_div(cfg.num_lps, 256)), dim3(256), 0, stream, v_,
+                       reinterpret_cast<unsigned long long*>(sum_));
     GPQ_HIP(hipGetLastError());
     GPQ_HIP(hipMemcpyAsync(n, sum_, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
     GPQ_HIP(hipStreamSynchronize(stream));

@@ -48,9 +48,6 @@ class LaqQueue final : public QueueBase {
   // ends of the first windows and the filing of the initial events in their buckets.
   bool calendar() const { return v_.wlog != nullptr; }
   gpq_status calendar_start();
-  // Calendar mode, step part 1: the log of the window the next step commits, grouped by LP
-  // into the staging area (part 2 is the model's commit kernel).
-  gpq_status calendar_group();
   static constexpr uint32_t kGatherCap = 1u << 16;   // records one LP may hold for lp_contents
   // Two-tier mode, several windows: block b follows the near horizon of ws_array[win[b]].
   void set_block_windows(const uint32_t* win, const WindowState* ws_array) {

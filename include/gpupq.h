@@ -202,6 +202,8 @@ typedef struct {
   uint32_t status;              /* 0 ran; 1 could not hold the workload; 2 its trace hash differed
                                    from the first layout's (never picked) */
   double events_per_second;
+  uint32_t calendar_windows;    /* LAQ calendar mode (0 = off); capacity_per_lp is then a window's
+                                   log records per LP */
 } gpq_phold_tuning;
 
 /* Autotuner (no reference counterpart; the reference fixes FEL_SIZE at compile time,
@@ -209,7 +211,8 @@ typedef struct {
  * staging slots, far store, near-horizon step and step kind -- on the model's own workload cut
  * to end time sample_end_time (0 = 5, at most model->end_time), each in a fresh queue driven
  * by GPQ_DRIVE_GRAPH, and writes the fastest (a later candidate must beat the pick so far by
- * more than 2%) into *queue (kind = GPQ_LAQ, capacity_per_lp, stage_per_lp, far_per_lp, far_step;
+ * more than 2%) into *queue (kind = GPQ_LAQ, capacity_per_lp, stage_per_lp, far_per_lp, far_step,
+ * calendar_windows;
  * update_ids and update_per_lp cleared; the other fields are kept) and model->drain.  results
  * (may be NULL) receives up to max_results candidates in the order timed, *n_results (may be
  * NULL) their count.  GPQ_ERR_CAPACITY when no layout holds the workload. */

@@ -22,13 +22,13 @@ namespace gpq {
 
 constexpr uint32_t kCalMaxWindows = 256;
 
-// Calendar mode: the window index `cur` of the step that commits next, and the window ends
-// w[(cur + o) mod B] of windows cur .. cur + B - 1 (float bits; every window after the one
-// that reaches the end time ends at the end time).
+// Calendar mode: the window index `cur` of the step that commits next, and the ends of the
+// windows x in [cur, cur + B] at w[x mod 2B] (float bits; every window after the one that
+// reaches the end time ends at the end time).
 struct CalState {
   uint32_t cur;
   uint32_t pad[3];
-  uint32_t w[kCalMaxWindows];
+  uint32_t w[2 * kCalMaxWindows];
 };
 
 struct LaqView {
@@ -61,7 +61,8 @@ struct LaqView {
   // upcoming window, wlog[(cur + o) mod cal_windows] for window cur + o, count wlog_n[.]; events
   // beyond the calendar in over[p] (p = parity of the step that appended them), count over_n[p];
   // events at or past the end time in parked, count *parked_n.  A step groups the log of its
-  // window by LP into the staging area (stage, ins) and commits from there.
+  // window by LP into the staging area (stage / ins for even windows, stage_alt / ins_alt for
+  // odd ones, both slot-major) and commits from there.
   Event* wlog;
   uint32_t* wlog_n;
   Event* over;
