@@ -167,6 +167,7 @@ SIGNATURES = [
     ("gpq_selftest_philox", ctypes.c_int, [c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_void_p]),
     ("gpq_queue_device_view", ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_int), c_void_p, c_u64]),
     ("gpq_queue_stream", ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
+    ("gpq_sort_events_noalloc", ctypes.c_int, [c_void_p, c_u32, c_void_p, c_void_p]),
 ]
 
 _lib = None

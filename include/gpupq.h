@@ -461,6 +461,17 @@ gpq_status gpq_selftest_select(const gpq_event* sorted, uint32_t n, uint32_t num
                                const gpq_event* bound, gpq_event* deq, uint32_t* ndeq, gpq_event* rest,
                                uint32_t* nrest, uint32_t* prefix);
 
+/* ---- allocation-free record sort ---------------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): sorts n device
+ * records d_in by (ts, src, seq) into d_out (device; d_out == d_in sorts in place) on `stream`
+ * (a hipStream_t; NULL = the default stream), with the kernels of gpq_selftest_sort.  Its
+ * partner buffer and count word live in a library-owned device array of 16 MiB + 256 B that
+ * every GPU it runs on holds until the process ends; it calls no hipMalloc or hipFree.
+ * n <= 2^20 (GPQ_ERR_CAPACITY above).  Calls that overlap on one GPU share that array and race:
+ * the caller serialises them.  Returns once the work is queued (asynchronous). */
+gpq_status gpq_sort_events_noalloc(const gpq_event* d_in, uint32_t n, gpq_event* d_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
