@@ -71,6 +71,8 @@ def make(gpq, kind, wl, stats=False, checks=False):
         args.update(capacity_per_lp=64, far_per_lp=FAR, far_step=FAR_STEP)
     if kind == "laq" and CAL and DRAIN == "fused":
         args.update(capacity_per_lp=CAL_SLOTS, stage_per_lp=32, far_per_lp=FAR or 128, calendar_windows=CAL)
+    if kind in SORT_TILES:
+        args["sort_tile"] = SORT_TILES[kind]
     q = gpq.Queue(kind, num_lps=wl["num_lps"], lookahead=0.1, horizon=wl["end_time"], block_threads=256,
                   collect_stats=stats, **args)
     m = gpq.Phold(q, init_per_lp=wl["init_per_lp"], seed=42, end_time=wl["end_time"], checks=checks, drain=DRAIN)
@@ -287,6 +289,26 @@ def phold_autotune_bench(gpq, wl, log, sample_end_time=5.0):
            "benchmarked_layout_is_pick": bench_layout == pick and (kw["far_step"] == FAR_STEP or not FAR or CAL),
            "candidates": cands}
     log("phold_autotune: %s" % json.dumps(out))
+    return out
+
+
+SORT_TILES = {}    # sort tile of the global queues picked by the sort autotuner, by kind
+
+
+def sort_autotune_bench(gpq, wl, log, sample_end_time=1.0):
+    """The sort-tile autotuner of the global queues (GAQ, HQ) on the workload's first
+    `sample_end_time` time units; the per-queue runs then use the picks."""
+    out = {}
+    for kind in ("gaq", "hq"):
+        kw = dict(QUEUES[kind])
+        if kind == "hq":
+            kw["node_size"] = max(wl["num_lps"], 32)
+        t0 = time.perf_counter()
+        tile, cands = gpq.autotune_sort(kind, wl["num_lps"], init_per_lp=wl["init_per_lp"], seed=42,
+                                        end_time=wl["end_time"], sample_end_time=sample_end_time, **kw)
+        SORT_TILES[kind] = tile
+        out[kind] = {"pick": tile, "seconds": time.perf_counter() - t0, "candidates": cands}
+    log("sort_autotune: %s" % json.dumps(out))
     return out
 
 
@@ -584,6 +606,8 @@ def main():
     if not args.no_autotune and args.queue == "laq":
         section("phold_autotune", phold_autotune_bench, gpq, wl, log)
     if not args.no_queues:
+        if not args.no_autotune:
+            section("sort_autotune", sort_autotune_bench, gpq, wl, log)
         section("per_queue", per_queue_bench, gpq, wl, log, drive=args.per_queue_drive)
         if "error" not in line["per_queue"]:
             hashes_q = {v["trace_hash"] for v in line["per_queue"].values()}

@@ -47,12 +47,12 @@ class Queue:
     def __init__(self, kind="laq", num_lps=1024, capacity_per_lp=64, stage_per_lp=16, node_size=0,
                  heap_nodes=0, total_capacity=0, lookahead=0.1, horizon=float("inf"), block_threads=256,
                  collect_stats=False, far_per_lp=0, far_step=0.0, update_ids=0, update_per_lp=0, heaps=0,
-                 calendar_windows=0):
+                 calendar_windows=0, sort_tile=0):
         self.kind = kind
         self.num_lps = num_lps
         cfg = QueueConfig(_lib.QUEUE_KINDS[kind], num_lps, capacity_per_lp, stage_per_lp, node_size,
                           heap_nodes, total_capacity, lookahead, horizon, block_threads, 1 if collect_stats else 0,
-                          far_per_lp, far_step, update_ids, update_per_lp, heaps, calendar_windows)
+                          far_per_lp, far_step, update_ids, update_per_lp, heaps, calendar_windows, sort_tile)
         self.cfg = cfg
         h = ctypes.c_void_p()
         check(lib.gpq_queue_create(ctypes.byref(cfg), ctypes.byref(h)), "queue_init")
@@ -179,6 +179,28 @@ def phold_autotune(num_lps, init_per_lp=32, seed=42, end_time=50.0, lookahead=0.
                   status={0: "ran", 1: "does not fit", 2: "trace differs"}[r.status],
                   events_per_s=r.events_per_second) for r in res[:n.value]]
     return kw, drain, cands
+
+
+def autotune_sort(kind, num_lps, init_per_lp=32, seed=42, end_time=50.0, lookahead=0.1, p_self=0.5,
+                  sample_end_time=0.0, **queue_kwargs):
+    """The sort-tile autotuner of the global queues (gpq_queue_autotune_sort) for kind "gaq" or
+    "hq" and the other Queue arguments given: returns (the picked sort_tile, [every candidate
+    with its status and rate])."""
+    cfg_args = dict(capacity_per_lp=64, stage_per_lp=16, node_size=0, heap_nodes=0, total_capacity=0,
+                    block_threads=256, heaps=0)
+    cfg_args.update(queue_kwargs)
+    qc = QueueConfig(_lib.QUEUE_KINDS[kind], num_lps, cfg_args["capacity_per_lp"], cfg_args["stage_per_lp"],
+                     cfg_args["node_size"], cfg_args["heap_nodes"], cfg_args["total_capacity"], lookahead, end_time,
+                     cfg_args["block_threads"], 0, 0, 0.0, 0, 0, cfg_args["heaps"], 0, 0)
+    pc = PholdConfig(num_lps, init_per_lp, seed, lookahead, end_time, min(int(round(p_self * 2 ** 32)), 0xFFFFFFFF),
+                     0, 0)
+    res = (_lib.SortTuning * 8)()
+    n = ctypes.c_uint32()
+    check(lib.gpq_queue_autotune_sort(ctypes.byref(pc), ctypes.byref(qc), sample_end_time, res, 8, ctypes.byref(n)),
+          "queue_autotune_sort")
+    cands = [dict(sort_tile=r.sort_tile, status={0: "ran", 1: "could not run", 2: "trace differs"}[r.status],
+                  events_per_s=r.events_per_second) for r in res[:n.value]]
+    return qc.sort_tile, cands
 
 
 class Phold:

@@ -26,12 +26,17 @@ class QueueConfig(ctypes.Structure):
                 ("stage_per_lp", c_u32), ("node_size", c_u32), ("heap_nodes", c_u32),
                 ("total_capacity", c_u64), ("lookahead", c_f32), ("horizon", c_f32),
                 ("block_threads", c_u32), ("collect_stats", c_u32), ("far_per_lp", c_u32), ("far_step", c_f32),
-                ("update_ids", c_u32), ("update_per_lp", c_u32), ("heaps", c_u32), ("calendar_windows", c_u32)]
+                ("update_ids", c_u32), ("update_per_lp", c_u32), ("heaps", c_u32), ("calendar_windows", c_u32),
+                ("sort_tile", c_u32)]
 
 
 class PholdConfig(ctypes.Structure):
     _fields_ = [("num_lps", c_u32), ("init_per_lp", c_u32), ("seed", c_u32), ("lookahead", c_f32),
                 ("end_time", c_f32), ("p_self_u32", c_u32), ("checks", c_u32), ("drain", c_u32)]
+
+
+class SortTuning(ctypes.Structure):
+    _fields_ = [("sort_tile", c_u32), ("status", c_u32), ("events_per_second", ctypes.c_double)]
 
 
 class PholdTuning(ctypes.Structure):
@@ -165,6 +170,7 @@ SIGNATURES = [
     ("gpq_selftest_select", ctypes.c_int, [c_void_p, c_u32, c_u32, c_f32, c_void_p, c_void_p, ctypes.POINTER(c_u32),
                                            c_void_p, ctypes.POINTER(c_u32), ctypes.POINTER(c_u32)]),
     ("gpq_selftest_philox", ctypes.c_int, [c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_void_p]),
+    ("gpq_queue_autotune_sort", ctypes.c_int, [c_void_p, c_void_p, c_f32, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     ("gpq_queue_device_view", ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_int), c_void_p, c_u64]),
     ("gpq_queue_stream", ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
     ("gpq_sort_events_noalloc", ctypes.c_int, [c_void_p, c_u32, c_void_p, c_void_p]),

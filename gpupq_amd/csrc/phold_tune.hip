@@ -118,3 +118,60 @@ extern "C" gpq_status gpq_phold_autotune(gpq_phold_config* model, gpq_queue_conf
   model->drain = pick->drain;
   return GPQ_OK;
 }
+
+// Sort-tile autotuner of the global queues: the record sort and merge kernels with tiles of
+// 2048 (the default, timed first), 1024 and 4096 records, each on the PHOLD prefix through the
+// given queue kind, the trace cross-checked.
+extern "C" gpq_status gpq_queue_autotune_sort(const gpq_phold_config* model, gpq_queue_config* queue,
+                                              float sample_end_time, gpq_sort_tuning* results, uint32_t max_results,
+                                              uint32_t* n_results) {
+  using namespace gpq;
+  GPQ_CHECK_ARG(model && queue, "sort autotune: null argument");
+  GPQ_CHECK_ARG(queue->kind == GPQ_GAQ || queue->kind == GPQ_HQ, "sort autotune: the queue kind must be GAQ or HQ");
+  GPQ_CHECK_ARG(model->num_lps >= 1 && model->num_lps == queue->num_lps, "sort autotune: num_lps must match");
+  GPQ_CHECK_ARG(model->end_time > 0.0f && queue->lookahead > 0.0f, "sort autotune: end_time and lookahead > 0");
+  GPQ_CHECK_ARG(!results || max_results >= 1, "sort autotune: max_results must be >= 1");
+  float t = sample_end_time > 0.0f ? sample_end_time : 2.0f;
+  if (model->end_time < t) t = model->end_time;
+  static const uint32_t kTiles[] = {2048u, 1024u, 4096u};
+  double best = 0.0;
+  uint32_t pick = 0, n = 0;
+  bool have_ref = false;
+  uint64_t ref_hash = 0;
+  for (uint32_t tile : kTiles) {
+    gpq_queue_config qc = *queue;
+    qc.sort_tile = tile;
+    qc.horizon = t;
+    qc.collect_stats = 0;
+    gpq_phold_config pc = *model;
+    pc.end_time = t;
+    pc.drain = 0;
+    pc.checks = 0;
+    double rate = 0.0;
+    uint64_t hash = 0;
+    uint32_t status = 1;
+    if (time_candidate(qc, pc, &rate, &hash)) {
+      status = 0;
+      if (!have_ref) {
+        have_ref = true;
+        ref_hash = hash;
+      } else if (hash != ref_hash) {
+        status = 2;
+      }
+    }
+    if (status) rate = 0.0;
+    if (results && n < max_results) results[n] = gpq_sort_tuning{tile, status, rate};
+    ++n;
+    if (rate > (pick ? best * kMargin : 0.0)) {
+      best = rate;
+      pick = tile;
+    }
+  }
+  if (n_results) *n_results = n < max_results || !results ? n : max_results;
+  if (!pick) {
+    set_error("sort autotune: no tile ran the workload");
+    return GPQ_ERR_CAPACITY;
+  }
+  queue->sort_tile = pick;
+  return GPQ_OK;
+}

@@ -93,6 +93,8 @@ gpq_status GaqQueue::init() {
   const uint64_t enq_cap = (uint64_t)cfg.num_lps * cfg.stage_per_lp;
   GPQ_CHECK_ARG(enq_cap < (1ull << 31), "GAQ: num_lps * stage_per_lp must be < 2^31");
   cap_ = (uint32_t)cap;
+  tile_ = cfg.sort_tile ? cfg.sort_tile : kSortTile;
+  GPQ_CHECK_ARG(valid_sort_tile(tile_), "GAQ: sort_tile must be 0, 1024, 2048 or 4096");
   v_.num_lps = cfg.num_lps;
   v_.enq_cap = (uint32_t)enq_cap;
   GPQ_TRY(arena.alloc(&fel_[0], cap));
@@ -126,10 +128,10 @@ gpq_status GaqQueue::pre() {
 
 gpq_status GaqQueue::post() {
   Event* sorted = nullptr;
-  GPQ_TRY(sort_records(v_.enq, sort_[0], sort_[1], &v_.st->m, v_.enq_cap, stream, &sorted));
+  GPQ_TRY(sort_records(v_.enq, sort_[0], sort_[1], &v_.st->m, v_.enq_cap, stream, &sorted, tile_));
   hipLaunchKernelGGL(gaq_plan_kernel, dim3(1), dim3(1), 0, stream, v_.st, task_, sorted, rest_, cap_, v_.enq_cap, ws);
   GPQ_HIP(hipGetLastError());
-  GPQ_TRY(merge_tasks(task_, 1, (uint64_t)cap_, stream));
+  GPQ_TRY(merge_tasks(task_, 1, (uint64_t)cap_, stream, tile_));
   hipLaunchKernelGGL(gaq_finish_kernel, dim3(1), dim3(1), 0, stream, v_.st, ws, cfg.lookahead, cfg.horizon);
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;

@@ -312,6 +312,8 @@ gpq_status HqQueue::init() {
   const uint64_t nodes = cfg.heap_nodes ? cfg.heap_nodes : (total + cfg.node_size - 1) / cfg.node_size + 1;
   GPQ_CHECK_ARG(nodes >= 1 && nodes < (1u << 20), "HQ: bad heap_nodes");
   r_ = cfg.node_size;
+  tile_ = cfg.sort_tile ? cfg.sort_tile : kSortTile;
+  GPQ_CHECK_ARG(valid_sort_tile(tile_), "HQ: sort_tile must be 0, 1024, 2048 or 4096");
   nodes_ = (uint32_t)nodes;
   // every heap takes a whole step's inserts (the events of one step may all go to one heap)
   const uint64_t enq_cap = (uint64_t)cfg.num_lps * (cfg.stage_per_lp ? cfg.stage_per_lp : 1);
@@ -409,10 +411,10 @@ gpq_status HqQueue::merge_root(uint32_t h, hipStream_t s) {
   MergeTask* t = tasks_ + (size_t)h * tasks_n_;
   Event* sorted = nullptr;
   GPQ_TRY(sort_records(v_.enq + (size_t)h * v_.enq_cap, sort_[0] + (size_t)h * v_.enq_cap,
-                       sort_[1] + (size_t)h * v_.enq_cap, &st->m, v_.enq_cap, s, &sorted));
+                       sort_[1] + (size_t)h * v_.enq_cap, &st->m, v_.enq_cap, s, &sorted, tile_));
   hipLaunchKernelGGL(hq_plan_merge_root, dim3(1), dim3(1), 0, s, st, counts_ + (size_t)h * nodes_,
                      nodes_buf_ + (size_t)h * nodes_ * r_, sorted, merged_ + (size_t)h * mcap_, v_.enq_cap, t);
-  GPQ_TRY(merge_tasks(t, 1, mcap_, s));
+  GPQ_TRY(merge_tasks(t, 1, mcap_, s, tile_));
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }
@@ -436,10 +438,10 @@ gpq_status HqQueue::step_heap(uint32_t h, bool select, hipStream_t s) {
                         v_.deq + (size_t)h * lps_per_heap_, rest, s));
   }
   hipLaunchKernelGGL(hq_plan_root, dim3(1), dim3(1), 0, s, st, counts, nodes, merged, rest, carry, r_, t);
-  GPQ_TRY(merge_tasks(t, 1, mcap_, s));
+  GPQ_TRY(merge_tasks(t, 1, mcap_, s, tile_));
   hipLaunchKernelGGL(hq_plan_refill, dim3(1), dim3(1), 0, s, st, counts, nodes, dirty, r_, tmp1 + (size_t)slots_ * r_, t);
-  GPQ_TRY(merge_tasks(t, 1, r_, s));
-  GPQ_TRY(merge_tasks(t + 1, 1, r_, s));
+  GPQ_TRY(merge_tasks(t, 1, r_, s, tile_));
+  GPQ_TRY(merge_tasks(t + 1, 1, r_, s, tile_));
   MergeTask* t1 = t;
   MergeTask* t2 = t + slots_;
   MergeTask* t3 = t + 2 * (size_t)slots_;
@@ -447,17 +449,17 @@ gpq_status HqQueue::step_heap(uint32_t h, bool select, hipStream_t s) {
     const uint32_t ns = 1u << lv;
     hipLaunchKernelGGL(hq_plan_level, dim3(ceil_div(ns, 256)), dim3(256), 0, s, st, counts, nodes, dirty, r_, nodes_,
                        lv, tmp2, tmp1, t1, t2, t3);
-    GPQ_TRY(merge_tasks(t1, ns, 2ull * r_, s));
-    GPQ_TRY(merge_tasks(t2, ns, 3ull * r_, s));
-    GPQ_TRY(merge_tasks(t3, ns, r_, s));
+    GPQ_TRY(merge_tasks(t1, ns, 2ull * r_, s, tile_));
+    GPQ_TRY(merge_tasks(t2, ns, 3ull * r_, s, tile_));
+    GPQ_TRY(merge_tasks(t3, ns, r_, s, tile_));
   }
   hipLaunchKernelGGL(hq_plan_chunks, dim3(1), dim3(1), 0, s, st, counts, carry, chunk_buf, r_, nodes_, ws);
   const uint32_t chunks = std::min<uint32_t>(kHqMaxChunks, ceil_div(v_.enq_cap, r_) + 1);
   for (uint32_t c = 0; c < chunks; ++c) {
     for (uint32_t lv = 1; lv <= depth_; ++lv) {
       hipLaunchKernelGGL(hq_plan_insert_level, dim3(1), dim3(1), 0, s, st, counts, nodes, r_, c, lv, tmp1, t);
-      GPQ_TRY(merge_tasks(t, 1, 2ull * r_, s));
-      GPQ_TRY(merge_tasks(t + 1, 1, 2ull * r_, s));
+      GPQ_TRY(merge_tasks(t, 1, 2ull * r_, s, tile_));
+      GPQ_TRY(merge_tasks(t + 1, 1, 2ull * r_, s, tile_));
     }
   }
   GPQ_HIP(hipGetLastError());

@@ -107,6 +107,7 @@ class GaqQueue final : public QueueBase {
   uint32_t* first_ = nullptr;
   uint32_t* scratch_ = nullptr;
   uint32_t cap_ = 0;
+  uint32_t tile_ = kSortTile;   // records per sort / merge block
 };
 
 class HqQueue final : public QueueBase {
@@ -149,6 +150,7 @@ class HqQueue final : public QueueBase {
   uint32_t* scratch_ = nullptr;
   MergeTask* tasks_ = nullptr;
   uint64_t* sum_ = nullptr;
+  uint32_t tile_ = kSortTile;   // records per sort / merge block
 };
 
 }  // namespace gpq

@@ -9,8 +9,13 @@
 
 namespace gpq {
 
-constexpr uint32_t kMergeTile = 2048;   // outputs per merge block (256 threads x 8)
-constexpr uint32_t kSortTile = 2048;    // records per tile-sort block
+// Tiles of the sort and merge kernels: 1024, 2048 (the default) or 4096 records per block
+// (128, 256 or 512 threads x 8 records), chosen per queue (gpq_queue_config.sort_tile).
+constexpr uint32_t kMergeTile = 2048;   // default outputs per merge block
+constexpr uint32_t kSortTile = 2048;    // default records per tile-sort block
+bool valid_sort_tile(uint32_t tile);
+// Merge passes of a sort of up to cap records with tiles of `tile` records.
+uint32_t sort_passes(uint32_t cap, uint32_t tile);
 
 // Stable merge of A = a1[0:n_a1) ++ a2[0:n_a2) (both parts sorted, a1 before a2) with
 // sorted B (ties take A first).  Output rank o goes to out[s][o - start_s], where segment
@@ -27,13 +32,14 @@ struct MergeTask {
 };
 
 // Sorts *d_n (<= cap) records of `in` (not modified) into buf0/buf1; returns the buffer
-// holding the sorted run.  Passes: one tile sort, then ceil(log2(cap / kSortTile)) merge passes.
+// holding the sorted run.  Passes: one tile sort, then ceil(log2(cap / tile)) merge passes.
 gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_t* d_n, uint32_t cap,
-                        hipStream_t stream, Event** sorted);
+                        hipStream_t stream, Event** sorted, uint32_t tile = kSortTile);
 
 // Runs `ntasks` merge tasks (device array) in one launch; max_total bounds every task's
 // n_a1 + n_a2 + n_b.
-gpq_status merge_tasks(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_total, hipStream_t stream);
+gpq_status merge_tasks(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_total, hipStream_t stream,
+                       uint32_t tile = kMergeTile);
 
 // One safe event per LP over the sorted array s[0:*d_n), s = *d_s: the prefix of records with
 // ts < *d_wend and key <= *d_bound (when *d_bound_valid) is scanned, the first record of

@@ -5,8 +5,9 @@
 namespace gpq {
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kPerThread = kMergeTile / kThreads;   // 8
+constexpr int kThreads = 256;                       // selection blocks; sort and merge blocks: kT of
+                                                    // 128, 256 or 512 threads (tiles of 8 kT records)
+constexpr int kPerThread = 8;                       // records per thread in the tile sort and merge
 constexpr uint32_t kSelectTile = 1024;              // records per selection block (256 x 4)
 
 __device__ __forceinline__ Event sentinel() { return Event{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}; }
@@ -35,9 +36,10 @@ struct OutMap {
 // round the 128 threads of a half probe 128 evenly spaced points of the remaining range and
 // keep the gap where the predicate turns true, so a range of n shrinks to nothing in
 // ceil(log_128 n) rounds of one load each, not log_2 n dependent loads.
+template <int kT>
 __device__ __forceinline__ void block_merge_splits(const ASrc& A, uint32_t na, const Event* B, uint32_t nb, uint32_t d0, uint32_t d1,
                                    uint32_t* s_split) {
-  constexpr uint32_t kProbes = kThreads / 2;
+  constexpr uint32_t kProbes = kT / 2;
   __shared__ uint32_t s_lo[2], s_hi[2], s_first[2];
   const uint32_t h = threadIdx.x / kProbes, k = threadIdx.x % kProbes;
   const uint32_t d = h ? d1 : d0;
@@ -78,17 +80,19 @@ __device__ __forceinline__ void block_merge_splits(const ASrc& A, uint32_t na, c
 // consecutive threads merge start in different banks.
 __device__ __forceinline__ uint32_t pad8(uint32_t i) { return i + (i >> 3); }
 
+template <int kT>
 __device__ __forceinline__ void merge_tile(const ASrc& A, uint32_t na, const Event* B, uint32_t nb, uint32_t t0, const OutMap& out) {
-  __shared__ Event s[kMergeTile + kMergeTile / 8];
+  constexpr uint32_t kTile = kT * kPerThread;
+  __shared__ Event s[kTile + kTile / 8];
   __shared__ uint32_t s_split[2];
   const uint32_t total = na + nb;
-  const uint32_t t1 = t0 + kMergeTile < total ? t0 + kMergeTile : total;
-  block_merge_splits(A, na, B, nb, t0, t1, s_split);
+  const uint32_t t1 = t0 + kTile < total ? t0 + kTile : total;
+  block_merge_splits<kT>(A, na, B, nb, t0, t1, s_split);
   const uint32_t i0 = s_split[0], i1 = s_split[1];
   const uint32_t j0 = t0 - i0;
   const uint32_t ca = i1 - i0, cb = (t1 - t0) - ca;
-  for (uint32_t x = threadIdx.x; x < ca; x += kThreads) s[pad8(x)] = A.get(i0 + x);
-  for (uint32_t x = threadIdx.x; x < cb; x += kThreads) s[pad8(ca + x)] = load_event(B + j0 + x);
+  for (uint32_t x = threadIdx.x; x < ca; x += kT) s[pad8(x)] = A.get(i0 + x);
+  for (uint32_t x = threadIdx.x; x < cb; x += kT) s[pad8(ca + x)] = load_event(B + j0 + x);
   __syncthreads();
   const uint32_t len = t1 - t0;
   const uint32_t d = threadIdx.x * kPerThread;
@@ -118,7 +122,7 @@ __device__ __forceinline__ void merge_tile(const ASrc& A, uint32_t na, const Eve
   for (int q = 0; q < kPerThread; ++q)
     if ((uint32_t)q < got) s[pad8(d + q)] = Event{res[q].x, res[q].y, res[q].z, res[q].w};
   __syncthreads();
-  for (uint32_t x = threadIdx.x; x < len; x += kThreads) store_event(out.at(t0 + x), s[pad8(x)]);
+  for (uint32_t x = threadIdx.x; x < len; x += kT) store_event(out.at(t0 + x), s[pad8(x)]);
 }
 
 __device__ __forceinline__ void compare_swap(Event& a, Event& b, bool up) {
@@ -134,13 +138,14 @@ __device__ __forceinline__ void compare_swap(Event& a, Event& b, bool up) {
 // the stages with j >= 8 run in LDS, each thread exchanging 4 of the tile's 1024 pairs.  The
 // LDS tile is padded one record per 8 (pad8) so that the 8-record runs of neighbouring threads
 // start in different banks.
-__global__ __launch_bounds__(kThreads) void tile_sort_kernel(const Event* in, Event* out, const uint32_t* d_n) {
-  static_assert(kSortTile == kThreads * 8, "eight records per thread");
-  __shared__ Event s[kSortTile + kSortTile / 8];
+template <int kT>
+__global__ __launch_bounds__(kT) void tile_sort_kernel(const Event* in, Event* out, const uint32_t* d_n) {
+  constexpr uint32_t kTile = kT * kPerThread;
+  __shared__ Event s[kTile + kTile / 8];
   const uint32_t n = *d_n;
-  const uint32_t base = blockIdx.x * kSortTile;
+  const uint32_t base = blockIdx.x * kTile;
   if (base >= n) return;
-  for (uint32_t i = threadIdx.x; i < kSortTile; i += kThreads)
+  for (uint32_t i = threadIdx.x; i < kTile; i += kT)
     s[pad8(i)] = base + i < n ? load_event(in + base + i) : sentinel();
   __syncthreads();
   const uint32_t t0 = threadIdx.x * 8u;
@@ -154,14 +159,14 @@ __global__ __launch_bounds__(kThreads) void tile_sort_kernel(const Event* in, Ev
 #pragma unroll
       for (uint32_t a = 0; a < 8; ++a)
         if ((a & j) == 0) compare_swap(r[a], r[a | j], ((t0 + a) & k) == 0);
-  for (uint32_t k = 16; k <= kSortTile; k <<= 1) {
+  for (uint32_t k = 16; k <= kTile; k <<= 1) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) s[pad8(t0 + q)] = r[q];
     __syncthreads();
     for (uint32_t j = k >> 1; j >= 8; j >>= 1) {
 #pragma unroll
-      for (uint32_t q = 0; q < kSortTile / 2 / kThreads; ++q) {
-        const uint32_t pr = threadIdx.x + q * kThreads;
+      for (uint32_t q = 0; q < kTile / 2 / kT; ++q) {
+        const uint32_t pr = threadIdx.x + q * kT;
         const uint32_t i = ((pr & ~(j - 1u)) << 1) | (pr & (j - 1u));   // pair (i, i + j), bit j of i clear
         Event x = s[pad8(i)], y = s[pad8(i | j)];
         compare_swap(x, y, (i & k) == 0);
@@ -183,7 +188,7 @@ __global__ __launch_bounds__(kThreads) void tile_sort_kernel(const Event* in, Ev
 #pragma unroll
   for (int q = 0; q < 8; ++q) s[pad8(t0 + q)] = r[q];
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kSortTile; i += kThreads)
+  for (uint32_t i = threadIdx.x; i < kTile; i += kT)
     if (base + i < n) store_event(out + base + i, s[pad8(i)]);
 }
 
@@ -191,14 +196,16 @@ __global__ __launch_bounds__(kThreads) void tile_sort_kernel(const Event* in, Ev
 // `passes`.  Once w >= n every remaining pass would only copy the one run, so those passes
 // do nothing when their number is even (the run already sits in the final buffer) and the
 // first of them copies when it is odd.
-__global__ __launch_bounds__(kThreads) void merge_pass_kernel(const Event* in, Event* out, const uint32_t* d_n,
-                                                              uint32_t w, uint32_t pass, uint32_t passes) {
+template <int kT>
+__global__ __launch_bounds__(kT) void merge_pass_kernel(const Event* in, Event* out, const uint32_t* d_n,
+                                                        uint32_t w, uint32_t pass, uint32_t passes) {
+  constexpr uint32_t kTile = kT * kPerThread;
   const uint32_t n = *d_n;
-  const uint32_t t0 = blockIdx.x * kMergeTile;
+  const uint32_t t0 = blockIdx.x * kTile;
   if (t0 >= n) return;
   if (w >= n) {
     uint32_t first = 0;   // the first pass whose width covers n
-    while (((uint64_t)kSortTile << first) < n) ++first;
+    while (((uint64_t)kTile << first) < n) ++first;
     if (((passes - first) & 1u) == 0 || pass != first) return;
   }
   const uint32_t base = t0 / (2u * w) * (2u * w);
@@ -207,18 +214,20 @@ __global__ __launch_bounds__(kThreads) void merge_pass_kernel(const Event* in, E
   const uint32_t nb = rest < w ? rest : w;
   const ASrc A{in + base, in + base, na};
   OutMap om{{out + base, out + base, out + base}, {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}};
-  merge_tile(A, na, in + base + na, nb, t0 - base, om);
+  merge_tile<kT>(A, na, in + base + na, nb, t0 - base, om);
 }
 
-__global__ __launch_bounds__(kThreads) void merge_tasks_kernel(const MergeTask* tasks) {
+template <int kT>
+__global__ __launch_bounds__(kT) void merge_tasks_kernel(const MergeTask* tasks) {
+  constexpr uint32_t kTile = kT * kPerThread;
   const MergeTask t = tasks[blockIdx.y];
   const uint32_t na = t.n_a1 + t.n_a2;
   const uint32_t total = na + t.n_b;
-  const uint32_t t0 = blockIdx.x * kMergeTile;
+  const uint32_t t0 = blockIdx.x * kTile;
   if (t0 >= total) return;
   const ASrc A{t.a1, t.a2, t.n_a1};
   const OutMap om{{t.out[0], t.out[1], t.out[2]}, {t.end[0], t.end[1], t.end[2]}};
-  merge_tile(A, na, t.b, t.n_b, t0, om);
+  merge_tile<kT>(A, na, t.b, t.n_b, t0, om);
 }
 
 __global__ void copy_records_kernel(const Event* src, Event* dst, const uint32_t* d_n, uint32_t cap) {
@@ -398,17 +407,20 @@ uint32_t grid_of(uint64_t items, uint32_t per_block) {
 
 }  // namespace
 
-gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_t* d_n, uint32_t cap,
-                        hipStream_t stream, Event** sorted) {
-  hipLaunchKernelGGL(tile_sort_kernel, dim3(grid_of(cap, kSortTile)), dim3(kThreads), 0, stream, in, buf0, d_n);
+namespace {
+template <int kT>
+gpq_status sort_records_t(const Event* in, Event* buf0, Event* buf1, const uint32_t* d_n, uint32_t cap,
+                          hipStream_t stream, Event** sorted) {
+  constexpr uint32_t kTile = kT * kPerThread;
+  hipLaunchKernelGGL(tile_sort_kernel<kT>, dim3(grid_of(cap, kTile)), dim3(kT), 0, stream, in, buf0, d_n);
   GPQ_HIP(hipGetLastError());
   Event* cur = buf0;
   Event* other = buf1;
   uint32_t passes = 0;
-  for (uint64_t w = kSortTile; w < cap; w *= 2) ++passes;
+  for (uint64_t w = kTile; w < cap; w *= 2) ++passes;
   uint32_t pass = 0;
-  for (uint64_t w = kSortTile; w < cap; w *= 2, ++pass) {
-    hipLaunchKernelGGL(merge_pass_kernel, dim3(grid_of(cap, kMergeTile)), dim3(kThreads), 0, stream, cur, other, d_n,
+  for (uint64_t w = kTile; w < cap; w *= 2, ++pass) {
+    hipLaunchKernelGGL(merge_pass_kernel<kT>, dim3(grid_of(cap, kTile)), dim3(kT), 0, stream, cur, other, d_n,
                        (uint32_t)w, pass, passes);
     GPQ_HIP(hipGetLastError());
     Event* t = cur;
@@ -418,11 +430,45 @@ gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_
   *sorted = cur;
   return GPQ_OK;
 }
+}  // namespace
 
-gpq_status merge_tasks(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_total, hipStream_t stream) {
+bool valid_sort_tile(uint32_t tile) { return tile == 1024 || tile == 2048 || tile == 4096; }
+
+uint32_t sort_passes(uint32_t cap, uint32_t tile) {
+  uint32_t passes = 0;
+  for (uint64_t w = tile; w < cap; w *= 2) ++passes;
+  return passes;
+}
+
+gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_t* d_n, uint32_t cap,
+                        hipStream_t stream, Event** sorted, uint32_t tile) {
+  switch (tile) {
+    case 1024: return sort_records_t<128>(in, buf0, buf1, d_n, cap, stream, sorted);
+    case 2048: return sort_records_t<256>(in, buf0, buf1, d_n, cap, stream, sorted);
+    case 4096: return sort_records_t<512>(in, buf0, buf1, d_n, cap, stream, sorted);
+    default:
+      set_error("sort tile must be 1024, 2048 or 4096 records");
+      return GPQ_ERR_INVALID;
+  }
+}
+
+gpq_status merge_tasks(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_total, hipStream_t stream,
+                       uint32_t tile) {
   if (ntasks == 0) return GPQ_OK;
-  hipLaunchKernelGGL(merge_tasks_kernel, dim3(grid_of(max_total, kMergeTile), ntasks), dim3(kThreads), 0, stream,
-                     d_tasks);
+  switch (tile) {
+    case 1024:
+      hipLaunchKernelGGL(merge_tasks_kernel<128>, dim3(grid_of(max_total, 1024), ntasks), dim3(128), 0, stream, d_tasks);
+      break;
+    case 2048:
+      hipLaunchKernelGGL(merge_tasks_kernel<256>, dim3(grid_of(max_total, 2048), ntasks), dim3(256), 0, stream, d_tasks);
+      break;
+    case 4096:
+      hipLaunchKernelGGL(merge_tasks_kernel<512>, dim3(grid_of(max_total, 4096), ntasks), dim3(512), 0, stream, d_tasks);
+      break;
+    default:
+      set_error("merge tile must be 1024, 2048 or 4096 records");
+      return GPQ_ERR_INVALID;
+  }
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }

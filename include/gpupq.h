@@ -87,6 +87,8 @@ typedef struct {
                                 calendar, at or past the horizon, or for a full bucket pass through a
                                 per-LP inbox of stage_per_lp records into the far store.  Same
                                 committed trace. */
+  uint32_t sort_tile;         /* GAQ / HQ: records per block of the record sort and merge kernels:
+                                1024, 2048 or 4096 (0 = 2048); gpq_queue_autotune_sort picks one */
 } gpq_queue_config;
 
 typedef struct gpq_queue gpq_queue;
@@ -218,6 +220,23 @@ typedef struct {
  * NULL) their count.  GPQ_ERR_CAPACITY when no layout holds the workload. */
 gpq_status gpq_phold_autotune(gpq_phold_config* model, gpq_queue_config* queue, float sample_end_time,
                               gpq_phold_tuning* results, uint32_t max_results, uint32_t* n_results);
+/* One sort tile the sort autotuner timed, and its rate (0 unless status is 0). */
+typedef struct {
+  uint32_t sort_tile;
+  uint32_t status;              /* 0 ran; 1 could not run; 2 its trace hash differed from the first's */
+  double events_per_second;
+} gpq_sort_tuning;
+
+/* Sort-tile autotuner for the global queues (no reference counterpart; the reference's
+ * moderngpu mergesort fixes its tile at compile time, lib/moderngpu/src/moderngpu/
+ * kernel_mergesort.hxx:27-148): times the tiles 2048, 1024 and 4096 of the record sort and
+ * merge kernels on the model's own workload cut to end time sample_end_time (0 = 2, at most
+ * model->end_time) through queue->kind (GPQ_GAQ or GPQ_HQ), each in a fresh queue driven by
+ * GPQ_DRIVE_GRAPH, checks that every tile commits the same trace, and writes the fastest (a later
+ * tile must beat the pick so far by more than 2%) into queue->sort_tile.  results (may be NULL)
+ * receives up to max_results candidates in the order timed, *n_results (may be NULL) their count. */
+gpq_status gpq_queue_autotune_sort(const gpq_phold_config* model, gpq_queue_config* queue, float sample_end_time,
+                                   gpq_sort_tuning* results, uint32_t max_results, uint32_t* n_results);
 gpq_status gpq_phold_destroy(gpq_phold* m);
 /* model_init() (src/phold.cu:124-163): k initial events per LP, each chunk followed by queue_post_init(). */
 gpq_status gpq_phold_init(gpq_phold* m);

@@ -48,7 +48,7 @@ def test_status_strings_without_device():
 def test_struct_layouts_match_header():
     from gpupq_amd import _lib
     assert ctypes.sizeof(_lib.Event) == 16
-    assert ctypes.sizeof(_lib.QueueConfig) == 72
+    assert ctypes.sizeof(_lib.QueueConfig) == 80
     assert ctypes.sizeof(_lib.PholdConfig) == 32
 
 

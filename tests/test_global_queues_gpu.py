@@ -180,3 +180,56 @@ def test_random_rounds_against_model(gpq, kind, r, heaps):
             now += 0.5
         assert q.queue_check_phold() == sum(len(v) for v in model.values())
         assert q.error_flags() == 0
+
+
+# ---------------- sort tiles of the global queues ----------------
+
+@pytest.mark.parametrize("tile", [1024, 2048, 4096])
+@pytest.mark.parametrize("kind", ["gaq", "hq"])
+def test_sort_tile_keeps_the_trace(gpq, oracle, golden, kind, tile):
+    # every tile of the record sort and merge kernels commits the oracle's trace (per LP), and
+    # the 65536-LP golden
+    def run(L, k, T):
+        kw = dict(capacity_per_lp=64, stage_per_lp=2) if kind == "gaq" else \
+            dict(capacity_per_lp=40, stage_per_lp=1, node_size=max(L, 32))
+        q = gpq.Queue(kind, num_lps=L, lookahead=0.1, horizon=T, sort_tile=tile, **kw)
+        m = gpq.Phold(q, init_per_lp=k, seed=42, end_time=T)
+        m.model_init()
+        rep = gpq.des_run(m, mode="graph", batch=16)
+        c, h = m.lp_state()
+        out = (rep["events"], m.trace_hash(), c, h, q.error_flags())
+        m.model_finish()
+        q.queue_finish()
+        return out
+    res, cnt, hsh = oracle.phold("seq", 3000, 8, 6.0)
+    ev_, th, c, h, fl = run(3000, 8, 6.0)
+    assert fl == 0 and ev_ == res.events and th == res.trace_hash
+    assert np.array_equal(c, cnt) and np.array_equal(h, hsh)
+    gd = golden["mid_l65536_k32_t10"]
+    ev_, th, _, _, fl = run(65536, 32, 10.0)
+    assert fl == 0 and ev_ == gd["events"] and "%016x" % th == gd["trace_hash"]
+
+
+@pytest.mark.parametrize("kind", ["gaq", "hq"])
+def test_sort_tile_autotune(gpq, oracle, kind):
+    L, k, T = 8192, 16, 4.0
+    kw = dict(capacity_per_lp=64, stage_per_lp=2) if kind == "gaq" else \
+        dict(capacity_per_lp=40, stage_per_lp=1, node_size=L)
+    tile, cands = gpq.autotune_sort(kind, L, init_per_lp=k, seed=42, end_time=T, sample_end_time=1.0, **kw)
+    assert [c["sort_tile"] for c in cands] == [2048, 1024, 4096]
+    assert all(c["status"] == "ran" and c["events_per_s"] > 0 for c in cands)
+    picked = [c for c in cands if c["sort_tile"] == tile]
+    assert len(picked) == 1 and picked[0]["events_per_s"] * 1.02 >= max(c["events_per_s"] for c in cands)
+    q = gpq.Queue(kind, num_lps=L, lookahead=0.1, horizon=T, sort_tile=tile, **kw)
+    m = gpq.Phold(q, init_per_lp=k, seed=42, end_time=T)
+    m.model_init()
+    gpq.des_run(m, mode="graph", batch=16)
+    res, _, _ = oracle.phold("seq", L, k, T)
+    assert m.trace_hash() == res.trace_hash
+    m.model_finish()
+    q.queue_finish()
+
+
+def test_sort_tile_rejected(gpq):
+    with pytest.raises(gpq.GpqError):
+        gpq.Queue("gaq", num_lps=64, capacity_per_lp=8, stage_per_lp=2, lookahead=0.1, horizon=1.0, sort_tile=3000)
