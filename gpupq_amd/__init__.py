@@ -402,6 +402,16 @@ class AstarBatch:
         check(lib.gpq_astar_batch_grid(self.handle, q, _ptr(out)), "astar_batch_grid")
         return out.reshape(side, side)
 
+    def path(self, q):
+        """An optimal path of query q as (x, y) cells, source first ([] when unreachable); q is
+        searched again alone (src/a_star.cu:417-438 backtracks the closed list the same way)."""
+        side = self.specs[q].side
+        n = ctypes.c_uint32()
+        check(lib.gpq_astar_batch_path(self.handle, q, None, 0, ctypes.byref(n)), "astar_batch_path")
+        out = np.zeros(max(n.value, 1), dtype=np.uint32)
+        check(lib.gpq_astar_batch_path(self.handle, q, _ptr(out), len(out), ctypes.byref(n)), "astar_batch_path")
+        return [(int(c) % side, int(c) // side) for c in out[:n.value]]
+
     def close(self):
         if self.handle:
             check(lib.gpq_astar_batch_destroy(self.handle), "astar_batch_destroy")

@@ -156,6 +156,7 @@ SIGNATURES = [
     ("gpq_astar_batch_run", ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     ("gpq_astar_batch_results", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     ("gpq_astar_batch_timing", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    ("gpq_astar_batch_path", ctypes.c_int, [c_void_p, c_u32, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     ("gpq_astar_batch_grid", ctypes.c_int, [c_void_p, c_u32, c_void_p]),
     ("gpq_phold_batch_create", ctypes.c_int, [c_void_p, c_u32, c_u32, c_u32, c_u32, ctypes.POINTER(c_void_p)]),
     ("gpq_phold_batch_destroy", ctypes.c_int, [c_void_p]),

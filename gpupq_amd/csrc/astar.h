@@ -106,6 +106,8 @@ class AstarBatch {
   gpq_status results(int64_t* costs, uint64_t* expanded);
   gpq_status timing(double* start_us, double* finish_us);
   gpq_status grid_bytes(uint32_t q, uint8_t* out);
+  // An optimal path of query q (source to target): q searched again alone in slot 0.
+  gpq_status path(uint32_t q, uint32_t* cells, uint32_t cap, uint32_t* len);
 
   // Searches the queries d_ids[0..n) (device array) in one persistent launch on stream s,
   // workgroups in slots [slot_base, slot_base + slots), work counter d_counter.

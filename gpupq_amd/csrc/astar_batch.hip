@@ -634,6 +634,69 @@ gpq_status AstarBatch::timing(double* start_us, double* finish_us) {
   return GPQ_OK;
 }
 
+// An optimal path of query q: q searched again alone in slot 0 (one persistent launch of one
+// workgroup), then the closed-list backtrack of src/a_star.cu:90-94,417-438 over that slot's
+// g values on the host: from the target (whose g is the query's cost), a free neighbour whose g
+// (of the slot's current tag) plus the step cost equals the current cell's g.  Along such a
+// chain every g is the optimal distance (the target's is; a neighbour meeting the equality
+// cannot be above its optimum), so the neighbour that relaxed a cell last always qualifies and
+// the chain reaches the source.
+gpq_status AstarBatch::path(uint32_t q, uint32_t* cells, uint32_t cap, uint32_t* len) {
+  GPQ_CHECK_ARG(q < specs.size(), "batch: query index out of range");
+  GPQ_CHECK_ARG(len && (cells || cap == 0), "batch path: null output");
+  const Dev* d = static_cast<const Dev*>(dev);
+  const gpq_query_spec& sp = specs[q];
+  const uint32_t side = sp.side;
+  const uint64_t n = (uint64_t)side * side;
+  GPQ_HIP(hipStreamSynchronize(stream));
+  GPQ_TRY(launch(d_ids_all + q, 1, 0, 1, counter, stream));
+  long long cost = -1;
+  SlotState st{};
+  std::vector<uint32_t> g(n);
+  GPQ_HIP(hipMemcpyAsync(&cost, d->cost + q, sizeof(cost), hipMemcpyDeviceToHost, stream));
+  GPQ_HIP(hipMemcpyAsync(&st, d->slot, sizeof(st), hipMemcpyDeviceToHost, stream));
+  GPQ_HIP(hipMemcpyAsync(g.data(), d->g, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream));
+  GPQ_HIP(hipStreamSynchronize(stream));
+  *len = 0;
+  if (cost < 0) return GPQ_OK;
+  std::vector<uint8_t> blocked(n);
+  GPQ_TRY(grid_bytes(q, blocked.data()));
+  const uint32_t mask = (1u << kTagShift) - 1u;
+  auto gval = [&](uint64_t c, uint32_t* out) {
+    if ((g[c] >> kTagShift) != st.tag) return false;
+    *out = g[c] & mask;
+    return true;
+  };
+  static const int dx[8] = {1, -1, 0, 0, 1, -1, 1, -1}, dy[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+  const uint64_t source = (uint64_t)sp.sy * side + sp.sx;
+  uint64_t cur = (uint64_t)sp.ty * side + sp.tx;
+  uint32_t gc = (uint32_t)cost;          // the search records the target's cost, not its g entry
+  std::vector<uint32_t> rev{(uint32_t)cur};
+  while (cur != source) {
+    const int x = (int)(cur % side), y = (int)(cur / side);
+    bool found = false;
+    for (uint32_t k = 0; k < sp.connectivity && !found; ++k) {
+      const int nx = x + dx[k], ny = y + dy[k];
+      if (nx < 0 || ny < 0 || nx >= (int)side || ny >= (int)side) continue;
+      const uint64_t nc = (uint64_t)ny * side + (uint64_t)nx;
+      const uint32_t step = sp.connectivity == 4 ? 1u : (k < 4 ? 10u : 14u);
+      uint32_t gn = 0;
+      if (blocked[nc] || !gval(nc, &gn) || gn + step != gc) continue;
+      cur = nc;
+      gc = gn;
+      found = true;
+    }
+    if (!found || rev.size() > n) {
+      set_error("batch path: no predecessor with a matching g");
+      return GPQ_ERR_STATE;
+    }
+    rev.push_back((uint32_t)cur);
+  }
+  *len = (uint32_t)rev.size();
+  for (uint32_t i = 0; i < *len && i < cap; ++i) cells[i] = rev[*len - 1 - i];
+  return GPQ_OK;
+}
+
 gpq_status AstarBatch::grid_bytes(uint32_t q, uint8_t* out) {
   GPQ_CHECK_ARG(q < specs.size(), "batch: query index out of range");
   const Dev* d = static_cast<const Dev*>(dev);

@@ -115,6 +115,12 @@ gpq_status gpq_astar_batch_timing(gpq_astar_batch* b, double* start_us, double* 
   return b->impl->timing(start_us, finish_us);
 }
 
+gpq_status gpq_astar_batch_path(gpq_astar_batch* b, uint32_t q, uint32_t* cells, uint32_t cap, uint32_t* len) {
+  GPQ_NONNULL(b);
+  GPQ_NONNULL(len);
+  return b->impl->path(q, cells, cap, len);
+}
+
 gpq_status gpq_astar_batch_grid(gpq_astar_batch* b, uint32_t q, uint8_t* blocked) {
   GPQ_NONNULL(b);
   GPQ_NONNULL(blocked);

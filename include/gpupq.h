@@ -380,6 +380,11 @@ gpq_status gpq_astar_batch_results(gpq_astar_batch* b, int64_t* costs, uint64_t*
 /* Device wall-clock times of the last run, per query, in microseconds from the earliest
  * pickup: when a workgroup picked the query up and when its search completed. */
 gpq_status gpq_astar_batch_timing(gpq_astar_batch* b, double* start_us, double* finish_us);
+/* An optimal path of query q, source to target, as cell indices y * side + x (the closed-list
+ * backtrack of src/a_star.cu:90-94,417-438): q is searched again alone in workgroup slot 0 and
+ * the path is read back from that slot's g values.  *len = path length (0 when the target is
+ * unreachable); at most cap cells are written.  Overwrites q's timing of the last run. */
+gpq_status gpq_astar_batch_path(gpq_astar_batch* b, uint32_t q, uint32_t* cells, uint32_t cap, uint32_t* len);
 /* Obstacle map of query q (side*side bytes). */
 gpq_status gpq_astar_batch_grid(gpq_astar_batch* b, uint32_t q, uint8_t* blocked);
 

@@ -352,3 +352,27 @@ def test_batch_slot_reuse_past_tag_exhaustion(gpq, oracle):
             grid = b.grid(q).reshape(-1)
             c, _ = oracle.astar(grid, s.side, s.side, 8, (s.sx, s.sy), (s.tx, s.ty))
             assert costs[q] == c, "query %d" % q
+
+
+def test_batch_paths(gpq, oracle):
+    # an optimal path of every query, read back from the query searched again alone: valid steps
+    # over free cells whose costs sum to the batch's (and the oracle's) cost
+    specs = synthetic(gpq, 11, 10)
+    S = gpq._lib.QuerySpec
+    specs.append(S(64, thr(0.0), 3, 3, 3, 3, 8, 1, 1, 10))             # source == target
+    specs.append(S(96, thr(0.25), 0, 0, 95, 95, 4, 5, 1, 11))          # 4-connected
+    with gpq.AstarBatch(specs, threads_per_query=128) as b:
+        b.run("graph")
+        costs, _ = b.results()
+        for q, s in enumerate(specs):
+            grid = b.grid(q).reshape(-1)
+            path = b.path(q)
+            if costs[q] < 0:
+                assert path == []
+                continue
+            check_path(path, grid, s.side, s.connectivity, (s.sx, s.sy), (s.tx, s.ty), costs[q])
+            if q < 10:
+                c, _ = oracle.astar(grid, s.side, s.side, 8, (s.sx, s.sy), (s.tx, s.ty))
+                assert costs[q] == c
+        # the batch's results stay as they were
+        assert np.array_equal(b.results()[0], costs)

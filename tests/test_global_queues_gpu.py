@@ -108,6 +108,49 @@ def test_hq_multi_insert(gpq):
         check_heap(gpq, q, 8)
 
 
+def test_hq_insert_update_fixture(gpq):
+    # src/test_hq.cu:605-640 (insert_update): nodes [18,23], [27,29], [30,31] and two new records
+    # 28, 31: the root keeps 18, 23; the chunk 28, 31 travels to the next free node (3) through
+    # node 1, where 28 stays and 29 moves on: [18,23], [27,28], [30,31], [29,31]
+    with heap(gpq) as q:
+        build(gpq, q, [((18, 23), (0, 1)), ((27, 29), (0, 0)), ((30, 31), (0, 0))])
+        q.insert(ev(gpq, [(28, 0, 1, 28), (31, 0, 1, 131)]))
+        q.queue_post_init()
+        assert [tsl(gpq, q.heap_node(v)) for v in range(4)] == [[18.0, 23.0], [27.0, 28.0], [30.0, 31.0],
+                                                                 [29.0, 31.0]]
+        check_heap(gpq, q, 8)
+
+
+def test_hq_insert_update_partial_node_fixture(gpq):
+    # src/test_hq.cu:642-680 (insert_update2): nodes [18,23], [27,29], [30] and two new records
+    # 28, 31: the reference fills node 2's free slot first ([28,30]) and starts node 3 with 31.
+    # Here the same seven records end in the same heap order, root [18,23] and node 1 [27,29]
+    # unchanged
+    with heap(gpq) as q:
+        build(gpq, q, [((18, 23), (0, 1)), ((27, 29), (0, 0)), ((30,), (0,))])
+        q.insert(ev(gpq, [(28, 0, 1, 28), (31, 0, 1, 131)]))
+        q.queue_post_init()
+        nodes = [tsl(gpq, q.heap_node(v)) for v in range(4)]
+        assert nodes[0] == [18.0, 23.0] and nodes[1] == [27.0, 29.0]
+        assert sorted(x for n in nodes[2:] for x in n) == [28.0, 30.0, 31.0]
+        assert check_heap(gpq, q, 8) == 7
+
+
+def test_hq_copy_last_nonempty_root_fixture(gpq):
+    # src/test_hq.cu:850-879 (copy_last_nonempty_root): after the step's dequeue the root holds
+    # only 25; the last node [30,31] refills it (root [25,31], node 2 [30]) and delete-update
+    # restores the order: the root keeps the two smallest of root and children, 25 and 27
+    with heap(gpq) as q:
+        build(gpq, q, [((18, 25), (0, 0)), ((27, 29), (1, 1)), ((30, 31), (1, 1))])
+        q.queue_post()
+        items, _ = q.peek()
+        assert tsl(gpq, items) == [18.0]           # one safe event per LP: 25 waits behind 18
+        nodes = [tsl(gpq, q.heap_node(v)) for v in range(3)]
+        assert nodes[0] == [25.0, 27.0]
+        assert sorted(nodes[1] + nodes[2]) == [29.0, 30.0, 31.0]
+        assert check_heap(gpq, q, 8) == 5
+
+
 def check_heap(gpq, q, nodes, heaps=1, lps_per_heap=None):
     """Every node sorted, parent maximum <= child minimum, counts add up; with several heaps,
     node v of heap h is h * nodes + v and holds only records of heap h's LPs."""
