@@ -273,6 +273,138 @@ def per_queue_bench(gpq, wl, log, drive="graph"):
     return out
 
 
+# Kernel groups of the other queue kinds, with the structure-independent byte model of the
+# operation each implements (per record the queue's work counters report).
+QUEUE_GROUPS = {
+    "gaq": ["sort", "merge", "select"],
+    "hq": ["sort", "merge", "select"],
+    "lhq": ["post"],
+    "lsq": ["post"],
+}
+GROUP_KERNELS = {
+    "sort": ("tile_sort_kernel", "merge_pass_kernel"),
+    "merge": ("merge_tasks_kernel",),
+    "select": ("select_",),
+}
+
+
+def queue_roofline(gpq, kind, wl, log, counts):
+    """One eager run (a launch sequence per step) of the workload through `kind`. With
+    `counts`, the queue's work counters are on and the line carries each kernel group's
+    algorithmic bytes; without, the run is the one a kernel trace times (the counters add
+    launches and atomics). The runs are deterministic, so both see the same work."""
+    global DRAIN, CAL
+    DRAIN = kind in ("lhq", "lsq")       # the local heaps drain each LP's window per step
+    CAL = 0
+    q, m = make(gpq, kind, wl, stats=counts)
+    q.sync()
+    t0 = time.perf_counter()
+    rep = gpq.des_run(m, mode="plain")
+    dt = time.perf_counter() - t0
+    st = q.stats()
+    h = m.trace_hash()
+    m.model_finish()
+    q.queue_finish()
+    out = {"queue_roofline": kind, "workload": wl, "drive": "plain (eager)", "events": rep["events"],
+           "windows": rep["windows"], "seconds": dt, "trace_hash": "%016x" % h}
+    if counts:
+        L = wl["num_lps"]
+        g = {}
+        if kind in ("gaq", "hq"):
+            g["sort"] = {"alg_bytes": 2.0 * RECORD * st["sorted"], "records": st["sorted"],
+                         "model": "sort of N records: 2 x 16 B per record (merge passes are overhead)"}
+            g["merge"] = {"alg_bytes": 2.0 * RECORD * st["merged"], "records": st["merged"],
+                          "model": "merge of runs a and b: 2 x 16 B x (a + b)"}
+            g["select"] = {"alg_bytes": 2.0 * RECORD * st["selected"] + st["selected"] / 8.0,
+                           "records": st["selected"],
+                           "model": "stable compaction of the scanned prefix of N records: 2 x 16 B x N + N/8"}
+        else:
+            moved = st["displaced"] if kind == "lhq" else 0
+            g["post"] = {"alg_bytes": RECORD * (2.0 * st["inserted"] + 2.0 * moved) + 4.0 * st["scanned_lps"]
+                         + 4.0 * ((L + 255) // 256) * st["post_launches"],
+                         "records": st["inserted"], "moved": moved,
+                         "splay_visits": st["displaced"] if kind == "lsq" else 0,
+                         "model": "insert of k records moving d (heap sift): 16 B x (2k + 2d), plus 4 B per "
+                                  "scanned LP head and per block partial; splay visits are overhead"}
+        for k, e in g.items():
+            e["kernels"] = list(GROUP_KERNELS.get(k, ("%s_post_kernel" % kind,)))
+        out["groups"] = g
+    log("queue_roofline: %s" % json.dumps(out))
+    return out
+
+
+def _short_kernel(name):
+    """`void gpq::(anonymous namespace)::merge_tasks_kernel<256>(...)` -> `merge_tasks_kernel<256>`."""
+    head = name.replace("(anonymous namespace)::", "")
+    if head.startswith("void "):
+        head = head[5:]
+    depth, start, end = 0, 0, len(head)
+    for i, ch in enumerate(head):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif depth == 0 and ch == ":" and head[i + 1:i + 2] == ":":
+            start = i + 2
+        elif ch == "(" and depth == 0:
+            end = i
+            break
+    return head[start:end].strip()
+
+
+def roofline_table(stats_csv, counts_json, fetch_csv=None, write_csv=None, scale=(2.0, 1.0)):
+    """Joins a rocprofv3 --kernel-trace --stats summary (kernel time) with the work counts of a
+    --queue-roofline --roofline-counts run (algorithmic bytes) and, optionally, FETCH_SIZE and
+    WRITE_SIZE counter runs (HBM bytes, scaled by the copy calibration), as a markdown table."""
+    import csv
+    with open(counts_json) as f:
+        c = [json.loads(ln) for ln in f if ln.startswith("{")][-1]
+    with open(stats_csv) as f:
+        rows = list(csv.DictReader(f))
+
+    def counter(path):
+        per = {}
+        if not path:
+            return per
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                k = _short_kernel(r["Kernel_Name"])
+                tot, n = per.get(k, (0.0, 0))
+                per[k] = (tot + float(r["Counter_Value"]), n + 1)
+        return per
+
+    fetch, write = counter(fetch_csv), counter(write_csv)
+    print("| group | kernels | launches | kernel time (ms) | mean (us) | algorithmic bytes | achieved (GB/s) | "
+          "of 8 TB/s | HBM bytes (counters) |")
+    print("|---|---|---|---|---|---|---|---|---|")
+    for gname, e in c["groups"].items():
+        ns, calls, names = 0.0, 0, set()
+        for r in rows:
+            k = _short_kernel(r["Name"])
+            if any(k.startswith(p) for p in e["kernels"]):
+                ns += float(r["TotalDurationNs"])
+                calls += int(r["Calls"])
+                names.add(k)
+        gbps = e["alg_bytes"] / ns if ns else 0.0
+        hbm = "—"
+        if fetch and write:
+            kb = sum(fetch[k][0] * scale[0] + write.get(k, (0.0, 0))[0] * scale[1] for k in fetch
+                     if any(k.startswith(p) for p in e["kernels"]))
+            hbm = "%.3g (%.2fx)" % (kb * 1024, kb * 1024 / e["alg_bytes"]) if e["alg_bytes"] else "%.3g" % (kb * 1024)
+        print("| %s | %s | %d | %.1f | %.1f | %.4g | %.0f | %.1f%% | %s |"
+              % (gname, ", ".join("`%s`" % n for n in sorted(names)), calls, ns / 1e6, ns / 1e3 / max(calls, 1),
+                 e["alg_bytes"], gbps, 100.0 * gbps / HBM_PEAK_GBPS, hbm))
+    print()
+    print("Byte models: " + "; ".join("%s: %s" % (g, e["model"]) for g, e in c["groups"].items()) + ".")
+    grouped = tuple(p for e in c["groups"].values() for p in e["kernels"])
+    rest = sorted(((float(r["TotalDurationNs"]), int(r["Calls"]), _short_kernel(r["Name"])) for r in rows
+                   if not any(_short_kernel(r["Name"]).startswith(p) for p in grouped)), reverse=True)[:8]
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    print()
+    print("Other kernels of the run (largest first): " + "; ".join(
+        "`%s` %d calls, %.1f ms (%.1f%%)" % (n, k, t / 1e6, 100.0 * t / total) for t, k, n in rest) + ".")
+
+
 def phold_autotune_bench(gpq, wl, log, sample_end_time=5.0):
     """The PHOLD layout autotuner on the workload's first `sample_end_time` time units: every
     candidate layout's rate, the pick, and whether the benchmarked layout is the pick."""
@@ -319,25 +451,39 @@ def phold_batch_bench(gpq, log, instances=32, seed=5, plain=True):
     import numpy as np
     rng = np.random.default_rng(seed)
     inst = [(int(round(2 ** rng.uniform(12, 18))), 16, int(rng.integers(1, 2 ** 31)), 20.0) for _ in range(instances)]
-    kw = dict(capacity_per_lp=64, far_per_lp=FAR) if FAR else {}
-    with gpq.PholdBatch(inst, **kw) as b:
-        b.run("graph", batch=16)                  # warm up (graph capture)
-        tg = b.run("graph", batch=16)
-        hg, eg = b.results()
+
+    def one(kw, layout):
+        with gpq.PholdBatch(inst, **kw) as b:
+            b.run("graph", batch=16)              # warm up (graph capture)
+            tg = b.run("graph", batch=16)
+            hg, eg = b.results()
+            if plain:
+                tp = b.run("plain")
+                hp, ep = b.results()
+        if plain and (list(hg) != list(hp) or list(eg) != list(ep)):
+            raise RuntimeError("batched PHOLD (%s): batched and plain results differ" % layout)
+        ev = int(eg.sum())
+        r = {"layout": layout, "events": ev, "events_per_s": ev / tg, "seconds": tg}
         if plain:
-            tp = b.run("plain")
-            hp, ep = b.results()
-    if plain and (list(hg) != list(hp) or list(eg) != list(ep)):
-        raise RuntimeError("batched PHOLD: pool and plain results differ")
-    ev = int(eg.sum())
-    if not plain:
-        out = {"instances": instances, "total_lps": sum(i[0] for i in inst), "events": ev, "events_per_s": ev / tg}
-        log("phold_batch (pool only): %s" % json.dumps(out))
-        return out
-    out = {"instances": instances, "lps": "2^U[12,18] x 16, end time 20", "total_lps": sum(i[0] for i in inst),
-           "laq_far_per_lp": FAR,
-           "events": ev, "events_per_s": ev / tg, "plain_events_per_s": ev / tp, "batched_vs_plain": tp / tg,
-           "results_batched_vs_plain": "match"}
+            r.update(plain_events_per_s=ev / tp, plain_seconds=tp, batched_vs_plain=tp / tg,
+                     results_batched_vs_plain="match")
+        return r, hg, eg
+
+    # calendar: every instance in one calendar LAQ on one window sequence; the plain path runs
+    # each instance alone in its own calendar LAQ (the same step kind)
+    if CAL:
+        main, hc, ec = one(dict(calendar_windows=CAL, capacity_per_lp=CAL_SLOTS, stage_per_lp=32, far_per_lp=32),
+                           "calendar LAQ (%d windows, logs of %d records per LP), one window sequence" % (CAL, CAL_SLOTS))
+    ring_kw = dict(capacity_per_lp=64, far_per_lp=FAR or 128)
+    ring, hr, er = one(ring_kw, "two-tier LAQ (ring 64, far 128), post kernel + drain step, a window per instance")
+    if not CAL:
+        main, hc, ec = ring, hr, er
+    elif list(hc) != list(hr) or list(ec) != list(er):
+        raise RuntimeError("batched PHOLD: calendar and two-tier results differ")
+    out = {"instances": instances, "lps": "2^U[12,18] x 16, end time 20", "total_lps": sum(i[0] for i in inst)}
+    out.update(main)
+    if CAL:
+        out["two_tier_ring"] = ring
     log("phold_batch: %s" % json.dumps(out))
     return out
 
@@ -493,9 +639,20 @@ def main():
     ap.add_argument("--skip-phold", action="store_true", help="profiling aid: A* and serving parts only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy five times (counter calibration)")
+    ap.add_argument("--queue-roofline", choices=sorted(QUEUE_GROUPS),
+                    help="only run the workload once, eagerly, through this queue kind (for a kernel trace)")
+    ap.add_argument("--roofline-counts", action="store_true",
+                    help="with --queue-roofline: work counters on; print each kernel group's algorithmic bytes")
+    ap.add_argument("--end-time", type=float, default=0.0, help="override the workload's end time (0: keep)")
+    ap.add_argument("--roofline-table", nargs="+", metavar="CSV",
+                    help="KERNEL_STATS_CSV COUNTS_JSON [FETCH_CSV WRITE_CSV]: print the queue-kernel roofline "
+                         "table of a --queue-roofline run (no GPU)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.roofline_table:
+        roofline_table(*args.roofline_table)
         return
     if args.gpus != 1:
         print("bench.py: the benchmark runs on one GPU; --gpus %d is not provided "
@@ -516,11 +673,21 @@ def main():
     global FAR_STEP, CAL, CAL_SLOTS
     FAR_STEP = args.far_step
     CAL, CAL_SLOTS = args.calendar, args.cal_slots
-    wl = WORKLOADS[args.workload]
+    wl = dict(WORKLOADS[args.workload])
     golden = None
     gpath = os.path.join(ROOT, "tests", "golden", "phold_golden.json")
     if os.path.exists(gpath):
         golden = json.load(open(gpath)).get(args.workload)
+    if args.end_time > 0 and args.end_time != wl["end_time"]:
+        wl["end_time"] = args.end_time
+        golden = None                    # the goldens hold the full workloads only
+
+    if args.queue_roofline:
+        out = queue_roofline(gpq, args.queue_roofline, wl, log, args.roofline_counts)
+        if golden is not None:
+            out["trace_hash_vs_golden"] = "match" if out["trace_hash"] == golden["trace_hash"] else "MISMATCH"
+        print(json.dumps(out), flush=True)
+        return
 
     if args.skip_phold:
         line = {"metric": METRIC, "value": None, "unit": "events/s", "n_gpus": 1, "note": "PHOLD part skipped"}

@@ -435,17 +435,24 @@ class AstarBatch:
 # ---------------------------------------------------------------------------------------
 class PholdBatch:
     """Many PHOLD instances of uneven size advanced together in one local array queue;
-    instances = [(num_lps, init_per_lp, seed, end_time), ...] (lookahead 0.1, p_self 0.5)."""
+    instances = [(num_lps, init_per_lp, seed, end_time[, lookahead]), ...] (lookahead by default
+    `lookahead`, p_self 0.5).  calendar_windows > 0: calendar-mode LAQs for both drives, the
+    graph drive with all instances in one calendar on one window sequence (capacity_per_lp is
+    then a window's log records per LP, far_per_lp the list capacity per LP)."""
 
-    def __init__(self, instances, capacity_per_lp=128, stage_per_lp=32, lookahead=0.1, p_self=0.5, far_per_lp=0):
+    def __init__(self, instances, capacity_per_lp=128, stage_per_lp=32, lookahead=0.1, p_self=0.5, far_per_lp=0,
+                 calendar_windows=0):
         ps = min(int(round(p_self * 2 ** 32)), 0xFFFFFFFF)
         self.n = len(instances)
-        self.inst = (_lib.PholdInstance * self.n)(*[_lib.PholdInstance(l, k, s, t, lookahead, ps)
-                                                     for (l, k, s, t) in instances])
+        rows = [tuple(i) + (lookahead,) if len(i) == 4 else tuple(i) for i in instances]
+        self.inst = (_lib.PholdInstance * self.n)(*[_lib.PholdInstance(l, k, s, t, la, ps)
+                                                     for (l, k, s, t, la) in rows])
         h = ctypes.c_void_p()
         check(lib.gpq_phold_batch_create(self.inst, self.n, capacity_per_lp, stage_per_lp, far_per_lp,
                                          ctypes.byref(h)), "phold_batch_create")
         self.handle = h
+        if calendar_windows:
+            check(lib.gpq_phold_batch_set_calendar(self.handle, calendar_windows), "phold_batch_set_calendar")
 
     def run(self, mode="graph", batch=16):
         sec = ctypes.c_double()

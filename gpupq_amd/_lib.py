@@ -52,7 +52,8 @@ class DesReport(ctypes.Structure):
 
 class QueueStats(ctypes.Structure):
     _fields_ = [("post_launches", c_u64), ("inserted", c_u64), ("displaced", c_u64), ("merged_lps", c_u64),
-                ("scanned_lps", c_u64), ("aux_bytes", c_u64)]
+                ("scanned_lps", c_u64), ("aux_bytes", c_u64), ("sorted", c_u64), ("merged", c_u64),
+                ("selected", c_u64)]
 
 
 class KernelProfile(ctypes.Structure):
@@ -159,6 +160,7 @@ SIGNATURES = [
     ("gpq_astar_batch_path", ctypes.c_int, [c_void_p, c_u32, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     ("gpq_astar_batch_grid", ctypes.c_int, [c_void_p, c_u32, c_void_p]),
     ("gpq_phold_batch_create", ctypes.c_int, [c_void_p, c_u32, c_u32, c_u32, c_u32, ctypes.POINTER(c_void_p)]),
+    ("gpq_phold_batch_set_calendar", ctypes.c_int, [c_void_p, c_u32]),
     ("gpq_phold_batch_destroy", ctypes.c_int, [c_void_p]),
     ("gpq_phold_batch_run", ctypes.c_int, [c_void_p, ctypes.c_int, c_u32, ctypes.POINTER(ctypes.c_double)]),
     ("gpq_phold_batch_results", ctypes.c_int, [c_void_p, c_void_p, c_void_p]),

@@ -45,6 +45,9 @@ gpq_status QueueBase::stats_get(gpq_queue_stats* out) {
   out->merged_lps = h[2];
   out->scanned_lps = h[3];
   out->aux_bytes = h[4];
+  out->sorted = h[kStatSorted];
+  out->merged = h[kStatMerged];
+  out->selected = h[kStatSelected];
   return GPQ_OK;
 }
 

@@ -137,6 +137,16 @@ gpq_status gpq_phold_batch_create(const gpq_phold_instance* inst, uint32_t count
   return GPQ_OK;
 }
 
+gpq_status gpq_phold_batch_set_calendar(gpq_phold_batch* b, uint32_t calendar_windows) {
+  GPQ_NONNULL(b);
+  GPQ_CHECK_ARG(calendar_windows == 0 ||
+                    ((calendar_windows & (calendar_windows - 1u)) == 0 && calendar_windows >= 8 &&
+                     calendar_windows <= gpq::kCalMaxWindows),
+                "phold batch: calendar_windows must be 0 or a power of two in [8, 256]");
+  b->impl->calendar_windows = calendar_windows;
+  return GPQ_OK;
+}
+
 gpq_status gpq_phold_batch_destroy(gpq_phold_batch* b) {
   if (!b) return GPQ_OK;
   delete b->impl;

@@ -111,11 +111,14 @@ class QueueBase {
   uint32_t partial_slots = 0;
   uint32_t threads = 256;
   // Work counters (device, kStatSlots u64) when cfg.collect_stats; nullptr otherwise.
-  // Slots: 0 inserted, 1 displaced, 2 merged LPs, 3 scanned LPs, 4 aux bytes.
+  // Slots: 0 inserted, 1 displaced, 2 merged LPs, 3 scanned LPs, 4 aux bytes, 5 records
+  // sorted, 6 records merged, 7 records scanned by the safe-event selection.
   unsigned long long* stats = nullptr;
+  unsigned long long* acct(int slot) const { return stats ? stats + slot : nullptr; }
 };
 
-constexpr int kStatSlots = 5;
+constexpr int kStatSlots = 8;
+constexpr int kStatSorted = 5, kStatMerged = 6, kStatSelected = 7;
 
 }  // namespace gpq
 

@@ -20,6 +20,24 @@ gpq_status with_view(QueueBase* q, F&& f) {
   }
 }
 
+// One PHOLD instance of several sharing a queue: its LP range and model parameters.
+struct PoolInst {
+  uint32_t base, lps, init_per_lp, seed;
+  float end_time, lookahead;
+  uint32_t p_self, active;
+  uint32_t block_begin, block_end;      // post blocks [block_begin, block_end) cover the range
+  uint32_t pad[2];
+};
+
+// Several instances in one calendar-mode LAQ, all on the queue's window sequence (the queue's
+// lookahead is at most every instance's, its horizon the latest end time): the LPs of 256-LP
+// block b belong to instance inst[block_inst[b]].  One calendar step over all of them, and
+// initial event j of every LP (staged for queue_post_init, or parked at or past the
+// instance's end time).
+gpq_status launch_calendar_instances(LaqQueue* q, const PoolInst* inst, const uint32_t* block_inst, uint32_t* count,
+                                     uint64_t* hash);
+gpq_status launch_populate_instances(LaqQueue* q, const PoolInst* inst, const uint32_t* block_inst, uint32_t j);
+
 // queue_insert (update = false) or queue_insert_or_update (true) of n device items.
 gpq_status launch_insert_items(QueueBase* q, const Event* d_items, uint32_t n, bool update = false);
 gpq_status launch_peek_items(QueueBase* q, int set_done, Event* d_out, int32_t* d_index, uint32_t* d_found,

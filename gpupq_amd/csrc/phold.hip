@@ -126,20 +126,49 @@ __global__ __launch_bounds__(kFusedBlock, 4) void phold_fused_kernel(LaqView q, 
 }
 
 // The step on a calendar-mode LAQ (laq_calendar.cuh): blocks [0, group_blocks) stage window
-// cur + 1, the others commit window cur; the whole queue is one instance.  (The staging blocks
-// run first and the commit blocks fill the device beside them; the same two parts as kernels
-// on two streams measured 7% slower.)
+// cur + 1, the others commit window cur.  With `inst` null the whole queue is one instance;
+// otherwise commit block b takes the model parameters and the end time of instance
+// inst[block_inst[b]] (several instances in one queue, every 256-LP block in one instance).
+// (The staging blocks run first and the commit blocks fill the device beside them; the same
+// two parts as kernels on two streams measured 7% slower.)
 __global__ __launch_bounds__(cal::kBlock) void phold_calendar_kernel(LaqView q, PholdDev m, float end_time,
-                                                                     uint32_t group_blocks) {
+                                                                     uint32_t group_blocks, const PoolInst* inst,
+                                                                     const uint32_t* block_inst) {
   if (q.ws->done) return;
   const uint32_t cur = q.cal_st->cur;
   if (blockIdx.x < group_blocks) {
     cal::group_block(q, end_time, cur + 1u, blockIdx.x, group_blocks);
   } else {
-    const FusedModel fm{q.ws, m.count, m.hash, m.last_ts, 0u, m.num_lps, m.init_per_lp, m.seed, m.p_self,
-                        m.lookahead};
-    cal::commit_block(q, fm, end_time, cur, blockIdx.x - group_blocks);
+    const uint32_t b = blockIdx.x - group_blocks;
+    FusedModel fm{q.ws, m.count, m.hash, m.last_ts, 0u, m.num_lps, m.init_per_lp, m.seed, m.p_self, m.lookahead};
+    float et = end_time;
+    if (inst) {
+      const PoolInst in = inst[block_inst[b]];
+      fm.base = in.base;
+      fm.lps = in.lps;
+      fm.init_per_lp = in.init_per_lp;
+      fm.seed = in.seed;
+      fm.p_self = in.p_self;
+      fm.lookahead = in.lookahead;
+      et = in.end_time;
+    }
+    cal::commit_block(q, fm, et, cur, b);
   }
+}
+
+// Initial event j of every LP of several instances in one calendar-mode LAQ (the populate of
+// one instance alone, with the LP's index inside its instance): staged for queue_post_init,
+// or parked when it falls at or past the instance's end time.
+__global__ void populate_instances_kernel(LaqView q, const PoolInst* inst, const uint32_t* block_inst, uint32_t j) {
+  const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lp >= q.num_lps) return;
+  const PoolInst in = inst[block_inst[lp / cal::kBlock]];
+  const uint32_t local = lp - in.base;
+  if (local >= in.lps || j >= in.init_per_lp) return;
+  const uint4 r = philox4x32_10(make_uint4(j, local, 1u, 0u), make_uint2(in.seed, 0u));
+  const Event e{f_to_bits(exp_from_u32(r.x)), lp, local, j};
+  if (e.ts < f_to_bits(in.end_time)) q.insert(e);
+  else cal::append_one(q, 0u, 0u, cal::Route::kParked, e);
 }
 
 // phold_populate (src/phold.cu:63-79): initial event j of every LP.
@@ -317,7 +346,7 @@ gpq_status PholdModel::launch_fused(uint32_t settle) {
     if (settle) return GPQ_OK;
     const uint32_t g = cal::kStepGroupBlocks;
     hipLaunchKernelGGL(phold_calendar_kernel, dim3(g + ceil_div(cfg.num_lps, cal::kBlock)), dim3(cal::kBlock), 0,
-                       q->stream, laq->view(), dev_of(this), cfg.end_time, g);
+                       q->stream, laq->view(), dev_of(this), cfg.end_time, g, nullptr, nullptr);
     GPQ_HIP(hipGetLastError());
     return GPQ_OK;
   }
@@ -528,6 +557,31 @@ gpq_status PholdModel::profile(uint32_t batch, gpq_kernel_profile* out) {
     set_error("device error flags " + std::to_string(w.error));
     return GPQ_ERR_CAPACITY;
   }
+  return GPQ_OK;
+}
+
+gpq_status launch_calendar_instances(LaqQueue* q, const PoolInst* inst, const uint32_t* block_inst, uint32_t* count,
+                                     uint64_t* hash) {
+  GPQ_CHECK_ARG(q->calendar() && inst && block_inst, "calendar step over instances: needs a calendar-mode LAQ");
+  PholdDev d{};
+  d.count = count;
+  d.hash = hash;
+  d.ws = q->ws;
+  d.num_lps = q->cfg.num_lps;
+  d.drain = 2;
+  d.lookahead = q->cfg.lookahead;
+  const uint32_t g = cal::kStepGroupBlocks;
+  hipLaunchKernelGGL(phold_calendar_kernel, dim3(g + ceil_div(q->cfg.num_lps, cal::kBlock)), dim3(cal::kBlock), 0,
+                     q->stream, q->view(), d, q->cfg.horizon, g, inst, block_inst);
+  GPQ_HIP(hipGetLastError());
+  return GPQ_OK;
+}
+
+gpq_status launch_populate_instances(LaqQueue* q, const PoolInst* inst, const uint32_t* block_inst, uint32_t j) {
+  GPQ_CHECK_ARG(q->calendar() && inst && block_inst, "populate over instances: needs a calendar-mode LAQ");
+  hipLaunchKernelGGL(populate_instances_kernel, dim3(ceil_div(q->cfg.num_lps, 256)), dim3(256), 0, q->stream,
+                     q->view(), inst, block_inst, j);
+  GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }
 

@@ -14,14 +14,6 @@
 
 namespace gpq {
 
-struct PoolInst {
-  uint32_t base, lps, init_per_lp, seed;
-  float end_time, lookahead;
-  uint32_t p_self, active;
-  uint32_t block_begin, block_end;      // post blocks [block_begin, block_end) cover the range
-  uint32_t pad[2];
-};
-
 class PholdPool {
  public:
   // capacity_lps: LPs the pool can hold at once (rounded up to whole 256-LP blocks).
@@ -84,6 +76,12 @@ class PholdPool {
 // The batched entry point: every instance admitted at once into a pool sized for all of
 // them; GPQ_DRIVE_GRAPH advances them together, GPQ_DRIVE_PLAIN runs each alone through its
 // own LAQ with a host synchronization per step (the reference's des_run pattern).
+//
+// With calendar_windows set, both drives use calendar-mode LAQs and the merge-and-commit step
+// (drain 2).  Then GPQ_DRIVE_GRAPH puts all instances in one calendar LAQ on one window
+// sequence (window width: the smallest lookahead; the last window ends at the latest end
+// time): an instance's committed events are its events below its end time whatever the
+// windows, so its trace equals its run alone, and one kernel per step advances all of them.
 class PholdBatch {
  public:
   static gpq_status create(const gpq_phold_instance* inst, uint32_t count, uint32_t capacity_per_lp,
@@ -92,8 +90,11 @@ class PholdBatch {
   gpq_status results(uint64_t* hashes, uint64_t* events);
 
   std::vector<gpq_phold_instance> inst;
-  uint32_t capacity_per_lp = 128, stage_per_lp = 32, far_per_lp = 0;
+  uint32_t capacity_per_lp = 128, stage_per_lp = 32, far_per_lp = 0, calendar_windows = 0;
   std::vector<uint64_t> hash, events;
+
+ private:
+  gpq_status run_calendar_union(uint32_t batch, double* seconds);
 };
 
 }  // namespace gpq

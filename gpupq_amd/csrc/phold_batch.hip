@@ -233,6 +233,9 @@ __global__ void pool_init_kernel(WindowState* ws, uint32_t slots) {
   if (i < slots) ws[i] = WindowState{kInfBits, 0u, 1u, 0u, 0ull, 0u, 0u};
 }
 
+void fold_instances(const std::vector<PoolInst>& in, const uint32_t* counts, const uint64_t* hashes,
+                    uint64_t* trace_hash, uint64_t* events);
+
 }  // namespace
 
 uint64_t PholdPool::bytes_per_lp(uint32_t capacity_per_lp, uint32_t stage_per_lp, uint32_t far_per_lp) {
@@ -453,11 +456,19 @@ gpq_status PholdPool::retire_many(const std::vector<uint32_t>& slots, uint64_t* 
   GPQ_HIP(hipStreamSynchronize(q_->stream));
   for (const auto& r : in)
     for (uint32_t b = r.block_begin; b < r.block_end; ++b) block_used_[b] = 0;
-  // The trace hash of one instance is a sequential fold over its LPs (the single-instance
-  // definition); instances fold on separate host threads, largest first.
+  fold_instances(in, h_count_, h_hash_, trace_hash, events);
+  return GPQ_OK;
+}
+
+namespace {
+// The trace hash of one instance is a sequential fold over its LPs (the single-instance
+// definition); instances fold on separate host threads, largest first.  counts and hashes are
+// indexed by the LP's index in the shared queue.
+void fold_instances(const std::vector<PoolInst>& in, const uint32_t* counts, const uint64_t* hashes,
+                    uint64_t* trace_hash, uint64_t* events) {
   auto fold = [&](size_t k) {
-    const uint32_t* c = h_count_ + in[k].base;
-    const uint64_t* h = h_hash_ + in[k].base;
+    const uint32_t* c = counts + in[k].base;
+    const uint64_t* h = hashes + in[k].base;
     uint64_t x = 0x6A09E667F3BCC908ull, ev = 0;
     for (uint32_t i = 0; i < in[k].lps; ++i) {
       x = host_mix64_b(x ^ h[i]);
@@ -470,10 +481,10 @@ gpq_status PholdPool::retire_many(const std::vector<uint32_t>& slots, uint64_t* 
   uint64_t total = 0;
   for (const auto& r : in) total += r.lps;
   const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  const unsigned nthreads = (unsigned)std::min<uint64_t>({(uint64_t)hw, (uint64_t)slots.size(), total / 16384 + 1});
+  const unsigned nthreads = (unsigned)std::min<uint64_t>({(uint64_t)hw, (uint64_t)in.size(), total / 16384 + 1});
   if (nthreads <= 1) {
     for (size_t k = 0; k < in.size(); ++k) fold(k);
-    return GPQ_OK;
+    return;
   }
   std::vector<size_t> order(in.size());
   for (size_t k = 0; k < order.size(); ++k) order[k] = k;
@@ -486,8 +497,8 @@ gpq_status PholdPool::retire_many(const std::vector<uint32_t>& slots, uint64_t* 
       for (size_t i; (i = next.fetch_add(1)) < order.size();) fold(order[i]);
     });
   for (auto& t : pool) t.join();
-  return GPQ_OK;
 }
+}  // namespace
 
 // ---------------------------------------------------------------------------------------
 // PholdBatch
@@ -497,7 +508,6 @@ gpq_status PholdBatch::create(const gpq_phold_instance* in, uint32_t count, uint
   GPQ_CHECK_ARG(in && count >= 1, "phold batch: no instances");
   for (uint32_t i = 0; i < count; ++i) {
     GPQ_CHECK_ARG(in[i].num_lps >= 1 && in[i].init_per_lp >= 1, "phold batch: empty instance");
-    GPQ_CHECK_ARG(in[i].init_per_lp <= cap, "phold batch: init_per_lp exceeds capacity_per_lp");
     GPQ_CHECK_ARG(in[i].lookahead > 0.0f && in[i].end_time > 0.0f, "phold batch: lookahead/end_time must be > 0");
   }
   auto b = std::make_unique<PholdBatch>();
@@ -514,6 +524,10 @@ gpq_status PholdBatch::create(const gpq_phold_instance* in, uint32_t count, uint
 gpq_status PholdBatch::run(gpq_drive_mode mode, uint32_t batch, double* seconds) {
   const uint32_t n = (uint32_t)inst.size();
   double sec = 0.0;
+  if (!calendar_windows)      // the sorted ring holds every initial event of an LP
+    for (const auto& c : inst)
+      GPQ_CHECK_ARG(c.init_per_lp <= capacity_per_lp, "phold batch: init_per_lp exceeds capacity_per_lp");
+  if (mode != GPQ_DRIVE_PLAIN && calendar_windows) return run_calendar_union(batch, seconds);
   if (mode == GPQ_DRIVE_PLAIN) {
     for (uint32_t i = 0; i < n; ++i) {
       gpq_queue_config qc{};
@@ -525,11 +539,12 @@ gpq_status PholdBatch::run(gpq_drive_mode mode, uint32_t batch, double* seconds)
       qc.lookahead = inst[i].lookahead;
       qc.horizon = inst[i].end_time;
       qc.block_threads = 256;
+      qc.calendar_windows = calendar_windows;
       std::unique_ptr<QueueBase> q(make_queue(qc));
       GPQ_TRY(q->init_common());
       GPQ_TRY(q->init());
       gpq_phold_config pc{inst[i].num_lps, inst[i].init_per_lp, inst[i].seed, inst[i].lookahead,
-                          inst[i].end_time, inst[i].p_self_u32, 0u, 1u};
+                          inst[i].end_time, inst[i].p_self_u32, 0u, calendar_windows ? 2u : 1u};
       PholdModel* raw = nullptr;
       GPQ_TRY(PholdModel::create(q.get(), pc, &raw));
       std::unique_ptr<PholdModel> m(raw);
@@ -577,6 +592,130 @@ gpq_status PholdBatch::run(gpq_drive_mode mode, uint32_t batch, double* seconds)
       }
     }
     sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (seconds) *seconds = sec;
+  return GPQ_OK;
+}
+
+// All instances in one calendar-mode LAQ on one window sequence.  The timed part is the step
+// loop, as for each instance run alone (population and graph capture come before it, reading
+// back and folding the instances' counters and hashes after it).
+gpq_status PholdBatch::run_calendar_union(uint32_t batch, double* seconds) {
+  const uint32_t n = (uint32_t)inst.size();
+  std::vector<PoolInst> tab(n);
+  uint64_t total = 0;
+  uint32_t max_init = 0;
+  float la = inst[0].lookahead, horizon = inst[0].end_time;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t blocks = (uint32_t)ceil_div(inst[i].num_lps, kBlk);
+    PoolInst& t = tab[i];
+    t = PoolInst{};
+    t.base = (uint32_t)total;
+    t.lps = inst[i].num_lps;
+    t.init_per_lp = inst[i].init_per_lp;
+    t.seed = inst[i].seed;
+    t.end_time = inst[i].end_time;
+    t.lookahead = inst[i].lookahead;
+    t.p_self = inst[i].p_self_u32;
+    t.active = 1;
+    t.block_begin = (uint32_t)(total / kBlk);
+    t.block_end = t.block_begin + blocks;
+    total += (uint64_t)blocks * kBlk;
+    max_init = std::max(max_init, inst[i].init_per_lp);
+    la = std::min(la, inst[i].lookahead);
+    horizon = std::max(horizon, inst[i].end_time);
+  }
+  GPQ_CHECK_ARG(total < (1ull << 31), "phold batch: too many LPs for one queue");
+  GPQ_CHECK_ARG(far_per_lp >= max_init, "phold batch (calendar): far_per_lp must hold every initial event of an LP "
+                "(the parked list takes an LP's events past its end time)");
+  gpq_queue_config qc{};
+  qc.kind = GPQ_LAQ;
+  qc.num_lps = (uint32_t)total;
+  qc.capacity_per_lp = capacity_per_lp;
+  qc.stage_per_lp = stage_per_lp;
+  qc.far_per_lp = far_per_lp;
+  qc.lookahead = la;
+  qc.horizon = horizon;
+  qc.block_threads = 256;
+  qc.calendar_windows = calendar_windows;
+  std::unique_ptr<LaqQueue> q(new LaqQueue(qc));
+  GPQ_TRY(q->init_common());
+  GPQ_TRY(q->init());
+  DeviceArena arena;
+  PoolInst* d_inst = nullptr;
+  uint32_t* d_block = nullptr;
+  uint32_t* d_count = nullptr;
+  uint64_t* d_hash = nullptr;
+  const uint32_t nblocks = (uint32_t)(total / kBlk);
+  GPQ_TRY(arena.alloc(&d_inst, n));
+  GPQ_TRY(arena.alloc(&d_block, nblocks));
+  GPQ_TRY(arena.alloc(&d_count, (size_t)total));
+  GPQ_TRY(arena.alloc(&d_hash, (size_t)total));
+  std::vector<uint32_t> block_of(nblocks);
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t b = tab[i].block_begin; b < tab[i].block_end; ++b) block_of[b] = i;
+  GPQ_HIP(hipMemcpyAsync(d_inst, tab.data(), sizeof(PoolInst) * n, hipMemcpyHostToDevice, q->stream));
+  GPQ_HIP(hipMemcpyAsync(d_block, block_of.data(), sizeof(uint32_t) * nblocks, hipMemcpyHostToDevice, q->stream));
+  GPQ_HIP(hipMemsetAsync(d_count, 0, sizeof(uint32_t) * total, q->stream));
+  GPQ_HIP(hipMemsetAsync(d_hash, 0, sizeof(uint64_t) * total, q->stream));
+  for (uint32_t j = 0; j < max_init; ++j) {       // one initial event per LP per chunk, as one instance alone
+    GPQ_TRY(launch_populate_instances(q.get(), d_inst, d_block, j));
+    GPQ_TRY(q->post_init());
+  }
+  GPQ_TRY(q->calendar_start());
+  GPQ_HIP(hipStreamSynchronize(q->stream));
+  uint32_t flags = 0;
+  GPQ_TRY(q->error_flags(&flags));
+  if (flags) {
+    set_error("phold batch (calendar): queue overflow while populating (flags " + std::to_string(flags) + ")");
+    return GPQ_ERR_CAPACITY;
+  }
+  // `batch` steps captured once
+  struct GraphGuard {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~GraphGuard() {
+      if (x) (void)hipGraphExecDestroy(x);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } gg;
+  const uint32_t steps = batch ? batch : 16;
+  GPQ_HIP(hipStreamBeginCapture(q->stream, hipStreamCaptureModeThreadLocal));
+  gpq_status st = GPQ_OK;
+  for (uint32_t i = 0; i < steps && st == GPQ_OK; ++i) {
+    st = launch_calendar_instances(q.get(), d_inst, d_block, d_count, d_hash);
+    if (st == GPQ_OK) st = q->post();
+  }
+  const hipError_t e = hipStreamEndCapture(q->stream, &gg.g);
+  GPQ_TRY(st);
+  GPQ_HIP(e);
+  GPQ_HIP(hipGraphInstantiate(&gg.x, gg.g, nullptr, nullptr, 0));
+  struct Pinned {              // host staging of the counters and hashes
+    void* p = nullptr;
+    ~Pinned() {
+      if (p) (void)hipHostFree(p);
+    }
+  } pc, ph;
+  GPQ_HIP(hipHostMalloc(&pc.p, sizeof(uint32_t) * total, hipHostMallocDefault));
+  GPQ_HIP(hipHostMalloc(&ph.p, sizeof(uint64_t) * total, hipHostMallocDefault));
+  uint32_t* h_count = static_cast<uint32_t*>(pc.p);
+  uint64_t* h_hash = static_cast<uint64_t*>(ph.p);
+  const auto t0 = std::chrono::steady_clock::now();
+  WindowState w{};
+  for (;;) {
+    GPQ_TRY(q->window(&w));
+    if (w.done || w.error) break;
+    GPQ_HIP(hipGraphLaunch(gg.x, q->stream));
+  }
+  GPQ_HIP(hipStreamSynchronize(q->stream));
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  GPQ_HIP(hipMemcpyAsync(h_count, d_count, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, q->stream));
+  GPQ_HIP(hipMemcpyAsync(h_hash, d_hash, sizeof(uint64_t) * total, hipMemcpyDeviceToHost, q->stream));
+  GPQ_HIP(hipStreamSynchronize(q->stream));
+  fold_instances(tab, h_count, h_hash, hash.data(), events.data());
+  if (w.error) {
+    set_error("phold batch (calendar): device error flags " + std::to_string(w.error));
+    return GPQ_ERR_CAPACITY;
   }
   if (seconds) *seconds = sec;
   return GPQ_OK;

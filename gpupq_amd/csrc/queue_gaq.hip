@@ -120,7 +120,8 @@ gpq_status GaqQueue::init() {
 gpq_status GaqQueue::pre() {
   GaqState* st = v_.st;
   const SelectBufs b{first_, scratch_, &st->prefix, &st->deq_n, &st->rest_n};
-  GPQ_TRY(select_safe(&st->front, &st->n, cap_, &ws->wend, nullptr, nullptr, b, v_.deq, rest_, stream));
+  GPQ_TRY(select_safe(&st->front, &st->n, cap_, &ws->wend, nullptr, nullptr, b, v_.deq, rest_, stream,
+                      acct(kStatSelected)));
   hipLaunchKernelGGL(gaq_mark_selected_kernel, dim3(1), dim3(1), 0, stream, st);
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
@@ -128,10 +129,11 @@ gpq_status GaqQueue::pre() {
 
 gpq_status GaqQueue::post() {
   Event* sorted = nullptr;
-  GPQ_TRY(sort_records(v_.enq, sort_[0], sort_[1], &v_.st->m, v_.enq_cap, stream, &sorted, tile_));
+  GPQ_TRY(sort_records(v_.enq, sort_[0], sort_[1], &v_.st->m, v_.enq_cap, stream, &sorted, tile_,
+                       acct(kStatSorted)));
   hipLaunchKernelGGL(gaq_plan_kernel, dim3(1), dim3(1), 0, stream, v_.st, task_, sorted, rest_, cap_, v_.enq_cap, ws);
   GPQ_HIP(hipGetLastError());
-  GPQ_TRY(merge_tasks(task_, 1, (uint64_t)cap_, stream, tile_));
+  GPQ_TRY(merge_tasks(task_, 1, (uint64_t)cap_, stream, tile_, acct(kStatMerged)));
   hipLaunchKernelGGL(gaq_finish_kernel, dim3(1), dim3(1), 0, stream, v_.st, ws, cfg.lookahead, cfg.horizon);
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;

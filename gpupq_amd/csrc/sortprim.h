@@ -33,21 +33,25 @@ struct MergeTask {
 
 // Sorts *d_n (<= cap) records of `in` (not modified) into buf0/buf1; returns the buffer
 // holding the sorted run.  Passes: one tile sort, then ceil(log2(cap / tile)) merge passes.
+// `acct` (optional): a device counter that receives the records sorted (*d_n), for the
+// algorithmic-byte accounting of queues created with collect_stats.
 gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_t* d_n, uint32_t cap,
-                        hipStream_t stream, Event** sorted, uint32_t tile = kSortTile);
+                        hipStream_t stream, Event** sorted, uint32_t tile = kSortTile,
+                        unsigned long long* acct = nullptr);
 
 // Runs `ntasks` merge tasks (device array) in one launch; max_total bounds every task's
-// n_a1 + n_a2 + n_b.
+// n_a1 + n_a2 + n_b.  `acct` (optional) receives the records merged (the tasks' totals).
 gpq_status merge_tasks(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_total, hipStream_t stream,
-                       uint32_t tile = kMergeTile);
+                       uint32_t tile = kMergeTile, unsigned long long* acct = nullptr);
 
 // One safe event per LP over the sorted array s[0:*d_n), s = *d_s: the prefix of records with
 // ts < *d_wend and key <= *d_bound (when *d_bound_valid) is scanned, the first record of
 // each LP in it is moved to `deq` (in order, count -> *d_deq_n) and the others to `rest`
 // (in order, count -> *d_rest_n); *d_prefix receives the prefix length.  `first` (one
 // u32 per LP, all 0xFFFFFFFF between calls) and `scratch` (>= 2 * ceil(cap / 1024) + 2
-// u32) are workspace.  Reference: src/global_array_queue.cu:147-179 (serial scan) and
-// src/heap_queue.cu:478-547 (atomicMin per LP).
+// u32) are workspace; `acct` (optional) receives the prefix length.  Reference:
+// src/global_array_queue.cu:147-179 (serial scan) and src/heap_queue.cu:478-547 (atomicMin
+// per LP).
 struct SelectBufs {
   uint32_t* first;
   uint32_t* scratch;
@@ -57,7 +61,7 @@ struct SelectBufs {
 };
 gpq_status select_safe(const Event* const* d_s, const uint32_t* d_n, uint32_t cap, const uint32_t* d_wend,
                        const Event* d_bound, const uint32_t* d_bound_valid, const SelectBufs& b, Event* deq,
-                       Event* rest, hipStream_t stream);
+                       Event* rest, hipStream_t stream, unsigned long long* acct = nullptr);
 
 // Copies records: dst[i] = src[i] for i < *d_n (bounded by cap).
 gpq_status copy_records(const Event* src, Event* dst, const uint32_t* d_n, uint32_t cap, hipStream_t stream);

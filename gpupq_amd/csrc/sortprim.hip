@@ -405,6 +405,20 @@ uint32_t grid_of(uint64_t items, uint32_t per_block) {
   return g ? (uint32_t)g : 1u;
 }
 
+// Work accounting (queues created with collect_stats): one small launch after a primitive
+// adds the records it handled to a counter, so the primitive's own kernels are unchanged.
+__global__ void acct_count_kernel(const uint32_t* d_n, uint32_t cap, unsigned long long* acct) {
+  if (threadIdx.x == 0) atomicAdd(acct, (unsigned long long)min(*d_n, cap));
+}
+
+__global__ __launch_bounds__(256) void acct_tasks_kernel(const MergeTask* tasks, uint32_t ntasks,
+                                                         unsigned long long* acct) {
+  unsigned long long s = 0;
+  for (uint32_t t = threadIdx.x; t < ntasks; t += blockDim.x)
+    s += (unsigned long long)tasks[t].n_a1 + tasks[t].n_a2 + tasks[t].n_b;
+  if (s) atomicAdd(acct, s);
+}
+
 }  // namespace
 
 namespace {
@@ -441,19 +455,24 @@ uint32_t sort_passes(uint32_t cap, uint32_t tile) {
 }
 
 gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_t* d_n, uint32_t cap,
-                        hipStream_t stream, Event** sorted, uint32_t tile) {
+                        hipStream_t stream, Event** sorted, uint32_t tile, unsigned long long* acct) {
+  gpq_status s;
   switch (tile) {
-    case 1024: return sort_records_t<128>(in, buf0, buf1, d_n, cap, stream, sorted);
-    case 2048: return sort_records_t<256>(in, buf0, buf1, d_n, cap, stream, sorted);
-    case 4096: return sort_records_t<512>(in, buf0, buf1, d_n, cap, stream, sorted);
+    case 1024: s = sort_records_t<128>(in, buf0, buf1, d_n, cap, stream, sorted); break;
+    case 2048: s = sort_records_t<256>(in, buf0, buf1, d_n, cap, stream, sorted); break;
+    case 4096: s = sort_records_t<512>(in, buf0, buf1, d_n, cap, stream, sorted); break;
     default:
       set_error("sort tile must be 1024, 2048 or 4096 records");
       return GPQ_ERR_INVALID;
   }
+  if (s != GPQ_OK || !acct) return s;
+  hipLaunchKernelGGL(acct_count_kernel, dim3(1), dim3(64), 0, stream, d_n, cap, acct);
+  GPQ_HIP(hipGetLastError());
+  return GPQ_OK;
 }
 
 gpq_status merge_tasks(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_total, hipStream_t stream,
-                       uint32_t tile) {
+                       uint32_t tile, unsigned long long* acct) {
   if (ntasks == 0) return GPQ_OK;
   switch (tile) {
     case 1024:
@@ -470,12 +489,16 @@ gpq_status merge_tasks(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_t
       return GPQ_ERR_INVALID;
   }
   GPQ_HIP(hipGetLastError());
+  if (acct) {
+    hipLaunchKernelGGL(acct_tasks_kernel, dim3(1), dim3(256), 0, stream, d_tasks, ntasks, acct);
+    GPQ_HIP(hipGetLastError());
+  }
   return GPQ_OK;
 }
 
 gpq_status select_safe(const Event* const* s, const uint32_t* d_n, uint32_t cap, const uint32_t* d_wend,
                        const Event* d_bound, const uint32_t* d_bound_valid, const SelectBufs& b, Event* deq,
-                       Event* rest, hipStream_t stream) {
+                       Event* rest, hipStream_t stream, unsigned long long* acct) {
   const uint32_t nblk = grid_of(cap, kSelectTile);
   uint32_t* counts = b.scratch;
   uint32_t* offsets = b.scratch + nblk;
@@ -489,6 +512,10 @@ gpq_status select_safe(const Event* const* s, const uint32_t* d_n, uint32_t cap,
   hipLaunchKernelGGL(select_scatter_kernel, dim3(nblk), dim3(kThreads), 0, stream, s, b.d_prefix, b.first, offsets, deq,
                      rest);
   GPQ_HIP(hipGetLastError());
+  if (acct) {
+    hipLaunchKernelGGL(acct_count_kernel, dim3(1), dim3(64), 0, stream, b.d_prefix, cap, acct);
+    GPQ_HIP(hipGetLastError());
+  }
   return GPQ_OK;
 }
 

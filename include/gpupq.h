@@ -133,7 +133,10 @@ typedef struct {
   uint64_t displaced;       /* existing records moved to make room (d summed over LPs) */
   uint64_t merged_lps;      /* LP merges performed */
   uint64_t scanned_lps;     /* LP heads folded into the window reductions */
-  uint64_t aux_bytes;       /* kind-specific compulsory bytes not covered above (HQ/GAQ merges) */
+  uint64_t aux_bytes;       /* far-store moves of the two-tier LAQ (bytes) */
+  uint64_t sorted;          /* GAQ / HQ: records sorted (the step's inserts) */
+  uint64_t merged;          /* GAQ / HQ: records merged (a + b summed over the merges) */
+  uint64_t selected;        /* GAQ / HQ: records scanned by the safe-event selection */
 } gpq_queue_stats;
 gpq_status gpq_queue_stats_get(gpq_queue* q, gpq_queue_stats* out);
 
@@ -408,6 +411,13 @@ typedef struct gpq_phold_batch gpq_phold_batch;
  * (every instance with its own near horizon; capacity_per_lp <= 128). */
 gpq_status gpq_phold_batch_create(const gpq_phold_instance* inst, uint32_t count, uint32_t capacity_per_lp,
                                   uint32_t stage_per_lp, uint32_t far_per_lp, gpq_phold_batch** out);
+/* Calendar mode (0: off; else a power of two in [8, 256] windows): both drives use calendar-mode
+ * LAQs and the merge-and-commit step; capacity_per_lp is then a window's log records per LP and
+ * far_per_lp (>= every init_per_lp) the list capacity per LP.  GPQ_DRIVE_GRAPH puts all
+ * instances in one calendar LAQ on one window sequence (width: the smallest lookahead; last end:
+ * the latest end time), one kernel per step for all of them; each instance's committed events
+ * are still exactly its events below its own end time, so its results equal its run alone. */
+gpq_status gpq_phold_batch_set_calendar(gpq_phold_batch* b, uint32_t calendar_windows);
 gpq_status gpq_phold_batch_destroy(gpq_phold_batch* b);
 /* GPQ_DRIVE_GRAPH: all instances in one pool, `batch` steps per graph replay.
  * GPQ_DRIVE_PLAIN: each instance alone in its own queue, a host synchronization per step.

@@ -276,3 +276,30 @@ def test_sort_tile_autotune(gpq, oracle, kind):
 def test_sort_tile_rejected(gpq):
     with pytest.raises(gpq.GpqError):
         gpq.Queue("gaq", num_lps=64, capacity_per_lp=8, stage_per_lp=2, lookahead=0.1, horizon=1.0, sort_tile=3000)
+
+
+@pytest.mark.parametrize("kind", ["gaq", "hq"])
+def test_work_counters(gpq, oracle, kind):
+    # the counters behind the primitives' algorithmic bytes: every record inserted is sorted
+    # once (the L*k initial events and one successor per committed event), every merge sees at
+    # least the sorted inserts, and every committed event came from a scanned prefix
+    L, k, T = 2048, 8, 5.0
+    kw = dict(capacity_per_lp=64, stage_per_lp=2) if kind == "gaq" else \
+        dict(capacity_per_lp=40, stage_per_lp=1, node_size=L)
+    q = gpq.Queue(kind, num_lps=L, lookahead=0.1, horizon=T, collect_stats=True, **kw)
+    m = gpq.Phold(q, init_per_lp=k, seed=42, end_time=T)
+    m.model_init()
+    rep = gpq.des_run(m, mode="graph", batch=16)
+    st = q.stats()
+    th = m.trace_hash()
+    m.model_finish()
+    q.queue_finish()
+    res, _, _ = oracle.phold("seq", L, k, T)
+    assert rep["events"] == res.events and th == res.trace_hash
+    assert st["sorted"] == L * k + rep["events"]
+    assert st["merged"] >= st["sorted"]
+    assert st["selected"] >= rep["events"]
+    # without collect_stats the counters stay zero
+    with gpq.Queue(kind, num_lps=L, lookahead=0.1, horizon=T, **kw) as q2:
+        s2 = q2.stats()
+        assert s2["sorted"] == s2["merged"] == s2["selected"] == 0
