@@ -106,6 +106,8 @@ gpq_status GaqQueue::init() {
   GPQ_TRY(arena.alloc(&v_.deq, cfg.num_lps));
   GPQ_TRY(arena.alloc(&v_.st, 1));
   GPQ_TRY(arena.alloc(&task_, 1));
+  split_stride_ = merge_split_words(cap, tile_);
+  GPQ_TRY(arena.alloc(&splits_, split_stride_));
   GPQ_TRY(arena.alloc(&first_, cfg.num_lps));
   GPQ_TRY(arena.alloc(&scratch_, 2 * (size_t)ceil_div(cap, 1024) + 2));
   v_.ws = ws;
@@ -133,7 +135,7 @@ gpq_status GaqQueue::post() {
                        acct(kStatSorted)));
   hipLaunchKernelGGL(gaq_plan_kernel, dim3(1), dim3(1), 0, stream, v_.st, task_, sorted, rest_, cap_, v_.enq_cap, ws);
   GPQ_HIP(hipGetLastError());
-  GPQ_TRY(merge_tasks(task_, 1, (uint64_t)cap_, stream, tile_, acct(kStatMerged)));
+  GPQ_TRY(merge_tasks(task_, 1, (uint64_t)cap_, stream, tile_, acct(kStatMerged), splits_, split_stride_));
   hipLaunchKernelGGL(gaq_finish_kernel, dim3(1), dim3(1), 0, stream, v_.st, ws, cfg.lookahead, cfg.horizon);
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;

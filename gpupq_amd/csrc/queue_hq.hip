@@ -349,6 +349,8 @@ gpq_status HqQueue::init() {
   GPQ_TRY(arena.alloc(&first_, cfg.num_lps));   // by LP: the heaps' LP ranges are disjoint
   GPQ_TRY(arena.alloc(&scratch_, (size_t)heaps_ * scratch_n_));
   GPQ_TRY(arena.alloc(&tasks_, (size_t)heaps_ * tasks_n_));
+  split_stride_ = merge_split_words(std::max<uint64_t>(mcap_, 3ull * r_), tile_);
+  GPQ_TRY(arena.alloc(&splits_, (size_t)heaps_ * tasks_n_ * split_stride_));
   if (heaps_ > 1) {
     // one stream per heap (default priority), forked from and joined back into the queue's
     // stream twice per step; a hipGraph capture of the step records the branches
@@ -415,7 +417,7 @@ gpq_status HqQueue::merge_root(uint32_t h, hipStream_t s) {
                        acct(kStatSorted)));
   hipLaunchKernelGGL(hq_plan_merge_root, dim3(1), dim3(1), 0, s, st, counts_ + (size_t)h * nodes_,
                      nodes_buf_ + (size_t)h * nodes_ * r_, sorted, merged_ + (size_t)h * mcap_, v_.enq_cap, t);
-  GPQ_TRY(merge_tasks(t, 1, mcap_, s, tile_, acct(kStatMerged)));
+  GPQ_TRY(merge_tasks(t, 1, mcap_, s, tile_, acct(kStatMerged), splits_of(t), split_stride_));
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }
@@ -439,10 +441,10 @@ gpq_status HqQueue::step_heap(uint32_t h, bool select, hipStream_t s) {
                         v_.deq + (size_t)h * lps_per_heap_, rest, s, acct(kStatSelected)));
   }
   hipLaunchKernelGGL(hq_plan_root, dim3(1), dim3(1), 0, s, st, counts, nodes, merged, rest, carry, r_, t);
-  GPQ_TRY(merge_tasks(t, 1, mcap_, s, tile_, acct(kStatMerged)));
+  GPQ_TRY(merge_tasks(t, 1, mcap_, s, tile_, acct(kStatMerged), splits_of(t), split_stride_));
   hipLaunchKernelGGL(hq_plan_refill, dim3(1), dim3(1), 0, s, st, counts, nodes, dirty, r_, tmp1 + (size_t)slots_ * r_, t);
-  GPQ_TRY(merge_tasks(t, 1, r_, s, tile_, acct(kStatMerged)));
-  GPQ_TRY(merge_tasks(t + 1, 1, r_, s, tile_, acct(kStatMerged)));
+  GPQ_TRY(merge_tasks(t, 1, r_, s, tile_, acct(kStatMerged), splits_of(t), split_stride_));
+  GPQ_TRY(merge_tasks(t + 1, 1, r_, s, tile_, acct(kStatMerged), splits_of(t + 1), split_stride_));
   MergeTask* t1 = t;
   MergeTask* t2 = t + slots_;
   MergeTask* t3 = t + 2 * (size_t)slots_;
@@ -450,17 +452,17 @@ gpq_status HqQueue::step_heap(uint32_t h, bool select, hipStream_t s) {
     const uint32_t ns = 1u << lv;
     hipLaunchKernelGGL(hq_plan_level, dim3(ceil_div(ns, 256)), dim3(256), 0, s, st, counts, nodes, dirty, r_, nodes_,
                        lv, tmp2, tmp1, t1, t2, t3);
-    GPQ_TRY(merge_tasks(t1, ns, 2ull * r_, s, tile_, acct(kStatMerged)));
-    GPQ_TRY(merge_tasks(t2, ns, 3ull * r_, s, tile_, acct(kStatMerged)));
-    GPQ_TRY(merge_tasks(t3, ns, r_, s, tile_, acct(kStatMerged)));
+    GPQ_TRY(merge_tasks(t1, ns, 2ull * r_, s, tile_, acct(kStatMerged), splits_of(t1), split_stride_));
+    GPQ_TRY(merge_tasks(t2, ns, 3ull * r_, s, tile_, acct(kStatMerged), splits_of(t2), split_stride_));
+    GPQ_TRY(merge_tasks(t3, ns, r_, s, tile_, acct(kStatMerged), splits_of(t3), split_stride_));
   }
   hipLaunchKernelGGL(hq_plan_chunks, dim3(1), dim3(1), 0, s, st, counts, carry, chunk_buf, r_, nodes_, ws);
   const uint32_t chunks = std::min<uint32_t>(kHqMaxChunks, ceil_div(v_.enq_cap, r_) + 1);
   for (uint32_t c = 0; c < chunks; ++c) {
     for (uint32_t lv = 1; lv <= depth_; ++lv) {
       hipLaunchKernelGGL(hq_plan_insert_level, dim3(1), dim3(1), 0, s, st, counts, nodes, r_, c, lv, tmp1, t);
-      GPQ_TRY(merge_tasks(t, 1, 2ull * r_, s, tile_, acct(kStatMerged)));
-      GPQ_TRY(merge_tasks(t + 1, 1, 2ull * r_, s, tile_, acct(kStatMerged)));
+      GPQ_TRY(merge_tasks(t, 1, 2ull * r_, s, tile_, acct(kStatMerged), splits_of(t), split_stride_));
+      GPQ_TRY(merge_tasks(t + 1, 1, 2ull * r_, s, tile_, acct(kStatMerged), splits_of(t + 1), split_stride_));
     }
   }
   GPQ_HIP(hipGetLastError());

@@ -106,6 +106,8 @@ class GaqQueue final : public QueueBase {
   MergeTask* task_ = nullptr;
   uint32_t* first_ = nullptr;
   uint32_t* scratch_ = nullptr;
+  uint32_t* splits_ = nullptr;  // merge-path splits of the merge's tile boundaries
+  uint32_t split_stride_ = 0;
   uint32_t cap_ = 0;
   uint32_t tile_ = kSortTile;   // records per sort / merge block
 };
@@ -149,8 +151,12 @@ class HqQueue final : public QueueBase {
   uint32_t* first_ = nullptr;
   uint32_t* scratch_ = nullptr;
   MergeTask* tasks_ = nullptr;
+  uint32_t* splits_ = nullptr;  // merge-path splits, split_stride_ words per task slot
+  uint32_t split_stride_ = 0;
   uint64_t* sum_ = nullptr;
   uint32_t tile_ = kSortTile;   // records per sort / merge block
+  // the split buffer of task slot t (tasks_ + t)
+  uint32_t* splits_of(const MergeTask* t) const { return splits_ + (size_t)(t - tasks_) * split_stride_; }
 };
 
 }  // namespace gpq

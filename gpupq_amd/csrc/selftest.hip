@@ -109,6 +109,22 @@ extern "C" gpq_status gpq_selftest_merge(const gpq_event* a, uint32_t na, const 
   GPQ_HIP(hipMemcpy(dt.p, &t, sizeof(t), hipMemcpyHostToDevice));
   GPQ_TRY(gpq::merge_tasks(static_cast<gpq::MergeTask*>(dt.p), 1, total, 0));
   GPQ_HIP(hipMemcpy(out, dout.p, sizeof(Event) * total, hipMemcpyDeviceToHost));
+  // the same merge with its splits computed by the partition launch, for every tile size:
+  // it must give the same records
+  std::vector<Event> again(total);
+  for (uint32_t tile : {1024u, 2048u, 4096u}) {
+    const uint32_t stride = gpq::merge_split_words(total, tile);
+    DevBuf dsplit;
+    GPQ_HIP(hipMalloc(&dsplit.p, sizeof(uint32_t) * stride));
+    GPQ_HIP(hipMemset(dout.p, 0, sizeof(Event) * total));
+    GPQ_TRY(gpq::merge_tasks(static_cast<gpq::MergeTask*>(dt.p), 1, total, 0, tile, nullptr,
+                             static_cast<uint32_t*>(dsplit.p), stride));
+    GPQ_HIP(hipMemcpy(again.data(), dout.p, sizeof(Event) * total, hipMemcpyDeviceToHost));
+    if (std::memcmp(again.data(), out, sizeof(Event) * total) != 0) {
+      gpq::set_error("merge with precomputed splits differs (tile " + std::to_string(tile) + ")");
+      return GPQ_ERR_STATE;
+    }
+  }
   return GPQ_OK;
 }
 

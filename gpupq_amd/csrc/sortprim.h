@@ -41,8 +41,13 @@ gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_
 
 // Runs `ntasks` merge tasks (device array) in one launch; max_total bounds every task's
 // n_a1 + n_a2 + n_b.  `acct` (optional) receives the records merged (the tasks' totals).
+// `splits` (optional, ntasks * stride u32 with stride >= merge_split_words(max_total, tile)):
+// the merge-path splits of every tile boundary are computed by a partition launch first (one
+// wave per boundary), instead of by each merge block before it can load its input.
 gpq_status merge_tasks(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_total, hipStream_t stream,
-                       uint32_t tile = kMergeTile, unsigned long long* acct = nullptr);
+                       uint32_t tile = kMergeTile, unsigned long long* acct = nullptr, uint32_t* splits = nullptr,
+                       uint32_t stride = 0);
+uint32_t merge_split_words(uint64_t max_total, uint32_t tile);
 
 // One safe event per LP over the sorted array s[0:*d_n), s = *d_s: the prefix of records with
 // ts < *d_wend and key <= *d_bound (when *d_bound_valid) is scanned, the first record of

@@ -80,15 +80,46 @@ __device__ __forceinline__ void block_merge_splits(const ASrc& A, uint32_t na, c
 // consecutive threads merge start in different banks.
 __device__ __forceinline__ uint32_t pad8(uint32_t i) { return i + (i >> 3); }
 
+// The same split for one diagonal d by one wave: 64 probes per round, ceil(log_64 n) rounds.
+__device__ __forceinline__ uint32_t wave_merge_split(const ASrc& A, uint32_t na, const Event* B, uint32_t nb,
+                                                     uint32_t d) {
+  uint32_t lo = d > nb ? d - nb : 0u, hi = d < na ? d : na;
+  const uint32_t k = (uint32_t)lane_id();
+  while (lo < hi) {                                   // wave-uniform
+    const uint32_t step = (hi - lo + kWave - 1u) / kWave;
+    const uint32_t i = lo + k * step;
+    const bool p = i < hi && key_less(load_event(B + (d - 1u - i)), A.get(i));
+    const uint64_t m = __ballot(p);
+    if (m == 0) {
+      lo = lo + (hi - lo - 1u) / step * step + 1u;    // past the last probe
+    } else {
+      const uint32_t f = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+      hi = lo + f * step;
+      if (f) lo = lo + (f - 1u) * step + 1u;
+    }
+  }
+  return lo;
+}
+
+// With `pre` set, the block's splits were computed before the launch (merge_partition_kernel):
+// pre[0] and pre[1] are those of its first and last output rank.
 template <int kT>
-__device__ __forceinline__ void merge_tile(const ASrc& A, uint32_t na, const Event* B, uint32_t nb, uint32_t t0, const OutMap& out) {
+__device__ __forceinline__ void merge_tile(const ASrc& A, uint32_t na, const Event* B, uint32_t nb, uint32_t t0,
+                                           const OutMap& out, const uint32_t* pre = nullptr) {
   constexpr uint32_t kTile = kT * kPerThread;
   __shared__ Event s[kTile + kTile / 8];
   __shared__ uint32_t s_split[2];
   const uint32_t total = na + nb;
   const uint32_t t1 = t0 + kTile < total ? t0 + kTile : total;
-  block_merge_splits<kT>(A, na, B, nb, t0, t1, s_split);
-  const uint32_t i0 = s_split[0], i1 = s_split[1];
+  uint32_t i0, i1;
+  if (pre) {
+    i0 = pre[0];
+    i1 = pre[1];
+  } else {
+    block_merge_splits<kT>(A, na, B, nb, t0, t1, s_split);
+    i0 = s_split[0];
+    i1 = s_split[1];
+  }
   const uint32_t j0 = t0 - i0;
   const uint32_t ca = i1 - i0, cb = (t1 - t0) - ca;
   for (uint32_t x = threadIdx.x; x < ca; x += kT) s[pad8(x)] = A.get(i0 + x);
@@ -217,8 +248,11 @@ __global__ __launch_bounds__(kT) void merge_pass_kernel(const Event* in, Event* 
   merge_tile<kT>(A, na, in + base + na, nb, t0 - base, om);
 }
 
+// splits (optional): task y's splits at splits[y * stride + x], for tile x's first output rank
+// (x <= the task's tile count; the last one is that of the task's total).
 template <int kT>
-__global__ __launch_bounds__(kT) void merge_tasks_kernel(const MergeTask* tasks) {
+__global__ __launch_bounds__(kT) void merge_tasks_kernel(const MergeTask* tasks, const uint32_t* splits,
+                                                         uint32_t stride) {
   constexpr uint32_t kTile = kT * kPerThread;
   const MergeTask t = tasks[blockIdx.y];
   const uint32_t na = t.n_a1 + t.n_a2;
@@ -227,7 +261,26 @@ __global__ __launch_bounds__(kT) void merge_tasks_kernel(const MergeTask* tasks)
   if (t0 >= total) return;
   const ASrc A{t.a1, t.a2, t.n_a1};
   const OutMap om{{t.out[0], t.out[1], t.out[2]}, {t.end[0], t.end[1], t.end[2]}};
-  merge_tile<kT>(A, na, t.b, t.n_b, t0, om);
+  merge_tile<kT>(A, na, t.b, t.n_b, t0, om, splits ? splits + (size_t)blockIdx.y * stride + blockIdx.x : nullptr);
+}
+
+// The merge-path splits of every tile boundary of every task, one wave per boundary, so that
+// the merge blocks start loading at once.  Boundary x of task y is output rank min(x * kTile,
+// total); grid (ceil(boundaries / 4), ntasks) of 256 threads.
+template <int kT>
+__global__ __launch_bounds__(256) void merge_partition_kernel(const MergeTask* tasks, uint32_t boundaries,
+                                                              uint32_t* splits, uint32_t stride) {
+  constexpr uint32_t kTile = kT * kPerThread;
+  const uint32_t x = blockIdx.x * 4u + threadIdx.x / kWave;
+  if (x >= boundaries) return;
+  const MergeTask t = tasks[blockIdx.y];
+  const uint32_t na = t.n_a1 + t.n_a2;
+  const uint32_t total = na + t.n_b;
+  if ((uint64_t)x * kTile > total + (uint64_t)kTile) return;   // past the task's last tile boundary
+  const uint32_t d = (uint64_t)x * kTile < total ? x * kTile : total;
+  const ASrc A{t.a1, t.a2, t.n_a1};
+  const uint32_t s = wave_merge_split(A, na, t.b, t.n_b, d);
+  if (lane_id() == 0) splits[(size_t)blockIdx.y * stride + x] = s;
 }
 
 __global__ void copy_records_kernel(const Event* src, Event* dst, const uint32_t* d_n, uint32_t cap) {
@@ -471,19 +524,34 @@ gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_
   return GPQ_OK;
 }
 
+uint32_t merge_split_words(uint64_t max_total, uint32_t tile) {
+  return (uint32_t)((max_total + tile - 1) / tile) + 2u;
+}
+
+namespace {
+template <int kT>
+gpq_status merge_tasks_t(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_total, hipStream_t stream,
+                         uint32_t* splits, uint32_t stride) {
+  constexpr uint32_t kTile = kT * kPerThread;
+  const uint32_t gx = grid_of(max_total, kTile);
+  if (splits) {
+    GPQ_CHECK_ARG(stride >= gx + 1u, "merge_tasks: split buffer stride below the tile count + 1");
+    hipLaunchKernelGGL(merge_partition_kernel<kT>, dim3(ceil_div(gx + 1u, 4u), ntasks), dim3(256), 0, stream, d_tasks,
+                       gx + 1u, splits, stride);
+  }
+  hipLaunchKernelGGL(merge_tasks_kernel<kT>, dim3(gx, ntasks), dim3(kT), 0, stream, d_tasks,
+                     static_cast<const uint32_t*>(splits), stride);
+  return GPQ_OK;
+}
+}  // namespace
+
 gpq_status merge_tasks(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_total, hipStream_t stream,
-                       uint32_t tile, unsigned long long* acct) {
+                       uint32_t tile, unsigned long long* acct, uint32_t* splits, uint32_t stride) {
   if (ntasks == 0) return GPQ_OK;
   switch (tile) {
-    case 1024:
-      hipLaunchKernelGGL(merge_tasks_kernel<128>, dim3(grid_of(max_total, 1024), ntasks), dim3(128), 0, stream, d_tasks);
-      break;
-    case 2048:
-      hipLaunchKernelGGL(merge_tasks_kernel<256>, dim3(grid_of(max_total, 2048), ntasks), dim3(256), 0, stream, d_tasks);
-      break;
-    case 4096:
-      hipLaunchKernelGGL(merge_tasks_kernel<512>, dim3(grid_of(max_total, 4096), ntasks), dim3(512), 0, stream, d_tasks);
-      break;
+    case 1024: GPQ_TRY(merge_tasks_t<128>(d_tasks, ntasks, max_total, stream, splits, stride)); break;
+    case 2048: GPQ_TRY(merge_tasks_t<256>(d_tasks, ntasks, max_total, stream, splits, stride)); break;
+    case 4096: GPQ_TRY(merge_tasks_t<512>(d_tasks, ntasks, max_total, stream, splits, stride)); break;
     default:
       set_error("merge tile must be 1024, 2048 or 4096 records");
       return GPQ_ERR_INVALID;
