@@ -97,10 +97,14 @@ gpq_status GaqQueue::init() {
   GPQ_CHECK_ARG(valid_sort_tile(tile_), "GAQ: sort_tile must be 0, 1024, 2048 or 4096");
   v_.num_lps = cfg.num_lps;
   v_.enq_cap = (uint32_t)enq_cap;
+  v_.seg_cap = insert_stripe_cap(enq_cap);
   GPQ_TRY(arena.alloc(&fel_[0], cap));
   GPQ_TRY(arena.alloc(&fel_[1], cap));
   GPQ_TRY(arena.alloc(&rest_, cap));
-  GPQ_TRY(arena.alloc(&v_.enq, enq_cap));
+  GPQ_TRY(arena.alloc(&v_.enq, (size_t)kInsertStripes * v_.seg_cap + enq_cap));
+  GPQ_TRY(arena.alloc(&v_.seg_m, kInsertStripes + 1));
+  GPQ_TRY(arena.alloc(&seg_pre_, kInsertStripes + 2));
+  GPQ_HIP(hipMemsetAsync(v_.seg_m, 0, sizeof(uint32_t) * (kInsertStripes + 1), stream));
   GPQ_TRY(arena.alloc(&sort_[0], enq_cap));
   GPQ_TRY(arena.alloc(&sort_[1], enq_cap));
   GPQ_TRY(arena.alloc(&v_.deq, cfg.num_lps));
@@ -131,8 +135,9 @@ gpq_status GaqQueue::pre() {
 
 gpq_status GaqQueue::post() {
   Event* sorted = nullptr;
+  GPQ_TRY(gather_inserts(v_.seg_m, v_.seg_cap, v_.enq_cap, v_.enq_cap, seg_pre_, &v_.st->m, ws, stream));
   GPQ_TRY(sort_records(v_.enq, sort_[0], sort_[1], &v_.st->m, v_.enq_cap, stream, &sorted, tile_,
-                       acct(kStatSorted)));
+                       acct(kStatSorted), seg_pre_, v_.seg_cap));
   hipLaunchKernelGGL(gaq_plan_kernel, dim3(1), dim3(1), 0, stream, v_.st, task_, sorted, rest_, cap_, v_.enq_cap, ws);
   GPQ_HIP(hipGetLastError());
   GPQ_TRY(merge_tasks(task_, 1, (uint64_t)cap_, stream, tile_, acct(kStatMerged), splits_, split_stride_));

@@ -108,6 +108,7 @@ class GaqQueue final : public QueueBase {
   uint32_t* scratch_ = nullptr;
   uint32_t* splits_ = nullptr;  // merge-path splits of the merge's tile boundaries
   uint32_t split_stride_ = 0;
+  uint32_t* seg_pre_ = nullptr; // segment offsets of the step's striped inserts
   uint32_t cap_ = 0;
   uint32_t tile_ = kSortTile;   // records per sort / merge block
 };
@@ -153,6 +154,7 @@ class HqQueue final : public QueueBase {
   MergeTask* tasks_ = nullptr;
   uint32_t* splits_ = nullptr;  // merge-path splits, split_stride_ words per task slot
   uint32_t split_stride_ = 0;
+  uint32_t* seg_pre_ = nullptr; // per heap, segment offsets of the step's striped inserts
   uint64_t* sum_ = nullptr;
   uint32_t tile_ = kSortTile;   // records per sort / merge block
   // the split buffer of task slot t (tasks_ + t)

@@ -35,9 +35,24 @@ struct MergeTask {
 // holding the sorted run.  Passes: one tile sort, then ceil(log2(cap / tile)) merge passes.
 // `acct` (optional): a device counter that receives the records sorted (*d_n), for the
 // algorithmic-byte accounting of queues created with collect_stats.
+// `seg_pre` (optional): `in` is a striped insert area (core.cuh) with seg_cap-record stripes,
+// its segments' records laid out by gather_inserts (*d_n = their total).
 gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_t* d_n, uint32_t cap,
                         hipStream_t stream, Event** sorted, uint32_t tile = kSortTile,
-                        unsigned long long* acct = nullptr);
+                        unsigned long long* acct = nullptr, const uint32_t* seg_pre = nullptr, uint32_t seg_cap = 0);
+
+// The step's inserts in a striped insert area (counters seg_m[0 .. kInsertStripes], stripes of
+// seg_cap records, spill segment of spill_cap): seg_pre[k] = records in the segments before k
+// (kInsertStripes + 2 words), *d_m = their total, capped at max_total (beyond it the queue's
+// error flag is set and the last segments are cut short); the counters are reset to zero.
+// Records per stripe of an insert area for `enq_cap` inserts per step: twice an even share, at
+// least 1024 (a stripe that fills up spills into the shared segment of enq_cap records).
+inline uint32_t insert_stripe_cap(uint64_t enq_cap) {
+  const uint64_t c = 2 * ((enq_cap + kInsertStripes - 1) / kInsertStripes);
+  return (uint32_t)(c < 1024 ? 1024 : c);
+}
+gpq_status gather_inserts(uint32_t* seg_m, uint32_t seg_cap, uint32_t spill_cap, uint32_t max_total, uint32_t* seg_pre,
+                          uint32_t* d_m, WindowState* ws, hipStream_t stream);
 
 // Runs `ntasks` merge tasks (device array) in one launch; max_total bounds every task's
 // n_a1 + n_a2 + n_b.  `acct` (optional) receives the records merged (the tasks' totals).

@@ -169,15 +169,37 @@ __device__ __forceinline__ void compare_swap(Event& a, Event& b, bool up) {
 // the stages with j >= 8 run in LDS, each thread exchanging 4 of the tile's 1024 pairs.  The
 // LDS tile is padded one record per 8 (pad8) so that the 8-record runs of neighbouring threads
 // start in different banks.
+// seg_pre (optional): `in` is a striped insert area (core.cuh) whose segment k holds records
+// [seg_pre[k], seg_pre[k + 1]) of the run (k < kInsertStripes: at k * seg_cap; the spill
+// segment k = kInsertStripes after the stripes).
 template <int kT>
-__global__ __launch_bounds__(kT) void tile_sort_kernel(const Event* in, Event* out, const uint32_t* d_n) {
+__global__ __launch_bounds__(kT) void tile_sort_kernel(const Event* in, Event* out, const uint32_t* d_n,
+                                                       const uint32_t* seg_pre, uint32_t seg_cap) {
   constexpr uint32_t kTile = kT * kPerThread;
   __shared__ Event s[kTile + kTile / 8];
+  __shared__ uint32_t s_pre[kInsertStripes + 2];
   const uint32_t n = *d_n;
   const uint32_t base = blockIdx.x * kTile;
   if (base >= n) return;
-  for (uint32_t i = threadIdx.x; i < kTile; i += kT)
-    s[pad8(i)] = base + i < n ? load_event(in + base + i) : sentinel();
+  if (seg_pre) {
+    for (uint32_t k = threadIdx.x; k < kInsertStripes + 2u; k += kT) s_pre[k] = seg_pre[k];
+    __syncthreads();
+  }
+  for (uint32_t i = threadIdx.x; i < kTile; i += kT) {
+    const uint32_t g = base + i;
+    if (g >= n) {
+      s[pad8(i)] = sentinel();
+    } else if (!seg_pre) {
+      s[pad8(i)] = load_event(in + g);
+    } else {
+      uint32_t lo = 0, hi = kInsertStripes;          // the last segment k with s_pre[k] <= g
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (s_pre[mid] <= g) lo = mid; else hi = mid - 1u;
+      }
+      s[pad8(i)] = load_event(in + (size_t)lo * seg_cap + (g - s_pre[lo]));
+    }
+  }
   __syncthreads();
   const uint32_t t0 = threadIdx.x * 8u;
   Event r[8];
@@ -281,6 +303,35 @@ __global__ __launch_bounds__(256) void merge_partition_kernel(const MergeTask* t
   const ASrc A{t.a1, t.a2, t.n_a1};
   const uint32_t s = wave_merge_split(A, na, t.b, t.n_b, d);
   if (lane_id() == 0) splits[(size_t)blockIdx.y * stride + x] = s;
+}
+
+// One wave: lane k < kInsertStripes + 1 takes segment k's record count (a stripe counter past
+// seg_cap also counts the lanes that spilled), the segments' offsets are a wave prefix sum,
+// and every offset is capped at max_total.
+__global__ __launch_bounds__(64) void gather_inserts_kernel(uint32_t* seg_m, uint32_t seg_cap, uint32_t spill_cap,
+                                                            uint32_t max_total, uint32_t* seg_pre, uint32_t* d_m,
+                                                            WindowState* ws) {
+  const uint32_t k = threadIdx.x;
+  uint32_t n = 0;
+  if (k <= kInsertStripes) {
+    const uint32_t c = seg_m[k], cap = k < kInsertStripes ? seg_cap : spill_cap;
+    n = c < cap ? c : cap;
+    seg_m[k] = 0;
+  }
+  uint32_t inc = n;
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)inc, off, kWave);
+    if (k >= (uint32_t)off) inc += t;
+  }
+  const uint32_t exc = inc - n;
+  const uint32_t total = (uint32_t)__shfl((int)inc, (int)kInsertStripes, kWave);
+  const uint32_t m = total < max_total ? total : max_total;
+  if (k <= kInsertStripes) seg_pre[k] = exc < max_total ? exc : max_total;
+  if (k == 0) {
+    seg_pre[kInsertStripes + 1] = m;
+    *d_m = m;
+    if (total > max_total) atomicOr(&ws->error, kErrQueueOverflow);
+  }
 }
 
 __global__ void copy_records_kernel(const Event* src, Event* dst, const uint32_t* d_n, uint32_t cap) {
@@ -477,9 +528,10 @@ __global__ __launch_bounds__(256) void acct_tasks_kernel(const MergeTask* tasks,
 namespace {
 template <int kT>
 gpq_status sort_records_t(const Event* in, Event* buf0, Event* buf1, const uint32_t* d_n, uint32_t cap,
-                          hipStream_t stream, Event** sorted) {
+                          hipStream_t stream, Event** sorted, const uint32_t* seg_pre, uint32_t seg_cap) {
   constexpr uint32_t kTile = kT * kPerThread;
-  hipLaunchKernelGGL(tile_sort_kernel<kT>, dim3(grid_of(cap, kTile)), dim3(kT), 0, stream, in, buf0, d_n);
+  hipLaunchKernelGGL(tile_sort_kernel<kT>, dim3(grid_of(cap, kTile)), dim3(kT), 0, stream, in, buf0, d_n, seg_pre,
+                     seg_cap);
   GPQ_HIP(hipGetLastError());
   Event* cur = buf0;
   Event* other = buf1;
@@ -508,12 +560,13 @@ uint32_t sort_passes(uint32_t cap, uint32_t tile) {
 }
 
 gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_t* d_n, uint32_t cap,
-                        hipStream_t stream, Event** sorted, uint32_t tile, unsigned long long* acct) {
+                        hipStream_t stream, Event** sorted, uint32_t tile, unsigned long long* acct,
+                        const uint32_t* seg_pre, uint32_t seg_cap) {
   gpq_status s;
   switch (tile) {
-    case 1024: s = sort_records_t<128>(in, buf0, buf1, d_n, cap, stream, sorted); break;
-    case 2048: s = sort_records_t<256>(in, buf0, buf1, d_n, cap, stream, sorted); break;
-    case 4096: s = sort_records_t<512>(in, buf0, buf1, d_n, cap, stream, sorted); break;
+    case 1024: s = sort_records_t<128>(in, buf0, buf1, d_n, cap, stream, sorted, seg_pre, seg_cap); break;
+    case 2048: s = sort_records_t<256>(in, buf0, buf1, d_n, cap, stream, sorted, seg_pre, seg_cap); break;
+    case 4096: s = sort_records_t<512>(in, buf0, buf1, d_n, cap, stream, sorted, seg_pre, seg_cap); break;
     default:
       set_error("sort tile must be 1024, 2048 or 4096 records");
       return GPQ_ERR_INVALID;
@@ -584,6 +637,15 @@ gpq_status select_safe(const Event* const* s, const uint32_t* d_n, uint32_t cap,
     hipLaunchKernelGGL(acct_count_kernel, dim3(1), dim3(64), 0, stream, b.d_prefix, cap, acct);
     GPQ_HIP(hipGetLastError());
   }
+  return GPQ_OK;
+}
+
+gpq_status gather_inserts(uint32_t* seg_m, uint32_t seg_cap, uint32_t spill_cap, uint32_t max_total, uint32_t* seg_pre,
+                          uint32_t* d_m, WindowState* ws, hipStream_t stream) {
+  static_assert(kInsertStripes + 1 <= kWave, "one wave gathers the stripes");
+  hipLaunchKernelGGL(gather_inserts_kernel, dim3(1), dim3(kWave), 0, stream, seg_m, seg_cap, spill_cap, max_total,
+                     seg_pre, d_m, ws);
+  GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }
 

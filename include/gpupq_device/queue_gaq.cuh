@@ -18,7 +18,7 @@ struct GaqState {
   const Event* front;   // fel[cur]
   uint32_t cur;
   uint32_t n;           // records in fel[cur]
-  uint32_t m;           // inserts appended this step
+  uint32_t m;           // inserts of this step (totalled over the stripes by queue_post)
   uint32_t prefix;      // records scanned by the last selection
   uint32_t deq_n;       // records in the dequeue buffer
   uint32_t rest_n;      // unselected records of the scanned prefix
@@ -27,24 +27,23 @@ struct GaqState {
 };
 
 struct GaqView {
-  Event* enq;
+  Event* enq;           // striped insert area (core.cuh): kInsertStripes * seg_cap + enq_cap records
   Event* deq;
   GaqState* st;
   WindowState* ws;
-  uint32_t enq_cap;
+  uint32_t* seg_m;      // kInsertStripes + 1 append counters
+  uint32_t enq_cap;     // inserts one step may hold (and the spill segment's size)
+  uint32_t seg_cap;
   uint32_t num_lps;
+  uint32_t pad;
 
   __device__ __forceinline__ uint32_t slots() const { return num_lps; }
 
-  // queue_insert: appended to the step's buffer (one counter atomic per wave).
+  // queue_insert: appended to the step's striped insert area (one counter atomic per wave).
   __device__ __forceinline__ bool insert(const Event& e) const {
-    const uint32_t pos = wave_append_slot(&st->m);
-    if (pos >= enq_cap) {
-      atomicOr(&ws->error, kErrQueueOverflow);
-      return false;
-    }
-    store_event(enq + pos, e);
-    return true;
+    if (striped_append(enq, seg_m, seg_cap, enq_cap, e)) return true;
+    atomicOr(&ws->error, kErrQueueOverflow);
+    return false;
   }
   // queue_peek(item, pos): dequeue-buffer slot pos (src/global_array_queue.cu:327-341).
   __device__ __forceinline__ int peek(Event** item, int pos) const {

@@ -17,7 +17,7 @@ constexpr int kHqMaxChunks = 8;
 
 struct HqState {
   uint32_t last;           // highest node holding records
-  uint32_t m;              // inserts appended this step
+  uint32_t m;              // inserts of this step (totalled over the stripes by queue_post)
   uint32_t mcount;         // records in the merged buffer
   uint32_t prefix, deq_n, rest_n;
   uint32_t bound_valid;
@@ -36,43 +36,39 @@ struct HqState {
 // Several heaps (HQ_NUM_QUEUES): heap h owns the LPs [h * lps_per_heap, (h + 1) * lps_per_heap),
 // its state st[h], its insert area enq + h * enq_cap and its dequeue slots deq + h * lps_per_heap.
 struct HqView {
-  Event* enq;
+  Event* enq;              // heap h's striped insert area (core.cuh) at enq + h * area
   Event* deq;
   HqState* st;
   WindowState* ws;
-  uint32_t enq_cap;        // per heap
+  uint32_t* seg_m;         // heap h's kInsertStripes + 1 append counters at seg_m + h * (kInsertStripes + 1)
+  uint32_t enq_cap;        // per heap: inserts one step may hold (and the spill segment's size)
+  uint32_t seg_cap;
   uint32_t num_lps;
   uint32_t heaps;
   uint32_t lps_per_heap;
+  uint32_t pad;
+  uint64_t area;           // records of one heap's insert area: kInsertStripes * seg_cap + enq_cap
 
   __device__ __forceinline__ uint32_t slots() const { return num_lps; }
   // heap_queue_insert (src/heap_queue.cu:1809-1826): into the insert area of the LP's heap,
-  // one counter atomic per wave and heap (every active lane of the wave takes part).
+  // one counter atomic per wave, stripe and heap (every active lane of the wave takes part).
   __device__ __forceinline__ bool insert(const Event& e) const {
-    uint32_t h = 0, pos = 0;
+    bool ok = true;
     if (heaps == 1) {
-      pos = wave_append_slot(&st->m);
+      ok = striped_append(enq, seg_m, seg_cap, enq_cap, e);
     } else {
-      h = e.dst / lps_per_heap;
-      const int lane = lane_id();
+      const uint32_t h = e.dst / lps_per_heap;
       uint64_t pending = __ballot(1);
       while (pending) {                                  // one round per heap present in the wave
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lh = (uint32_t)__shfl((int)h, leader, kWave);
         const uint64_t same = __ballot(h == lh) & pending;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&st[lh].m, (uint32_t)__popcll(same));
-        base = (uint32_t)__shfl((int)base, leader, kWave);
-        if (h == lh) pos = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        if (h == lh) ok = striped_append(enq + lh * area, seg_m + lh * (kInsertStripes + 1u), seg_cap, enq_cap, e);
         pending &= ~same;
       }
     }
-    if (pos >= enq_cap) {
-      atomicOr(&ws->error, kErrStageOverflow);
-      return false;
-    }
-    store_event(enq + (size_t)h * enq_cap + pos, e);
-    return true;
+    if (!ok) atomicOr(&ws->error, kErrStageOverflow);
+    return ok;
   }
   // heap_queue_peek(item, pos): dequeue slot pos, i.e. slot pos % lps_per_heap of heap
   // pos / lps_per_heap (src/heap_queue.cu:1860-1896).
