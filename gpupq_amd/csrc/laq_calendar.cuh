@@ -34,12 +34,16 @@
 namespace gpq {
 namespace cal {
 
-constexpr int kBlock = 256;       // LPs per commit block
+// LPs per commit block.  Every commit block adds its run to the counter of each window it
+// files successors for (about 40 of them per step), so fewer, larger blocks mean less
+// contention on those counters: 256 LPs 1.12e10, 512 LPs 1.25-1.28e10, 1024 LPs 1.25-1.28e10
+// events/s at 2^20 LPs (same box, the staging threads kept at 98 K).
+constexpr int kBlock = 512;
 constexpr int kRegs = 8;          // staged records one thread sorts in registers (fast path)
 constexpr int kWaveCand = 64;     // staged records of one LP on the slow path (one per lane)
 constexpr int kGroupBlocks = 2048;        // blocks of the standalone filing / staging launches
-constexpr uint32_t kStepGroupBlocks = 384;   // staging blocks of a step (beside the commit blocks; 384 measured
-                                             // best of 128, 256, 384, 512, 1024, 2048)
+constexpr uint32_t kStepGroupBlocks = 192;   // staging blocks of a step (beside the commit blocks; with 256-thread
+                                             // blocks 384 measured best of 128, 256, 384, 512, 1024, 2048)
 
 struct Key {
   uint32_t ts, src, seq;

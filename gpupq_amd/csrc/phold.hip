@@ -131,6 +131,7 @@ __global__ __launch_bounds__(kFusedBlock, 4) void phold_fused_kernel(LaqView q, 
 // inst[block_inst[b]] (several instances in one queue, every 256-LP block in one instance).
 // (The staging blocks run first and the commit blocks fill the device beside them; the same
 // two parts as kernels on two streams measured 7% slower.)
+static_assert(cal::kBlock == (int)LaqQueue::kCalBlockLps, "calendar commit block size");
 __global__ __launch_bounds__(cal::kBlock) void phold_calendar_kernel(LaqView q, PholdDev m, float end_time,
                                                                      uint32_t group_blocks, const PoolInst* inst,
                                                                      const uint32_t* block_inst) {

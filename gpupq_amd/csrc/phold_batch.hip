@@ -602,12 +602,13 @@ gpq_status PholdBatch::run(gpq_drive_mode mode, uint32_t batch, double* seconds)
 // back and folding the instances' counters and hashes after it).
 gpq_status PholdBatch::run_calendar_union(uint32_t batch, double* seconds) {
   const uint32_t n = (uint32_t)inst.size();
+  constexpr uint32_t kCalBlk = LaqQueue::kCalBlockLps;   // an instance owns whole commit blocks
   std::vector<PoolInst> tab(n);
   uint64_t total = 0;
   uint32_t max_init = 0;
   float la = inst[0].lookahead, horizon = inst[0].end_time;
   for (uint32_t i = 0; i < n; ++i) {
-    const uint32_t blocks = (uint32_t)ceil_div(inst[i].num_lps, kBlk);
+    const uint32_t blocks = (uint32_t)ceil_div(inst[i].num_lps, kCalBlk);
     PoolInst& t = tab[i];
     t = PoolInst{};
     t.base = (uint32_t)total;
@@ -618,9 +619,9 @@ gpq_status PholdBatch::run_calendar_union(uint32_t batch, double* seconds) {
     t.lookahead = inst[i].lookahead;
     t.p_self = inst[i].p_self_u32;
     t.active = 1;
-    t.block_begin = (uint32_t)(total / kBlk);
+    t.block_begin = (uint32_t)(total / kCalBlk);
     t.block_end = t.block_begin + blocks;
-    total += (uint64_t)blocks * kBlk;
+    total += (uint64_t)blocks * kCalBlk;
     max_init = std::max(max_init, inst[i].init_per_lp);
     la = std::min(la, inst[i].lookahead);
     horizon = std::max(horizon, inst[i].end_time);
@@ -646,7 +647,7 @@ gpq_status PholdBatch::run_calendar_union(uint32_t batch, double* seconds) {
   uint32_t* d_block = nullptr;
   uint32_t* d_count = nullptr;
   uint64_t* d_hash = nullptr;
-  const uint32_t nblocks = (uint32_t)(total / kBlk);
+  const uint32_t nblocks = (uint32_t)(total / kCalBlk);
   GPQ_TRY(arena.alloc(&d_inst, n));
   GPQ_TRY(arena.alloc(&d_block, nblocks));
   GPQ_TRY(arena.alloc(&d_count, (size_t)total));
