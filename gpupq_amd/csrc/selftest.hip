@@ -118,7 +118,7 @@ extern "C" gpq_status gpq_selftest_merge(const gpq_event* a, uint32_t na, const 
     GPQ_HIP(hipMalloc(&dsplit.p, sizeof(uint32_t) * stride));
     GPQ_HIP(hipMemset(dout.p, 0, sizeof(Event) * total));
     GPQ_TRY(gpq::merge_tasks(static_cast<gpq::MergeTask*>(dt.p), 1, total, 0, tile, nullptr,
-                             static_cast<uint32_t*>(dsplit.p), stride));
+                             static_cast<uint32_t*>(dsplit.p), stride, 0));
     GPQ_HIP(hipMemcpy(again.data(), dout.p, sizeof(Event) * total, hipMemcpyDeviceToHost));
     if (std::memcmp(again.data(), out, sizeof(Event) * total) != 0) {
       gpq::set_error("merge with precomputed splits differs (tile " + std::to_string(tile) + ")");

@@ -41,16 +41,17 @@ gpq_status sort_records(const Event* in, Event* buf0, Event* buf1, const uint32_
                         hipStream_t stream, Event** sorted, uint32_t tile = kSortTile,
                         unsigned long long* acct = nullptr, const uint32_t* seg_pre = nullptr, uint32_t seg_cap = 0);
 
-// The step's inserts in a striped insert area (counters seg_m[0 .. kInsertStripes], stripes of
-// seg_cap records, spill segment of spill_cap): seg_pre[k] = records in the segments before k
-// (kInsertStripes + 2 words), *d_m = their total, capped at max_total (beyond it the queue's
-// error flag is set and the last segments are cut short); the counters are reset to zero.
 // Records per stripe of an insert area for `enq_cap` inserts per step: twice an even share, at
 // least 1024 (a stripe that fills up spills into the shared segment of enq_cap records).
 inline uint32_t insert_stripe_cap(uint64_t enq_cap) {
   const uint64_t c = 2 * ((enq_cap + kInsertStripes - 1) / kInsertStripes);
   return (uint32_t)(c < 1024 ? 1024 : c);
 }
+
+// The step's inserts in a striped insert area (counters seg_m[0 .. kInsertStripes], stripes of
+// seg_cap records, spill segment of spill_cap): seg_pre[k] = records in the segments before k
+// (kInsertStripes + 2 words), *d_m = their total, capped at max_total (beyond it the queue's
+// error flag is set and the last segments are cut short); the counters are reset to zero.
 gpq_status gather_inserts(uint32_t* seg_m, uint32_t seg_cap, uint32_t spill_cap, uint32_t max_total, uint32_t* seg_pre,
                           uint32_t* d_m, WindowState* ws, hipStream_t stream);
 
@@ -59,9 +60,12 @@ gpq_status gather_inserts(uint32_t* seg_m, uint32_t seg_cap, uint32_t spill_cap,
 // `splits` (optional, ntasks * stride u32 with stride >= merge_split_words(max_total, tile)):
 // the merge-path splits of every tile boundary are computed by a partition launch first (one
 // wave per boundary), instead of by each merge block before it can load its input.
+// Merges whose max_total is below partition_min search their splits in the merge blocks even
+// with `splits` given.
+constexpr uint64_t kMergePartitionMin = 1ull << 20;
 gpq_status merge_tasks(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max_total, hipStream_t stream,
                        uint32_t tile = kMergeTile, unsigned long long* acct = nullptr, uint32_t* splits = nullptr,
-                       uint32_t stride = 0);
+                       uint32_t stride = 0, uint64_t partition_min = kMergePartitionMin);
 uint32_t merge_split_words(uint64_t max_total, uint32_t tile);
 
 // One safe event per LP over the sorted array s[0:*d_n), s = *d_s: the prefix of records with
