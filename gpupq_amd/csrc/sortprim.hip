@@ -9,6 +9,7 @@ constexpr int kThreads = 256;                       // selection blocks; sort an
                                                     // 128, 256 or 512 threads (tiles of 8 kT records)
 constexpr int kPerThread = 8;                       // records per thread in the tile sort and merge
 constexpr uint32_t kSelectTile = 1024;              // records per selection block (256 x 4)
+constexpr uint32_t kSplitLanes = 8;                 // lanes per tile boundary in the merge partition
 
 __device__ __forceinline__ Event sentinel() { return Event{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}; }
 
@@ -80,22 +81,29 @@ __device__ __forceinline__ void block_merge_splits(const ASrc& A, uint32_t na, c
 // consecutive threads merge start in different banks.
 __device__ __forceinline__ uint32_t pad8(uint32_t i) { return i + (i >> 3); }
 
-// The same split for one diagonal d by one wave: 64 probes per round, ceil(log_64 n) rounds.
-__device__ __forceinline__ uint32_t wave_merge_split(const ASrc& A, uint32_t na, const Event* B, uint32_t nb,
-                                                     uint32_t d) {
-  uint32_t lo = d > nb ? d - nb : 0u, hi = d < na ? d : na;
-  const uint32_t k = (uint32_t)lane_id();
-  while (lo < hi) {                                   // wave-uniform
-    const uint32_t step = (hi - lo + kWave - 1u) / kWave;
+// The same split for diagonal d by a group of P lanes (P a power of two dividing the wave; every
+// lane of a group passes the same d, `active` false for a group with no diagonal): P probes per
+// round, ceil(log_P n) rounds.  The loop runs while any group of the wave is searching.
+template <int P>
+__device__ __forceinline__ uint32_t group_merge_split(const ASrc& A, uint32_t na, const Event* B, uint32_t nb,
+                                                      uint32_t d, bool active) {
+  const uint32_t lane = (uint32_t)lane_id(), k = lane % P, gbase = lane - k;
+  const uint64_t gmask = (P == kWave ? ~0ull : ((1ull << P) - 1ull)) << gbase;
+  uint32_t lo = active && d > nb ? d - nb : 0u, hi = active ? (d < na ? d : na) : 0u;
+  while (__ballot(lo < hi)) {
+    const bool run = lo < hi;                         // uniform within the group
+    const uint32_t step = run ? (hi - lo + P - 1u) / P : 1u;
     const uint32_t i = lo + k * step;
-    const bool p = i < hi && key_less(load_event(B + (d - 1u - i)), A.get(i));
-    const uint64_t m = __ballot(p);
-    if (m == 0) {
-      lo = lo + (hi - lo - 1u) / step * step + 1u;    // past the last probe
-    } else {
-      const uint32_t f = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-      hi = lo + f * step;
-      if (f) lo = lo + (f - 1u) * step + 1u;
+    const bool p = run && i < hi && key_less(load_event(B + (d - 1u - i)), A.get(i));
+    const uint64_t m = (__ballot(p) & gmask) >> gbase;
+    if (run) {
+      if (m == 0) {
+        lo = lo + (hi - lo - 1u) / step * step + 1u;  // past the last probe
+      } else {
+        const uint32_t f = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+        hi = lo + f * step;
+        if (f) lo = lo + (f - 1u) * step + 1u;
+      }
     }
   }
   return lo;
@@ -286,23 +294,27 @@ __global__ __launch_bounds__(kT) void merge_tasks_kernel(const MergeTask* tasks,
   merge_tile<kT>(A, na, t.b, t.n_b, t0, om, splits ? splits + (size_t)blockIdx.y * stride + blockIdx.x : nullptr);
 }
 
-// The merge-path splits of every tile boundary of every task, one wave per boundary, so that
-// the merge blocks start loading at once.  Boundary x of task y is output rank min(x * kTile,
-// total); grid (ceil(boundaries / 4), ntasks) of 256 threads.
+// The merge-path splits of every tile boundary of every task, kSplitLanes lanes per boundary,
+// so that the merge blocks start loading at once.  Boundary x of task y is output rank
+// min(x * kTile, total); grid (ceil(boundaries / (256 / kSplitLanes)), ntasks) of 256 threads.
+// Fewer lanes per boundary take more rounds but far fewer loads, which is what limits this
+// launch: thousands of boundaries search at once and their probes are scattered lines.
 template <int kT>
 __global__ __launch_bounds__(256) void merge_partition_kernel(const MergeTask* tasks, uint32_t boundaries,
                                                               uint32_t* splits, uint32_t stride) {
   constexpr uint32_t kTile = kT * kPerThread;
-  const uint32_t x = blockIdx.x * 4u + threadIdx.x / kWave;
-  if (x >= boundaries) return;
+  constexpr uint32_t kPer = 256u / kSplitLanes;         // boundaries per block
+  const uint32_t x = blockIdx.x * kPer + threadIdx.x / kSplitLanes;
+  if (blockIdx.x * kPer >= boundaries) return;          // the whole block is past the boundaries
   const MergeTask t = tasks[blockIdx.y];
   const uint32_t na = t.n_a1 + t.n_a2;
   const uint32_t total = na + t.n_b;
-  if ((uint64_t)x * kTile > total + (uint64_t)kTile) return;   // past the task's last tile boundary
+  // boundaries past the task's last tile boundary do not search (their groups stay in the loop)
+  const bool active = x < boundaries && (uint64_t)x * kTile <= total + (uint64_t)kTile;
   const uint32_t d = (uint64_t)x * kTile < total ? x * kTile : total;
   const ASrc A{t.a1, t.a2, t.n_a1};
-  const uint32_t s = wave_merge_split(A, na, t.b, t.n_b, d);
-  if (lane_id() == 0) splits[(size_t)blockIdx.y * stride + x] = s;
+  const uint32_t s = group_merge_split<kSplitLanes>(A, na, t.b, t.n_b, d, active);
+  if (active && threadIdx.x % kSplitLanes == 0) splits[(size_t)blockIdx.y * stride + x] = s;
 }
 
 // One wave: lane k < kInsertStripes + 1 takes segment k's record count (a stripe counter past
@@ -589,8 +601,8 @@ gpq_status merge_tasks_t(const MergeTask* d_tasks, uint32_t ntasks, uint64_t max
   const uint32_t gx = grid_of(max_total, kTile);
   if (splits) {
     GPQ_CHECK_ARG(stride >= gx + 1u, "merge_tasks: split buffer stride below the tile count + 1");
-    hipLaunchKernelGGL(merge_partition_kernel<kT>, dim3(ceil_div(gx + 1u, 4u), ntasks), dim3(256), 0, stream, d_tasks,
-                       gx + 1u, splits, stride);
+    hipLaunchKernelGGL(merge_partition_kernel<kT>, dim3(ceil_div(gx + 1u, 256u / kSplitLanes), ntasks), dim3(256), 0,
+                       stream, d_tasks, gx + 1u, splits, stride);
   }
   hipLaunchKernelGGL(merge_tasks_kernel<kT>, dim3(gx, ntasks), dim3(kT), 0, stream, d_tasks,
                      static_cast<const uint32_t*>(splits), stride);
