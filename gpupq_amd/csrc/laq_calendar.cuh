@@ -3,27 +3,25 @@
 // The merge-and-commit PHOLD step (drain mode 2) ends window k at w_k = min(w_{k-1} + lookahead,
 // end time) in float arithmetic, so every window end is known as soon as the first one is.  In
 // calendar mode the queue uses that: it keeps one append log per upcoming window (a calendar
-// queue, R. Brown, CACM 31(10), 1988, with one day per window, shared by all LPs), and an event
-// is appended to the log of the window it falls in.  Step c is one launch with two kinds of
-// blocks that run side by side:
-//   * commit blocks: per LP, the staged events of window c in key order (sorted in registers),
-//     their successors appended to the logs of their windows, aggregated per block and window
-//     so that every log grows by coalesced runs; a successor for window c + 1 goes straight to
-//     that window's staging area;
-//   * group blocks: the log of window c + 1, grouped by LP into the other staging area (the
-//     staging area and its counters are small enough to stay in the device's last-level cache
-//     while they are written in random order).
-// No sorted run is kept, merged or shifted.
+// queue, R. Brown, CACM 31(10), 1988, with one day per window), split by the block of LPs the
+// events are for, and an event is appended to the log of its window and LP block.  Step c is one
+// launch with two kinds of blocks that run side by side:
+//   * commit blocks, one per block of kBlock LPs: the block's log of window c is grouped by LP in
+//     LDS, each LP's events are sorted in registers and committed in key order, and their
+//     successors are appended to the logs of their windows and LP blocks (one atomic each);
+//   * a few list blocks: the events beyond the calendar (over[(c + 1) & 1]) whose window the
+//     calendar now reaches move into the logs.
+// No sorted run is kept, merged or shifted, and no separate pass groups the events by LP.
 //
-// Layout (B = cal_windows, a power of two; L = num_lps):
-//   wlog   [B][wlog_cap] Event   window x's log is wlog[x mod B], count wlog_n[x mod B]
-//   over   [2][over_cap] Event   events beyond the calendar: appended to over[(x + 1) & 1] by the
-//                                steps that stage window x and read when window x + 1 is staged
-//   parked [parked_cap]  Event   events at or past the end time: never committed, only kept
-//   stage / stage_alt [S][L]     staging areas of windows with even / odd index, slot-major
-//   ins / ins_alt [L]            their counts
-//   cal_st                       `cur` (the window the next step commits) and the ends of windows
-//                                x in [cur, cur + B] at w[x mod 2B]
+// Layout (B = cal_windows, a power of two; L = num_lps; K = kBlock LPs per block):
+//   wlog   [B][blocks][sub_cap]  Event   window x, LP block b: count wlog_n[(x mod B) * blocks + b]
+//   over   [2][over_cap]         Event   events beyond the calendar: appended to over[c & 1] by step
+//                                        c, read (and moved on) by step c + 1
+//   parked [parked_cap]          Event   events at or past the end time: never committed, only kept
+//   cal_st                               `cur` (the window the next step commits) and the ends of
+//                                        windows x in [cur, cur + B] at w[x mod 2B]
+// A step appends only to windows c + 1 .. c + B - 1, so the log slot of window c (the one its
+// commit blocks read and then empty) takes no appends during the step.
 //
 // Reference: the per-LP queue of src/local_array_queue.cu and its heap-backed far tier
 // (:515-681); the step is handle_next_event (src/phold.cu:81-117) for every event of a window.
@@ -39,11 +37,10 @@ namespace cal {
 // contention on those counters: 256 LPs 1.12e10, 512 LPs 1.25-1.28e10, 1024 LPs 1.25-1.28e10
 // events/s at 2^20 LPs (same box, the staging threads kept at 98 K).
 constexpr int kBlock = 512;
-constexpr int kRegs = 8;          // staged records one thread sorts in registers (fast path)
-constexpr int kWaveCand = 64;     // staged records of one LP on the slow path (one per lane)
-constexpr int kGroupBlocks = 2048;        // blocks of the standalone filing / staging launches
-constexpr uint32_t kStepGroupBlocks = 192;   // staging blocks of a step (beside the commit blocks; with 256-thread
-                                             // blocks 384 measured best of 128, 256, 384, 512, 1024, 2048)
+constexpr int kRegs = 8;          // events of one LP sorted in registers by one thread (fast path)
+constexpr int kWaveCand = 64;     // events of one LP on the slow path (one per lane)
+constexpr int kGroupBlocks = 2048;        // blocks of the standalone filing launch
+constexpr uint32_t kStepListBlocks = 32;  // blocks of a step that move the beyond-the-calendar events
 
 struct Key {
   uint32_t ts, src, seq;
@@ -79,11 +76,11 @@ __device__ __forceinline__ void sort8(uint32_t* t, uint32_t* s, uint32_t* q) {
   cswap3<1, 2>(t, s, q); cswap3<3, 4>(t, s, q); cswap3<5, 6>(t, s, q);
 }
 
-// Window offset of timestamp bits ts: the smallest o in [o_min, B) with ts < sw[o], where
-// sw[o] is the end of window cur + o; B when ts >= sw[B - 1] (beyond the calendar).
-__device__ __forceinline__ uint32_t find_window(const uint32_t* sw, uint32_t B, uint32_t ts, uint32_t o_min) {
-  if (!(ts < sw[B - 1])) return B;
-  uint32_t lo = o_min, hi = B - 1;
+// Window offset of timestamp bits ts: the smallest o in [o_min, n) with ts < sw[o], where
+// sw[o] is the end of window x + o; n when ts >= sw[n - 1] (beyond the windows looked at).
+__device__ __forceinline__ uint32_t find_window(const uint32_t* sw, uint32_t n, uint32_t ts, uint32_t o_min) {
+  if (!(ts < sw[n - 1])) return n;
+  uint32_t lo = o_min, hi = n - 1;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
     if (ts < sw[mid]) hi = mid;
@@ -92,109 +89,86 @@ __device__ __forceinline__ uint32_t find_window(const uint32_t* sw, uint32_t B, 
   return lo;
 }
 
-// Where an event goes, seen from window x (the window being committed, or staged): a window
-// offset o < B (the log of window x + o), kOver (beyond the calendar) or kParked (at or past the
-// end time).
+// Where an event goes, seen from window x with n windows looked at: a window offset o < n
+// (the log of window x + o), kOver (beyond them) or kParked (at or past the end time).
 struct Route {
   static constexpr uint32_t kOver = kCalMaxWindows;
   static constexpr uint32_t kParked = kCalMaxWindows + 1;
   static constexpr uint32_t kNone = kCalMaxWindows + 2;   // no record
 };
+constexpr uint32_t kRemote = 0x10000u;   // flag on a window route: the event is for another block's LPs
 
-__device__ __forceinline__ uint32_t route_of(const uint32_t* sw, uint32_t B, uint32_t ts, uint32_t end_bits,
+__device__ __forceinline__ uint32_t route_of(const uint32_t* sw, uint32_t n, uint32_t ts, uint32_t end_bits,
                                              uint32_t o_min) {
   if (!(ts < end_bits)) return Route::kParked;
-  const uint32_t o = find_window(sw, B, ts, o_min);
-  return o < B ? o : Route::kOver;
+  const uint32_t o = find_window(sw, n, ts, o_min);
+  return o < n ? o : Route::kOver;
 }
 
-// The counter and record array of route r seen from window x; the beyond-the-calendar list is
-// over[op].
-__device__ __forceinline__ uint32_t* route_count(const LaqView& q, uint32_t x, uint32_t op, uint32_t r) {
-  if (r == Route::kParked) return q.parked_n;
-  if (r == Route::kOver) return &q.over_n[op];
-  return &q.wlog_n[(x + r) & (q.cal_windows - 1u)];
+// The log of window x for LP block b: its counter and its first record.
+__device__ __forceinline__ uint32_t* log_count(const LaqView& q, uint32_t x, uint32_t b) {
+  return &q.wlog_n[(size_t)(x & (q.cal_windows - 1u)) * q.cal_blocks + b];
+}
+__device__ __forceinline__ Event* log_base(const LaqView& q, uint32_t x, uint32_t b) {
+  return q.wlog + (size_t)(x & (q.cal_windows - 1u)) * q.wlog_cap + (size_t)b * q.cal_sub_cap;
 }
 
-__device__ __forceinline__ Event* route_slot(const LaqView& q, uint32_t x, uint32_t op, uint32_t r, uint32_t pos,
-                                             bool* ok) {
-  if (r == Route::kParked) {
-    *ok = pos < q.parked_cap;
-    return q.parked + pos;
-  }
-  if (r == Route::kOver) {
-    *ok = pos < q.over_cap;
-    return q.over + (size_t)op * q.over_cap + pos;
-  }
-  *ok = pos < q.wlog_cap;
-  return q.wlog + (size_t)((x + r) & (q.cal_windows - 1u)) * q.wlog_cap + pos;
-}
-
-// One event appended by its own atomic (the rare paths).
-__device__ __forceinline__ void append_one(const LaqView& q, uint32_t x, uint32_t op, uint32_t r, const Event& e) {
-  const uint32_t pos = atomicAdd(route_count(q, x, op, r), 1u);
-  bool ok;
-  Event* p = route_slot(q, x, op, r, pos, &ok);
-  if (ok) store_event(p, e);
+// e appended to the log of window x for its LP's block, by its own atomic.
+__device__ __forceinline__ void log_append(const LaqView& q, uint32_t x, const Event& e) {
+  const uint32_t b = e.dst >> q.cal_block_log2;
+  const uint32_t pos = atomicAdd(log_count(q, x, b), 1u);
+  if (pos < q.cal_sub_cap) store_event(log_base(q, x, b) + pos, e);
   else atomicOr(&q.ws->error, kErrQueueOverflow);
 }
 
-__device__ __forceinline__ uint32_t route_index(uint32_t r, uint32_t B) { return r < B ? r : B + (r - kCalMaxWindows); }
-__device__ __forceinline__ uint32_t index_route(uint32_t i, uint32_t B) { return i < B ? i : kCalMaxWindows + (i - B); }
+// The lists: beyond the calendar (over[op]) and parked.
+__device__ __forceinline__ uint32_t* list_count(const LaqView& q, uint32_t op, uint32_t r) {
+  return r == Route::kParked ? q.parked_n : &q.over_n[op];
+}
+__device__ __forceinline__ void list_store(const LaqView& q, uint32_t op, uint32_t r, uint32_t pos, const uint4& e) {
+  const bool parked = r == Route::kParked;
+  if (pos < (parked ? q.parked_cap : q.over_cap))
+    *reinterpret_cast<uint4*>(parked ? q.parked + pos : q.over + (size_t)op * q.over_cap + pos) = e;
+  else
+    atomicOr(&q.ws->error, kErrQueueOverflow);
+}
 
-// Block-aggregated append: every thread of the block calls with at most one record (route
-// kNone: none); each takes a rank in the block's LDS count of its route, one thread per used
-// route reserves the block's run with a single atomic, and the records are written into the
-// runs.  x and op are block-uniform.
-struct BlockAppend {
-  uint32_t cnt[kCalMaxWindows + 2];
-  uint32_t base[kCalMaxWindows + 2];
+// One event filed by its own atomic: route r seen from window x (a window offset, kOver: the
+// list over[op], or kParked).
+__device__ __forceinline__ void append_one(const LaqView& q, uint32_t x, uint32_t op, uint32_t r, const Event& e) {
+  if (r < kCalMaxWindows) {
+    log_append(q, x + r, e);
+    return;
+  }
+  list_store(q, op, r, atomicAdd(list_count(q, op, r), 1u), make_uint4(e.ts, e.dst, e.src, e.seq));
+}
+
+// Block-wide filing of at most one event per thread: a window route goes to its log by the
+// event's own atomic; the list routes take a rank in the block's LDS count of their list, one
+// thread per list reserves the block's run with a single atomic, and the events are written
+// into the runs.  op is block-uniform; every thread of the block calls.
+struct ListAppend {
+  uint32_t cnt[2];
+  uint32_t base[2];
 };
 
-__device__ __forceinline__ void block_append(const LaqView& q, uint32_t x, uint32_t op, uint32_t r, const Event& e,
-                                             BlockAppend& ba) {
-  const uint32_t B = q.cal_windows;
-  for (uint32_t i = threadIdx.x; i < B + 2u; i += blockDim.x) ba.cnt[i] = 0u;
+__device__ __forceinline__ void block_file(const LaqView& q, uint32_t x, uint32_t op, uint32_t r, const Event& e,
+                                           ListAppend& la) {
+  if (threadIdx.x < 2) la.cnt[threadIdx.x] = 0u;
   __syncthreads();
+  const bool list = r == Route::kOver || r == Route::kParked;
+  const uint32_t li = r == Route::kParked ? 1u : 0u;
   uint32_t rank = 0;
-  if (r != Route::kNone) rank = atomicAdd(&ba.cnt[route_index(r, B)], 1u);
+  if (r < kCalMaxWindows) log_append(q, x + r, e);
+  else if (list) rank = atomicAdd(&la.cnt[li], 1u);
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < B + 2u; i += blockDim.x) {
-    const uint32_t n = ba.cnt[i];
-    ba.base[i] = n ? atomicAdd(route_count(q, x, op, index_route(i, B)), n) : 0u;
+  if (threadIdx.x < 2) {
+    const uint32_t n = la.cnt[threadIdx.x];
+    la.base[threadIdx.x] = n ? atomicAdd(list_count(q, op, threadIdx.x ? Route::kParked : Route::kOver), n) : 0u;
   }
   __syncthreads();
-  if (r != Route::kNone) {
-    bool ok;
-    Event* p = route_slot(q, x, op, r, ba.base[route_index(r, B)] + rank, &ok);
-    if (ok) store_event(p, e);
-    else atomicOr(&q.ws->error, kErrQueueOverflow);
-  }
+  if (list) list_store(q, op, r, la.base[li] + rank, make_uint4(e.ts, e.dst, e.src, e.seq));
   __syncthreads();
-}
-
-// The staging area of window x (slot-major: slot s of LP lp at s * num_lps + lp, so that the
-// grouping pass writes into few, densely filled lines -- slots 0 .. 3 of all LPs -- and the
-// LPs of a wave read slot s at consecutive addresses; while the initial events are populated
-// the even area is LP-major, as for every other LAQ mode).
-struct Stage {
-  Event* rec;
-  uint32_t* n;
-};
-
-__device__ __forceinline__ Stage stage_of(const LaqView& q, uint32_t x) {
-  return (x & 1u) ? Stage{q.stage_alt, q.ins_alt} : Stage{q.stage, q.ins};
-}
-
-__device__ __forceinline__ size_t gslot(const LaqView& q, uint32_t slot, uint32_t lp) {
-  return (size_t)slot * q.num_lps + lp;
-}
-
-// queue_insert into a staging area: e goes to its LP's next slot.
-__device__ __forceinline__ void stage_one(const LaqView& q, const Stage& st, const Event& e) {
-  const uint32_t slot = atomicAdd(&st.n[e.dst], 1u);
-  if (slot < q.stage_cap) store_event(&st.rec[gslot(q, slot, e.dst)], e);
-  else atomicOr(&q.ws->error, kErrStageOverflow);
 }
 
 // Block-wide: the ends of windows x .. x + B - 1 into sw (every thread of the block calls).
@@ -213,41 +187,49 @@ __device__ __forceinline__ Event successor(const FusedModel& m, uint32_t lp, uin
   return Event{f_to_bits(t_new), dst, local, m.init_per_lp + n};
 }
 
-// Slow path, one wave for one LP with more than kRegs staged events: lane i holds event i and
-// its rank among them; the successors are filed by their own atomics.
+// Slow path, one wave for one LP with more than kRegs events in window c: the block's log of the
+// window is scanned for them (lane i gets the LP's i-th), each lane ranks its event among the
+// others, and the successors are filed by their own atomics.
 __device__ __forceinline__ void slow_lp(const LaqView& q, const FusedModel& m, const uint32_t* sw, uint32_t c,
-                                        uint32_t end_bits, uint32_t lp, Key* sorted) {
+                                        uint32_t end_bits, uint32_t lp, const Event* lg, uint32_t n, Key* keys) {
   const int lane = lane_id();
-  const Stage st = stage_of(q, c), next = stage_of(q, c + 1u);
-  uint32_t k = st.n[lp];
-  if (k > q.stage_cap) k = q.stage_cap;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t k = 0;
+  for (uint32_t base = 0; base < n; base += kWave) {     // wave-uniform trip count
+    const uint32_t i = base + (uint32_t)lane;
+    const Event x = i < n ? load_event(&lg[i]) : Event{kInfBits, 0xFFFFFFFFu, 0u, 0u};
+    const bool mine = x.dst == lp;
+    const uint64_t mm = __ballot(mine);
+    const uint32_t pos = k + (uint32_t)__popcll(mm & below);
+    if (mine && pos < (uint32_t)kWaveCand) keys[pos] = Key{x.ts, x.src, x.seq};
+    k += (uint32_t)__popcll(mm);
+  }
   if (k > (uint32_t)kWaveCand) {
     if (lane == 0) atomicOr(&q.ws->error, kErrStageOverflow);
     k = kWaveCand;
   }
+  laq::wave_lds_sync();
   const bool has = (uint32_t)lane < k;
-  const Event e = has ? load_event(&st.rec[gslot(q, (uint32_t)lane, lp)]) : Event{kInfBits, 0u, 0u, 0u};
-  const Key me{e.ts, e.src, e.seq};
+  const Key me = has ? keys[lane] : Key{kInfBits, 0u, 0u};
   uint32_t rank = 0;
   for (uint32_t j = 0; j < k; ++j) {
-    const Key o{(uint32_t)__shfl((int)e.ts, (int)j, kWave), (uint32_t)__shfl((int)e.src, (int)j, kWave),
-                (uint32_t)__shfl((int)e.seq, (int)j, kWave)};
+    const Key o{(uint32_t)__shfl((int)me.ts, (int)j, kWave), (uint32_t)__shfl((int)me.src, (int)j, kWave),
+                (uint32_t)__shfl((int)me.seq, (int)j, kWave)};
     rank += key_lt(o, me) ? 1u : 0u;
   }
+  laq::wave_lds_sync();                                  // every lane has read its key
   const uint32_t cnt = m.count[lp];
   if (has) {
-    sorted[rank] = me;
-    const Event ne = successor(m, lp, cnt + rank, e.ts);
-    const uint32_t r = route_of(sw, q.cal_windows, ne.ts, end_bits, 1u);
-    if (r == 1u) stage_one(q, next, ne);
-    else append_one(q, c, c & 1u, r, ne);
+    keys[rank] = me;
+    const Event ne = successor(m, lp, cnt + rank, me.ts);
+    append_one(q, c, c & 1u, route_of(sw, q.cal_windows, ne.ts, end_bits, 1u), ne);
   }
   laq::wave_lds_sync();
   if (lane == 0) {
     uint64_t h = m.hash[lp];
     uint32_t last = m.last_ts ? m.last_ts[lp] : 0u;
     for (uint32_t r = 0; r < k; ++r) {
-      const Key x = sorted[r];
+      const Key x = keys[r];
       h = trace_step(h, x.ts, x.src, x.seq);
       if (m.last_ts) {
         if (x.ts < last) atomicOr(&q.ws->error, kErrBadState);
@@ -257,37 +239,57 @@ __device__ __forceinline__ void slow_lp(const LaqView& q, const FusedModel& m, c
     m.count[lp] = cnt + k;
     m.hash[lp] = h;
     if (m.last_ts) m.last_ts[lp] = last;
-    st.n[lp] = 0;
   }
   laq::wave_lds_sync();
 }
 
-// Commit blocks of step c (window c), block `blk` of the LPs.  Fast path, one thread per LP with
-// at most kRegs staged events: sorted in registers and committed in order; a successor for
-// window c + 1 goes straight to that window's staging area, every other one takes a rank in the
-// block's LDS count of its route, one thread per route reserves the block's run with a single
-// atomic, and the successors are written into the runs.  LPs with more staged events are handled
-// afterwards, one wave each (slow_lp).
+// Commit block `blk` of step c (window c): the block's log of the window is grouped by LP in LDS
+// (counts, and the log index of each LP's first kRegs events).  Fast path, one thread per LP
+// with at most kRegs events: sorted in registers and committed in order.  A successor for one of
+// this block's LPs (half of them in PHOLD: the self-addressed ones) and one for a list take a
+// rank in the block's LDS count of their window or list, one thread per window and list
+// reserves the block's run, and they are written into the runs; a successor for another block
+// goes to that block's log by its own atomic, all of a thread's issued before any is waited for.
+// LPs with more events are handled afterwards, one wave each (slow_lp).  The log is emptied at
+// the end.
 __device__ __forceinline__ void commit_block(const LaqView& q, const FusedModel& m, float end_time, uint32_t c,
                                              uint32_t blk) {
   __shared__ uint32_t sw[kCalMaxWindows];
-  __shared__ uint32_t s_cnt[kCalMaxWindows + 2], s_base[kCalMaxWindows + 2];
+  __shared__ uint32_t s_cnt[kBlock];
+  __shared__ uint16_t s_idx[kBlock][kRegs];
+  __shared__ uint32_t s_lcnt[2], s_lbase[2];
+  __shared__ uint32_t s_wcnt[kCalMaxWindows], s_wbase[kCalMaxWindows];
   __shared__ uint32_t s_slow[kBlock];
   __shared__ uint32_t s_nslow;
-  __shared__ Key s_sorted[kBlock / kWave][kWaveCand];
+  __shared__ Key s_keys[kBlock / kWave][kWaveCand];
   load_windows(q, c, sw);
   const uint32_t B = q.cal_windows;
-  for (uint32_t i = threadIdx.x; i < B + 2u; i += blockDim.x) s_cnt[i] = 0u;
+  s_cnt[threadIdx.x] = 0u;
+  for (uint32_t i = threadIdx.x; i < B; i += kBlock) s_wcnt[i] = 0u;
+  if (threadIdx.x < 2) s_lcnt[threadIdx.x] = 0u;
   if (threadIdx.x == 0) s_nslow = 0;
   __syncthreads();
-  const Stage st = stage_of(q, c), next = stage_of(q, c + 1u);
+  const uint32_t n0 = *log_count(q, c, blk);
+  const uint32_t n = n0 < q.cal_sub_cap ? n0 : q.cal_sub_cap;
+  const Event* lg = log_base(q, c, blk);
+  const uint32_t lp0 = blk << q.cal_block_log2;
+  for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+    const uint32_t l = lg[i].dst - lp0;                 // < kBlock: the log holds this block's LPs only
+    if (l >= (uint32_t)kBlock) {
+      atomicOr(&q.ws->error, kErrBadState);
+      continue;
+    }
+    const uint32_t k = atomicAdd(&s_cnt[l], 1u);
+    if (k < (uint32_t)kRegs) s_idx[l][k] = (uint16_t)i;
+  }
+  __syncthreads();
   const uint32_t end_bits = f_to_bits(end_time);
-  const uint32_t lp = blk * kBlock + threadIdx.x;
-  uint4 ne[kRegs];      // successors as uint4: an array of the 16-byte-aligned Event goes to scratch memory
+  const uint32_t lp = lp0 + threadIdx.x;
+  uint4 ne[kRegs];      // list successors as uint4: an array of the 16-byte-aligned Event goes to scratch memory
   uint32_t route[kRegs], rank[kRegs];
   uint32_t cn = 0;
   if (lp < q.num_lps && lp - m.base < m.lps) {
-    const uint32_t k = st.n[lp];
+    const uint32_t k = s_cnt[threadIdx.x];
     if (k > (uint32_t)kRegs) {
       s_slow[atomicAdd(&s_nslow, 1u)] = lp;
     } else if (k) {
@@ -296,7 +298,7 @@ __device__ __forceinline__ void commit_block(const LaqView& q, const FusedModel&
 #pragma unroll
       for (int s = 0; s < kRegs; ++s) {
         if ((uint32_t)s < cn) {
-          const Event e = load_event(&st.rec[gslot(q, (uint32_t)s, lp)]);
+          const Event e = load_event(&lg[s_idx[threadIdx.x][s]]);
           kt[s] = e.ts;
           ks[s] = e.src;
           kq[s] = e.seq;
@@ -310,6 +312,7 @@ __device__ __forceinline__ void commit_block(const LaqView& q, const FusedModel&
       sort8(kt, ks, kq);
 #pragma unroll
       for (int j = 0; j < kRegs; ++j) {
+        route[j] = Route::kNone;
         if ((uint32_t)j < cn) {
           h = trace_step(h, kt[j], ks[j], kq[j]);
           if (m.last_ts) {
@@ -317,75 +320,90 @@ __device__ __forceinline__ void commit_block(const LaqView& q, const FusedModel&
             last = kt[j];
           }
           const Event sc = successor(m, lp, cnt + (uint32_t)j, kt[j]);
+          const uint32_t r = route_of(sw, B, sc.ts, end_bits, 1u);
+          route[j] = r;
           ne[j] = make_uint4(sc.ts, sc.dst, sc.src, sc.seq);
-          route[j] = route_of(sw, B, sc.ts, end_bits, 1u);
-          if (route[j] == 1u) {                 // window c + 1: straight to its staging area
-            stage_one(q, next, sc);
-            route[j] = Route::kNone;
+          if (r < kCalMaxWindows) {
+            if ((sc.dst >> q.cal_block_log2) == blk) rank[j] = atomicAdd(&s_wcnt[r], 1u);   // this block's log
+            else route[j] = r | kRemote;
           } else {
-            rank[j] = atomicAdd(&s_cnt[route_index(route[j], B)], 1u);
+            rank[j] = atomicAdd(&s_lcnt[r == Route::kParked ? 1u : 0u], 1u);
           }
         }
       }
       m.count[lp] = cnt + cn;
       m.hash[lp] = h;
       if (m.last_ts) m.last_ts[lp] = last;
-      st.n[lp] = 0;
     }
   }
+  // successors for other blocks' logs: every slot atomic of the thread in flight at once
+  uint32_t pos[kRegs];
+#pragma unroll
+  for (int j = 0; j < kRegs; ++j)
+    pos[j] = (uint32_t)j < cn && (route[j] & kRemote)
+                 ? atomicAdd(log_count(q, c + (route[j] & ~kRemote), ne[j].y >> q.cal_block_log2), 1u)
+                 : 0u;
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < B + 2u; i += blockDim.x) {
-    const uint32_t n = s_cnt[i];
-    s_base[i] = n ? atomicAdd(route_count(q, c, c & 1u, index_route(i, B)), n) : 0u;
+  // one run per window (this block's own log) and per list for the block
+  for (uint32_t i = threadIdx.x; i < B; i += kBlock) {
+    const uint32_t t = s_wcnt[i];
+    s_wbase[i] = t ? atomicAdd(log_count(q, c + i, blk), t) : 0u;
+  }
+  if (threadIdx.x < 2) {
+    const uint32_t t = s_lcnt[threadIdx.x];
+    s_lbase[threadIdx.x] = t ? atomicAdd(list_count(q, c & 1u, threadIdx.x ? Route::kParked : Route::kOver), t) : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < kRegs; ++j) {
+    if ((uint32_t)j < cn && (route[j] & kRemote)) {
+      const uint32_t r = route[j] & ~kRemote;
+      if (pos[j] < q.cal_sub_cap) *reinterpret_cast<uint4*>(log_base(q, c + r, ne[j].y >> q.cal_block_log2) + pos[j]) = ne[j];
+      else atomicOr(&q.ws->error, kErrQueueOverflow);
+    }
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kRegs; ++j) {
-    if ((uint32_t)j < cn && route[j] != Route::kNone) {
-      bool ok;
-      Event* p = route_slot(q, c, c & 1u, route[j], s_base[route_index(route[j], B)] + rank[j], &ok);
-      if (ok) *reinterpret_cast<uint4*>(p) = ne[j];
+    if ((uint32_t)j >= cn || (route[j] & kRemote)) continue;
+    const uint32_t r = route[j];
+    if (r < kCalMaxWindows) {
+      const uint32_t p = s_wbase[r] + rank[j];
+      if (p < q.cal_sub_cap) *reinterpret_cast<uint4*>(log_base(q, c + r, blk) + p) = ne[j];
       else atomicOr(&q.ws->error, kErrQueueOverflow);
+    } else {
+      list_store(q, c & 1u, r, s_lbase[r == Route::kParked ? 1u : 0u] + rank[j], ne[j]);
     }
   }
   const uint32_t w = threadIdx.x / kWave;
-  for (uint32_t i = w; i < s_nslow; i += kBlock / kWave) slow_lp(q, m, sw, c, end_bits, s_slow[i], s_sorted[w]);
+  for (uint32_t i = w; i < s_nslow; i += kBlock / kWave) slow_lp(q, m, sw, c, end_bits, s_slow[i], lg, n, s_keys[w]);
+  __syncthreads();
+  if (threadIdx.x == 0) *log_count(q, c, blk) = 0u;     // the block's log of window c is consumed
 }
 
-// Group blocks staging window x (block blk of nblk): every record of its log goes to the
-// staging area of its LP; every record of the list over[x & 1] goes to that staging area
-// (window x), to the log of its window, on to over[(x + 1) & 1], or with the parked ones.
-__device__ __forceinline__ void group_block(const LaqView& q, float end_time, uint32_t x, uint32_t blk,
-                                            uint32_t nblk) {
+// List block `blk` of `nblk` in the step that commits window x - 1: the events of over[x & 1]
+// (beyond the windows the step before could reach) are seen from window x over its first B - 1
+// windows -- x .. x + B - 2, all of them beyond the window being committed and short of the
+// one whose log slot it is reading -- and go to the log of their window, on to over[(x + 1) & 1],
+// or with the parked ones.
+__device__ __forceinline__ void list_block(const LaqView& q, float end_time, uint32_t x, uint32_t blk,
+                                           uint32_t nblk) {
   __shared__ uint32_t sw[kCalMaxWindows];
-  __shared__ BlockAppend ba;
+  __shared__ ListAppend la;
   load_windows(q, x, sw);
   __syncthreads();
   const uint32_t B = q.cal_windows;
-  const Stage st = stage_of(q, x);
-  const uint32_t n = q.wlog_n[x & (B - 1u)];
-  const uint32_t nl = n < q.wlog_cap ? n : q.wlog_cap;
-  const Event* lg = q.wlog + (size_t)(x & (B - 1u)) * q.wlog_cap;
-  const uint32_t stride = nblk * blockDim.x;
-  // One record per thread and round.  Two or more in flight per thread (loads, then slot
-  // atomics, then stores) measured slower: 1.02e10 (four) and 1.05e10 (eight) against 1.08e10
-  // events/s; the pass is bound by the rate of the random atomics and stores, not their latency.
-  for (uint32_t i = blk * blockDim.x + threadIdx.x; i < nl; i += stride) stage_one(q, st, load_event(&lg[i]));
   const uint32_t rp = x & 1u, wp = rp ^ 1u;
   const uint32_t no0 = q.over_n[rp];
   const uint32_t no = no0 < q.over_cap ? no0 : q.over_cap;
   const Event* ov = q.over + (size_t)rp * q.over_cap;
   const uint32_t end_bits = f_to_bits(end_time);
+  const uint32_t stride = nblk * blockDim.x;
   for (uint32_t base = blk * blockDim.x; base < no; base += stride) {   // block-uniform trip count
     const uint32_t i = base + threadIdx.x;
     const bool has = i < no;
     const Event e = has ? load_event(&ov[i]) : Event{};
-    uint32_t r = has ? route_of(sw, B, e.ts, end_bits, 0u) : Route::kNone;
-    if (r == 0u) {
-      stage_one(q, st, e);
-      r = Route::kNone;
-    }
-    block_append(q, x, wp, r, e, ba);
+    const uint32_t r = has ? route_of(sw, B - 1u, e.ts, end_bits, 0u) : Route::kNone;
+    block_file(q, x, wp, r, e, la);
   }
 }
 

@@ -125,22 +125,21 @@ __global__ __launch_bounds__(kFusedBlock, 4) void phold_fused_kernel(LaqView q, 
   fused_block<G, SL>(q, fm, settle);
 }
 
-// The step on a calendar-mode LAQ (laq_calendar.cuh): blocks [0, group_blocks) stage window
-// cur + 1, the others commit window cur.  With `inst` null the whole queue is one instance;
-// otherwise commit block b takes the model parameters and the end time of instance
-// inst[block_inst[b]] (several instances in one queue, every 256-LP block in one instance).
-// (The staging blocks run first and the commit blocks fill the device beside them; the same
-// two parts as kernels on two streams measured 7% slower.)
+// The step on a calendar-mode LAQ (laq_calendar.cuh): blocks [0, list_blocks) move the events
+// beyond the calendar that it now reaches into their logs, the others commit window cur, one
+// block of LPs each.  With `inst` null the whole queue is one instance; otherwise commit block b
+// takes the model parameters and the end time of instance inst[block_inst[b]] (several
+// instances in one queue, every commit block in one instance).
 static_assert(cal::kBlock == (int)LaqQueue::kCalBlockLps, "calendar commit block size");
 __global__ __launch_bounds__(cal::kBlock) void phold_calendar_kernel(LaqView q, PholdDev m, float end_time,
-                                                                     uint32_t group_blocks, const PoolInst* inst,
+                                                                     uint32_t list_blocks, const PoolInst* inst,
                                                                      const uint32_t* block_inst) {
   if (q.ws->done) return;
   const uint32_t cur = q.cal_st->cur;
-  if (blockIdx.x < group_blocks) {
-    cal::group_block(q, end_time, cur + 1u, blockIdx.x, group_blocks);
+  if (blockIdx.x < list_blocks) {
+    cal::list_block(q, end_time, cur + 1u, blockIdx.x, list_blocks);
   } else {
-    const uint32_t b = blockIdx.x - group_blocks;
+    const uint32_t b = blockIdx.x - list_blocks;
     FusedModel fm{q.ws, m.count, m.hash, m.last_ts, 0u, m.num_lps, m.init_per_lp, m.seed, m.p_self, m.lookahead};
     float et = end_time;
     if (inst) {
@@ -345,7 +344,7 @@ gpq_status PholdModel::launch_fused(uint32_t settle) {
   auto* laq = static_cast<LaqQueue*>(q);
   if (laq->calendar()) {     // nothing is staged between calendar steps: no closing fold
     if (settle) return GPQ_OK;
-    const uint32_t g = cal::kStepGroupBlocks;
+    const uint32_t g = cal::kStepListBlocks;
     hipLaunchKernelGGL(phold_calendar_kernel, dim3(g + ceil_div(cfg.num_lps, cal::kBlock)), dim3(cal::kBlock), 0,
                        q->stream, laq->view(), dev_of(this), cfg.end_time, g, nullptr, nullptr);
     GPQ_HIP(hipGetLastError());
@@ -571,7 +570,7 @@ gpq_status launch_calendar_instances(LaqQueue* q, const PoolInst* inst, const ui
   d.num_lps = q->cfg.num_lps;
   d.drain = 2;
   d.lookahead = q->cfg.lookahead;
-  const uint32_t g = cal::kStepGroupBlocks;
+  const uint32_t g = cal::kStepListBlocks;
   hipLaunchKernelGGL(phold_calendar_kernel, dim3(g + ceil_div(q->cfg.num_lps, cal::kBlock)), dim3(cal::kBlock), 0,
                      q->stream, q->view(), d, q->cfg.horizon, g, inst, block_inst);
   GPQ_HIP(hipGetLastError());

@@ -369,7 +369,7 @@ __global__ void laq_update_kernel(LaqView q, uint32_t* partial) {
 
 // ---- calendar mode (laq_calendar.cuh) -------------------------------------------------
 
-// queue_post_init: the staged initial events go to the list over[1]
+// queue_post_init: the staged initial events go to the list over[0]
 // (the window ends, and so the logs, are known only once every initial event is in); the block
 // minima of their timestamps feed cal_init_publish_kernel.
 __global__ __launch_bounds__(kLaqBlock) void cal_post_init_kernel(LaqView q, uint32_t* partial) {
@@ -381,8 +381,8 @@ __global__ __launch_bounds__(kLaqBlock) void cal_post_init_kernel(LaqView q, uin
     for (uint32_t i = 0; i < k; ++i) {
       const Event e = load_event(&q.stage[(size_t)lp * q.stage_cap + i]);
       m = e.ts < m ? e.ts : m;
-      const uint32_t pos = wave_append_slot(&q.over_n[1]);
-      if (pos < q.over_cap) store_event(&q.over[q.over_cap + pos], e);
+      const uint32_t pos = wave_append_slot(&q.over_n[0]);
+      if (pos < q.over_cap) store_event(&q.over[pos], e);
       else atomicOr(&q.ws->error, kErrQueueOverflow);
     }
     if (k0) q.ins[lp] = 0;
@@ -429,51 +429,38 @@ __global__ void cal_start_kernel(LaqView q, float lookahead, float end_time) {
   q.ws->sel = 0;
 }
 
-// Files every initial event (the list over[1]) in the log of its window (windows 0 .. B - 1),
-// in over[0] (read when window 0 is staged), or with the parked ones; appends are aggregated per
-// block and route.
+// Files every initial event (the list over[0]) in the log of its window (windows 0 .. B - 1)
+// and LP block, in over[1] (read by the first step), or with the parked ones.
 __global__ __launch_bounds__(cal::kBlock) void cal_fill_kernel(LaqView q, float end_time) {
   __shared__ uint32_t sw[kCalMaxWindows];
-  __shared__ cal::BlockAppend ba;
+  __shared__ cal::ListAppend la;
   cal::load_windows(q, 0u, sw);
   __syncthreads();
   const uint32_t B = q.cal_windows, end_bits = f_to_bits(end_time);
-  const uint32_t n1 = q.over_n[1], n = n1 < q.over_cap ? n1 : q.over_cap;
-  const Event* src = q.over + q.over_cap;
+  const uint32_t n0 = q.over_n[0], n = n0 < q.over_cap ? n0 : q.over_cap;
+  const Event* src = q.over;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {   // block-uniform trip count
     const uint32_t i = base + threadIdx.x;
     const bool has = i < n;
     const Event e = has ? load_event(&src[i]) : Event{};
     const uint32_t r = has ? cal::route_of(sw, B, e.ts, end_bits, 0u) : cal::Route::kNone;
-    cal::block_append(q, 0u, 0u, r, e, ba);
+    cal::block_file(q, 0u, 1u, r, e, la);
   }
 }
 
-__global__ void cal_fill_done_kernel(LaqView q) { q.over_n[1] = 0; }
+__global__ void cal_fill_done_kernel(LaqView q) { q.over_n[0] = 0; }
 
-// Stages window 0 (before the first step; later windows are staged by the step before theirs).
-__global__ __launch_bounds__(cal::kBlock) void cal_group_kernel(LaqView q, float end_time) {
-  cal::group_block(q, end_time, 0u, blockIdx.x, gridDim.x);
-}
-
-// After window 0 was staged: its log and the list it read are empty.
-__global__ void cal_group_done_kernel(LaqView q) {
-  q.wlog_n[0] = 0;
-  q.over_n[0] = 0;
-}
-
-// queue_post after a calendar step (window c committed, window c + 1 staged): the log of window
-// c + 1 and the list the staging read are emptied; the window advances by the lookahead (capped
-// at the end time) and the end of window c + 1 + B joins the ring; the window that reached the
-// end time ends the run.
+// queue_post after a calendar step (window c committed; its logs were emptied by the commit
+// blocks): the list the step read is emptied; the window advances by the lookahead (capped at
+// the end time) and the end of window c + 1 + B joins the ring; the window that reached the end
+// time ends the run.
 __global__ void cal_publish_kernel(LaqView q, float lookahead, float end_time) {
   WindowState w = *q.ws;
   if (w.done) return;
   CalState* st = q.cal_st;
   const uint32_t B = q.cal_windows, M = 2u * B - 1u;
   const uint32_t c = st->cur;
-  q.wlog_n[(c + 1u) & (B - 1u)] = 0;
   q.over_n[(c + 1u) & 1u] = 0;
   w.windows += 1;
   w.gmin = w.wend;                                 // a lower bound of the pending minimum
@@ -490,15 +477,17 @@ __global__ void cal_publish_kernel(LaqView q, float lookahead, float end_time) {
   *q.ws = w;
 }
 
-// Pending records: every window log, both beyond-the-calendar lists and the parked ones (by
-// the first thread), and both staging areas (all threads, one LP each).
+// Pending records: every log (grid-stride over the B x blocks counters), both beyond-the-calendar
+// lists and the parked ones (by the first thread), and the staging area (one LP per thread:
+// events inserted since the last queue_post).
 __global__ void cal_count_kernel(LaqView q, unsigned long long* sum) {
   __shared__ unsigned long long s[16];
-  const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long n = 0;
-  if (lp < q.num_lps) n = min(q.ins[lp], q.stage_cap) + min(q.ins_alt[lp], q.stage_cap);
-  if (lp == 0) {
-    for (uint32_t b = 0; b < q.cal_windows; ++b) n += min(q.wlog_n[b], q.wlog_cap);
+  if (t < q.num_lps) n = min(q.ins[t], q.stage_cap);
+  const uint32_t logs = q.cal_windows * q.cal_blocks;
+  for (uint32_t i = t; i < logs; i += gridDim.x * blockDim.x) n += min(q.wlog_n[i], q.cal_sub_cap);
+  if (t == 0) {
     for (uint32_t p = 0; p < 2; ++p) n += min(q.over_n[p], q.over_cap);
     n += min(*q.parked_n, q.parked_cap);
   }
@@ -506,14 +495,14 @@ __global__ void cal_count_kernel(LaqView q, unsigned long long* sum) {
   if (lane_id() == 0) s[threadIdx.x / kWave] = n;
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s[w];
-    if (t) atomicAdd(sum, t);
+    unsigned long long tot = 0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) tot += s[w];
+    if (tot) atomicAdd(sum, tot);
   }
 }
 
 // Every pending record owned by LP lp (unsorted) into out[0 .. *n) (at most cap), grid-stride
-// over every list.
+// over the logs of its block, both lists, the parked ones and its staging slots.
 __device__ __forceinline__ void gather_one(const Event* e, uint32_t lp, Event* out, uint32_t cap, uint32_t* n) {
   if (e->dst == lp) {
     const uint32_t k = atomicAdd(n, 1u);
@@ -524,9 +513,11 @@ __device__ __forceinline__ void gather_one(const Event* e, uint32_t lp, Event* o
 __global__ void cal_gather_kernel(LaqView q, uint32_t lp, Event* out, uint32_t cap, uint32_t* n) {
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  for (uint32_t b = 0; b < q.cal_windows; ++b) {
-    const uint32_t c = min(q.wlog_n[b], q.wlog_cap);
-    for (uint32_t i = t; i < c; i += stride) gather_one(&q.wlog[(size_t)b * q.wlog_cap + i], lp, out, cap, n);
+  const uint32_t b = lp >> q.cal_block_log2;
+  for (uint32_t x = 0; x < q.cal_windows; ++x) {
+    const uint32_t c = min(*cal::log_count(q, x, b), q.cal_sub_cap);
+    const Event* lg = cal::log_base(q, x, b);
+    for (uint32_t i = t; i < c; i += stride) gather_one(&lg[i], lp, out, cap, n);
   }
   for (uint32_t p = 0; p < 2; ++p) {
     const uint32_t c = min(q.over_n[p], q.over_cap);
@@ -534,10 +525,9 @@ __global__ void cal_gather_kernel(LaqView q, uint32_t lp, Event* out, uint32_t c
   }
   const uint32_t c = min(*q.parked_n, q.parked_cap);
   for (uint32_t i = t; i < c; i += stride) gather_one(&q.parked[i], lp, out, cap, n);
-  if (t == 0) {                         // both staging areas (slot-major between steps)
-    const uint32_t k0 = min(q.ins[lp], q.stage_cap), k1 = min(q.ins_alt[lp], q.stage_cap);
-    for (uint32_t i = 0; i < k0; ++i) gather_one(&q.stage[(size_t)i * q.num_lps + lp], lp, out, cap, n);
-    for (uint32_t i = 0; i < k1; ++i) gather_one(&q.stage_alt[(size_t)i * q.num_lps + lp], lp, out, cap, n);
+  if (t == 0) {
+    const uint32_t k0 = min(q.ins[lp], q.stage_cap);
+    for (uint32_t i = 0; i < k0; ++i) gather_one(&q.stage[(size_t)lp * q.stage_cap + i], lp, out, cap, n);
   }
 }
 
@@ -610,13 +600,21 @@ gpq_status LaqQueue::init_calendar() {
   GPQ_CHECK_ARG(std::isfinite(cfg.lookahead) && cfg.lookahead > 0.0f && std::isfinite(cfg.horizon),
                 "LAQ calendar: needs a finite lookahead > 0 and a finite horizon");
   GPQ_CHECK_ARG(!cfg.update_ids, "LAQ calendar: no decrease-key mode");
-  GPQ_CHECK_ARG((uint64_t)L * cfg.capacity_per_lp < (1ull << 32) && (uint64_t)L * cfg.far_per_lp < (1ull << 32),
+  // one log per window and block of kCalBlockLps LPs (the commit block), each holding
+  // capacity_per_lp records per LP of the block
+  const uint32_t blocks = (uint32_t)ceil_div(L, kCalBlockLps);
+  const uint64_t sub_cap = (uint64_t)kCalBlockLps * cfg.capacity_per_lp;
+  GPQ_CHECK_ARG((uint64_t)blocks * sub_cap < (1ull << 32) && (uint64_t)L * cfg.far_per_lp < (1ull << 32),
                 "LAQ calendar: num_lps too large for the window logs or the lists");
+  static_assert(kCalBlockLps * 64u <= 65536u, "a block's log indexes fit 16 bits");
   v_.num_lps = L;
   v_.cap_log2 = 1;                    // no sorted run in this mode: a two-slot ring keeps the words valid
   v_.stage_cap = cfg.stage_per_lp;
   v_.cal_windows = B;
-  v_.wlog_cap = L * cfg.capacity_per_lp;
+  v_.cal_blocks = blocks;
+  v_.cal_sub_cap = (uint32_t)sub_cap;
+  v_.cal_block_log2 = log2u(kCalBlockLps);
+  v_.wlog_cap = blocks * v_.cal_sub_cap;
   v_.over_cap = L * cfg.far_per_lp;
   v_.parked_cap = L * cfg.far_per_lp;
   far_step_ = 0.0f;
@@ -627,18 +625,16 @@ gpq_status LaqQueue::init_calendar() {
   GPQ_TRY(arena.alloc(&v_.ins, L));
   GPQ_TRY(arena.alloc(&v_.head, L));
   GPQ_TRY(arena.alloc(&sum_, 1));
+  const size_t counters = (size_t)B * blocks + 2;  // the logs' counters, then over_n[2]
   GPQ_TRY(arena.alloc(&v_.wlog, (size_t)B * v_.wlog_cap));
-  GPQ_TRY(arena.alloc(&v_.wlog_n, B + 2));        // + over_n[2]
-  v_.over_n = v_.wlog_n + B;
+  GPQ_TRY(arena.alloc(&v_.wlog_n, counters));
+  v_.over_n = v_.wlog_n + (size_t)B * blocks;
   GPQ_TRY(arena.alloc(&v_.over, 2 * (size_t)v_.over_cap));
   GPQ_TRY(arena.alloc(&v_.parked, (size_t)v_.parked_cap));
   GPQ_TRY(arena.alloc(&v_.parked_n, 1));
   GPQ_TRY(arena.alloc(&v_.cal_st, 1));
-  GPQ_TRY(arena.alloc(&v_.stage_alt, (size_t)L * v_.stage_cap));
-  GPQ_TRY(arena.alloc(&v_.ins_alt, L));
   GPQ_TRY(arena.alloc(&gather_, kGatherCap + 1));
-  GPQ_HIP(hipMemsetAsync(v_.ins_alt, 0, sizeof(uint32_t) * L, stream));
-  GPQ_HIP(hipMemsetAsync(v_.wlog_n, 0, sizeof(uint32_t) * (B + 2), stream));
+  GPQ_HIP(hipMemsetAsync(v_.wlog_n, 0, sizeof(uint32_t) * counters, stream));
   GPQ_HIP(hipMemsetAsync(v_.parked_n, 0, sizeof(uint32_t), stream));
   GPQ_HIP(hipMemsetAsync(v_.cal_st, 0, sizeof(CalState), stream));
   v_.ws = ws;
@@ -655,8 +651,6 @@ gpq_status LaqQueue::calendar_start() {
   hipLaunchKernelGGL(cal_start_kernel, dim3(1), dim3(1), 0, stream, v_, cfg.lookahead, cfg.horizon);
   hipLaunchKernelGGL(cal_fill_kernel, dim3(cal::kGroupBlocks), dim3(cal::kBlock), 0, stream, v_, cfg.horizon);
   hipLaunchKernelGGL(cal_fill_done_kernel, dim3(1), dim3(1), 0, stream, v_);
-  hipLaunchKernelGGL(cal_group_kernel, dim3(cal::kGroupBlocks), dim3(cal::kBlock), 0, stream, v_, cfg.horizon);
-  hipLaunchKernelGGL(cal_group_done_kernel, dim3(1), dim3(1), 0, stream, v_);
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
 }

@@ -58,11 +58,12 @@ struct LaqView {
   uint32_t upd_cap;
   uint32_t update_ids;
   // Calendar mode (cal_windows > 0; the PHOLD merge-and-commit step only): one append log per
-  // upcoming window, wlog[(cur + o) mod cal_windows] for window cur + o, count wlog_n[.]; events
-  // beyond the calendar in over[p] (p = parity of the step that appended them), count over_n[p];
-  // events at or past the end time in parked, count *parked_n.  A step groups the log of its
-  // window by LP into the staging area (stage / ins for even windows, stage_alt / ins_alt for
-  // odd ones, both slot-major) and commits from there.
+  // upcoming window and block of 2^cal_block_log2 LPs: the events of window cur + o for the LPs of
+  // block b are at wlog + ((cur + o) mod cal_windows) * wlog_cap + b * cal_sub_cap, count
+  // wlog_n[((cur + o) mod cal_windows) * cal_blocks + b]; events beyond the calendar in over[p]
+  // (p = parity of the step that appended them), count over_n[p]; events at or past the end
+  // time in parked, count *parked_n.  A step's commit blocks read the log of their own LPs for
+  // its window and group it by LP in LDS.
   Event* wlog;
   uint32_t* wlog_n;
   Event* over;
@@ -71,9 +72,13 @@ struct LaqView {
   uint32_t* parked_n;
   CalState* cal_st;
   uint32_t cal_windows;
-  uint32_t wlog_cap;
+  uint32_t wlog_cap;            // records per window: cal_blocks * cal_sub_cap
   uint32_t over_cap;
   uint32_t parked_cap;
+  uint32_t cal_sub_cap;         // records per window and block
+  uint32_t cal_blocks;          // blocks of 2^cal_block_log2 LPs (the commit block size)
+  uint32_t cal_block_log2;
+  uint32_t cal_pad;
   Event* far;                   // two-tier mode only (far_cap > 0)
   uint32_t* nfar;
   uint32_t* far_min;
