@@ -32,11 +32,10 @@
 namespace gpq {
 namespace cal {
 
-// LPs per commit block.  Every commit block adds its run to the counter of each window it
-// files successors for (about 40 of them per step), so fewer, larger blocks mean less
-// contention on those counters: 256 LPs 1.12e10, 512 LPs 1.25-1.28e10, 1024 LPs 1.25-1.28e10
-// events/s at 2^20 LPs (same box, the staging threads kept at 98 K).
-constexpr int kBlock = 512;
+// LPs per commit block, and so per log.  Smaller blocks mean more logs and shorter runs of the
+// self-addressed successors, larger ones fewer blocks in flight: at 2^20 LPs 64 LPs 1.37e10,
+// 128 LPs 1.59-1.66e10, 256 LPs 1.74-1.83e10, 512 LPs 1.60e10, 1024 LPs 1.42e10 events/s.
+constexpr int kBlock = 256;
 constexpr int kRegs = 8;          // events of one LP sorted in registers by one thread (fast path)
 constexpr int kWaveCand = 64;     // events of one LP on the slow path (one per lane)
 constexpr int kGroupBlocks = 2048;        // blocks of the standalone filing launch

@@ -32,7 +32,7 @@ class LaqQueue final : public QueueBase {
   // no window publication (callers that keep several windows publish their own).
   gpq_status merge_only();
   static constexpr uint32_t kBlockLps = 256;
-  static constexpr uint32_t kCalBlockLps = 512;   // LPs per commit block of the calendar step
+  static constexpr uint32_t kCalBlockLps = 256;   // LPs per commit block of the calendar step
   // Post blocks whose flag is 0 are skipped (their LPs hold nothing); nullptr = all live.
   void set_block_live(const uint8_t* flags) { v_.block_live = flags; }
   // Allocates the second staging buffer (stage_alt / ins_alt, empty) that a model step which
