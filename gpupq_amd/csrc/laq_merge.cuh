@@ -138,10 +138,12 @@ __device__ uint32_t laq_merge_group(const LaqView& q, uint32_t lp, const Stg* st
 // into the far store, in place after the far records kept untouched (all of them when the
 // horizon does not move, none when it does).  Returns the number of near keys; *n_far and
 // *fmin get the far count and the smallest far ts bits (+inf when empty).
-template <int G>
-__device__ __forceinline__ uint32_t split_candidates(const LaqView& q, uint32_t lp, const Event* stg, uint32_t k,
-                                                     uint32_t f, bool moving, uint32_t N, Key3* near, int g,
-                                                     uint64_t gmask, uint32_t* n_far_out, uint32_t* fmin_out) {
+// split_candidates_by takes the test for "below the horizon" as a predicate on the record.
+template <int G, class BelowFn>
+__device__ __forceinline__ uint32_t split_candidates_by(const LaqView& q, uint32_t lp, const Event* stg, uint32_t k,
+                                                        uint32_t f, bool moving, BelowFn below_horizon, Key3* near,
+                                                        int g, uint64_t gmask, uint32_t* n_far_out,
+                                                        uint32_t* fmin_out) {
   const int lane = lane_id();
   const uint32_t cands = (moving ? f : 0u) + k;
   Event* far = q.far + (size_t)lp * q.far_cap;
@@ -166,7 +168,7 @@ __device__ __forceinline__ uint32_t split_candidates(const LaqView& q, uint32_t 
       if (base + u * G >= cands) break;       // uniform across the group
       const Event& e = ev[u];
       const bool valid = base + u * G + (uint32_t)g < cands;
-      const bool is_near = valid && e.ts < N;
+      const bool is_near = valid && below_horizon(e);
       const bool is_far = valid && !is_near;
       const uint64_t bn = __ballot(is_near) & gmask, bf = __ballot(is_far) & gmask;
       const uint32_t pn = n_near + (uint32_t)__popcll(bn & below);
@@ -192,6 +194,14 @@ __device__ __forceinline__ uint32_t split_candidates(const LaqView& q, uint32_t 
   *n_far_out = n_far;
   *fmin_out = fmin;
   return n_near;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t split_candidates(const LaqView& q, uint32_t lp, const Event* stg, uint32_t k,
+                                                     uint32_t f, bool moving, uint32_t N, Key3* near, int g,
+                                                     uint64_t gmask, uint32_t* n_far_out, uint32_t* fmin_out) {
+  return split_candidates_by<G>(q, lp, stg, k, f, moving, [N](const Event& e) { return e.ts < N; }, near, g, gmask,
+                                n_far_out, fmin_out);
 }
 
 }  // namespace laq

@@ -272,7 +272,8 @@ gpq_status PholdModel::create(QueueBase* queue, const gpq_phold_config& c, Phold
   GPQ_CHECK_ARG(queue != nullptr, "null queue");
   GPQ_CHECK_ARG(c.num_lps == queue->cfg.num_lps, "phold num_lps must equal the queue's num_lps");
   GPQ_CHECK_ARG(c.num_lps > 0 && c.init_per_lp > 0, "phold: num_lps and init_per_lp must be > 0");
-  GPQ_CHECK_ARG(c.lookahead > 0.0f, "phold: lookahead must be > 0 (conservative window)");
+  GPQ_CHECK_ARG(c.lookahead > 0.0f && std::isfinite(c.lookahead), "phold: lookahead must be finite and > 0 "
+                "(conservative window)");
   GPQ_CHECK_ARG(c.lookahead == queue->cfg.lookahead, "phold lookahead must equal the queue's lookahead");
   GPQ_CHECK_ARG(c.end_time == queue->cfg.horizon, "phold end_time must equal the queue's horizon");
   GPQ_CHECK_ARG(!c.drain || queue->cfg.kind == GPQ_LAQ || queue->cfg.kind == GPQ_LHQ || queue->cfg.kind == GPQ_LSQ,

@@ -212,6 +212,323 @@ __global__ __launch_bounds__(kLaqBlock, 5) void laq_post_far_kernel(LaqView q, u
   }
 }
 
+// Two-tier post with a horizon per LP (open lists: lookahead = +inf, as the A* model's), after the
+// heap-backed LAQ's exchange for A* (src/local_array_queue.cu:150-180,515-681): there, an LP's
+// array that holds fewer than A_STAR_MAX_PARALLEL_OL_ENTRIES items takes its heap's best node,
+// and the items beyond the array's bound go to the heap.  Here the far store is the LP's
+// unsorted backing store, and the horizon -- a full key (ts, src, seq), near_lp[0/1/2][lp] --
+// separates the two: every ring key is below it and every far key at or above it, so the ring
+// head is the LP's minimum.  Per LP with staged records, or with far records while its ring
+// holds fewer than kOpenRefill:
+//  1. refill (ring short, far store not empty): the new horizon is chosen so that at most
+//     min(C/2, kFarList) - n - k far records come in (C ring slots, n ring records, k staged),
+//     and at least one; with no room left and records in the ring, the refill waits;
+//  2. split the candidates (the far records when refilling, then the staged ones) at the
+//     horizon, as the two-tier post does;
+//  3. evict (ring + near candidates > C, ring not empty): the horizon drops to the key of ring
+//     record j = C/2 - nn (0 when negative), but stays above the LP's smallest key; ring
+//     records and candidates at or above it go to the far store;
+//  4. merge the near candidates into the ring.
+// Limits (the queue's error flag, GPQ_ERR_CAPACITY): a full far store; more than C records
+// with the LP's smallest key (keys are unique in the A* model: (f, h, cell)).
+constexpr uint32_t kOpenRefill = 8;     // A_STAR_MAX_PARALLEL_OL_ENTRIES (src/config.h.a_star_heaps:92)
+constexpr uint32_t kOpenBuckets = 32;
+
+__device__ __forceinline__ uint32_t key_word(const Event& e, int lvl) {
+  return lvl == 0 ? e.ts : (lvl == 1 ? e.src : e.seq);
+}
+
+// The key whose words above lvl are (p0, p1), word lvl is v (a carry past 32 bits goes up), the
+// words below are 0.  Past the largest key: all ones (above every record, whose ts is a float's
+// bits or an A* f, below 2^31).
+__device__ __forceinline__ Key3 key_at(uint32_t p0, uint32_t p1, int lvl, uint64_t v) {
+  const Key3 top{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+  if (lvl == 0) return v >> 32 ? top : Key3{(uint32_t)v, 0u, 0u};
+  if (lvl == 1) return v >> 32 ? (p0 == 0xFFFFFFFFu ? top : Key3{p0 + 1u, 0u, 0u}) : Key3{p0, (uint32_t)v, 0u};
+  if (!(v >> 32)) return Key3{p0, p1, (uint32_t)v};
+  return p1 != 0xFFFFFFFFu ? Key3{p0, p1 + 1u, 0u} : (p0 == 0xFFFFFFFFu ? top : Key3{p0 + 1u, 0u, 0u});
+}
+
+// The smallest key above k.
+__device__ __forceinline__ Key3 key_next(const Key3& k) {
+  if (k.seq != 0xFFFFFFFFu) return Key3{k.ts, k.src, k.seq + 1u};
+  return key_at(k.ts, k.src, 1, (uint64_t)k.src + 1u);
+}
+
+template <int G>
+__device__ __forceinline__ Key3 group_min_key(Key3 k) {
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    const Key3 o{(uint32_t)__shfl_xor((int)k.ts, off, kWave), (uint32_t)__shfl_xor((int)k.src, off, kWave),
+                 (uint32_t)__shfl_xor((int)k.seq, off, kWave)};
+    if (key_less3(o, k)) k = o;
+  }
+  return k;
+}
+
+// Step 1: the new horizon of LP lp from its f > 0 far records: below it at most `budget`
+// records, and at least one.  Word by word (ts, then src, then seq) over the records whose
+// higher words equal the prefix already fixed: the range [vmin, vmax] of the word is cut into
+// kOpenBuckets buckets of width w; the buckets that fit the budget give the horizon; when the
+// first bucket alone is too full, its range is cut again, and a width-1 bucket fixes the word.
+template <int G>
+__device__ Key3 open_refill_key(const LaqView& q, uint32_t lp, uint32_t f, uint32_t budget, uint32_t* bkt, int g,
+                                uint64_t gmask) {
+  const Event* far = q.far + (size_t)lp * q.far_cap;
+  uint32_t p0 = 0, p1 = 0, lo = 0, hi = 0xFFFFFFFFu;
+  int lvl = 0;
+  // each round narrows the range 32-fold or fixes a word: at most 7 rounds per word
+  for (int round = 0; round < 24; ++round) {
+    uint32_t vmin = 0xFFFFFFFFu, vmax = 0;
+    for (uint32_t i = (uint32_t)g; i < f; i += G) {
+      const Event e = load_event(&far[i]);
+      const uint32_t v = key_word(e, lvl);
+      if ((lvl < 1 || e.ts == p0) && (lvl < 2 || e.src == p1) && v >= lo && v <= hi) {
+        vmin = min(vmin, v);
+        vmax = max(vmax, v);
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+      vmin = min(vmin, (uint32_t)__shfl_xor((int)vmin, off, kWave));
+      vmax = max(vmax, (uint32_t)__shfl_xor((int)vmax, off, kWave));
+    }
+    const uint64_t w = ((uint64_t)vmax - vmin + kOpenBuckets) / kOpenBuckets;   // >= 1: the range fits
+    for (uint32_t b = (uint32_t)g; b < kOpenBuckets; b += G) bkt[b] = 0;
+    wave_lds_sync();
+    for (uint32_t i = (uint32_t)g; i < f; i += G) {
+      const Event e = load_event(&far[i]);
+      const uint32_t v = key_word(e, lvl);
+      if ((lvl < 1 || e.ts == p0) && (lvl < 2 || e.src == p1) && v >= vmin && v <= vmax)
+        atomicAdd(&bkt[(uint32_t)(((uint64_t)v - vmin) / w)], 1u);
+    }
+    wave_lds_sync();
+    uint32_t cum = 0, nb = 0;
+    for (uint32_t b = 0; b < kOpenBuckets; ++b) {
+      cum += bkt[b];
+      if (cum > budget) break;
+      nb = b + 1;
+    }
+    wave_lds_sync();                               // the buckets are read before the next round clears them
+    if (nb) return key_at(p0, p1, lvl, (uint64_t)vmin + nb * w);
+    if (w > 1) {                                   // the first bucket alone is too full: cut it again
+      lo = vmin;
+      hi = (uint32_t)((uint64_t)vmin + w - 1u);
+      continue;
+    }
+    if (lvl == 2) break;                           // one full key held more than the budget: take it
+    if (lvl == 0) p0 = vmin;
+    else p1 = vmin;
+    ++lvl;
+    lo = 0;
+    hi = 0xFFFFFFFFu;
+  }
+  // copies of one full key fill more than the budget: the smallest far key and its copies
+  Key3 m{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+  for (uint32_t i = (uint32_t)g; i < f; i += G) {
+    const Event e = load_event(&far[i]);
+    const Key3 k{e.ts, e.src, e.seq};
+    if (key_less3(k, m)) m = k;
+  }
+  return key_next(group_min_key<G>(m));
+}
+
+// Step 3: ring run of *n records (> 0) at slot r, near candidates near[0 .. *nn), with
+// *n + *nn > C.  The new horizon x is the key of ring record j = C/2 - nn (0 when negative), but
+// at least the key after the LP's smallest, so that the LP's minimum stays in the ring.  Moves
+// the ring's records from the first at or above x on, and the candidates at or above x, to the
+// far store after its *n_far records; updates *n, *nn, *n_far, *fmin and the horizon *H.
+template <int G>
+__device__ void open_evict(const LaqView& q, uint32_t lp, uint32_t r, uint32_t* n_io, Key3* near, uint32_t* nn_io,
+                           uint32_t* n_far_io, uint32_t* fmin_io, Key3* H_io, int g, uint64_t gmask) {
+  const uint64_t below = (1ull << lane_id()) - 1ull;
+  const uint32_t mask = (1u << q.cap_log2) - 1u;
+  const uint32_t half = (1u << q.cap_log2) / 2u;
+  const uint32_t n = *n_io, nn = *nn_io;
+  const Event* ring = q.ring + ((size_t)lp << q.cap_log2);
+  Event* far = q.far + (size_t)lp * q.far_cap;
+  const uint32_t j = half > nn ? half - nn : 0u;   // j < n, since n + nn > C >= half
+  const Event h = load_event(&ring[r & mask]);     // the smallest key: the run's first or a candidate's
+  Key3 m{h.ts, h.src, h.seq};
+  for (uint32_t i = (uint32_t)g; i < nn; i += G)
+    if (key_less3(near[i], m)) m = near[i];
+  m = group_min_key<G>(m);
+  const Event ej = load_event(&ring[(r + j) & mask]);
+  Key3 x{ej.ts, ej.src, ej.seq};
+  const Key3 m1 = key_next(m);
+  if (key_less3(x, m1)) x = m1;
+  uint32_t j0 = 0;                                 // ring records below x: a prefix of the sorted run
+  for (uint32_t base = 0; base < n; base += G) {
+    const uint32_t i = base + (uint32_t)g;
+    bool lt = false;
+    if (i < n) {
+      const Event e = load_event(&ring[(r + i) & mask]);
+      lt = key_less3(Key3{e.ts, e.src, e.seq}, x);
+    }
+    j0 += (uint32_t)__popcll(__ballot(lt) & gmask);
+  }
+  uint32_t nf = *n_far_io, fmin = *fmin_io;
+  for (uint32_t i = j0 + (uint32_t)g; i < n; i += G) {   // ring records [j0, n): x and above
+    const Event e = load_event(&ring[(r + i) & mask]);
+    const uint32_t dst = nf + (i - j0);
+    if (dst < q.far_cap) store_event(&far[dst], e);
+    else atomicOr(&q.ws->error, kErrQueueOverflow);
+    fmin = min(fmin, e.ts);
+  }
+  nf += n - j0;
+  uint32_t kept = 0;                               // candidates: below x kept (compacted in place), the rest out
+  for (uint32_t base = 0; base < nn; base += G) {
+    const uint32_t i = base + (uint32_t)g;
+    const bool valid = i < nn;
+    const Key3 k = valid ? near[i] : Key3{0u, 0u, 0u};
+    const bool keep = valid && key_less3(k, x), out = valid && !keep;
+    const uint64_t bk = __ballot(keep) & gmask, bo = __ballot(out) & gmask;
+    if (keep) near[kept + (uint32_t)__popcll(bk & below)] = k;
+    if (out) {
+      const uint32_t dst = nf + (uint32_t)__popcll(bo & below);
+      if (dst < q.far_cap) store_event(&far[dst], Event{k.ts, lp, k.src, k.seq});
+      else atomicOr(&q.ws->error, kErrQueueOverflow);
+      fmin = min(fmin, k.ts);
+    }
+    kept += (uint32_t)__popcll(bk);
+    nf += (uint32_t)__popcll(bo);
+  }
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) fmin = min(fmin, (uint32_t)__shfl_xor((int)fmin, off, kWave));
+  *n_io = j0;
+  *nn_io = kept;
+  *n_far_io = nf;
+  *fmin_io = fmin;
+  *H_io = x;
+}
+
+template <int G, int SL>
+__global__ __launch_bounds__(kLaqBlock) void laq_post_open_kernel(LaqView q, uint32_t* partial) {
+  __shared__ uint32_t s_work[kLaqBlock];
+  __shared__ uint32_t s_k[kLaqBlock], s_n[kLaqBlock], s_r[kLaqBlock], s_f[kLaqBlock];
+  __shared__ uint32_t s_hist[kLaqBlock];
+  __shared__ uint32_t s_bkt[kLaqBlock / G][kOpenBuckets];
+  __shared__ uint32_t s_count, s_inserted, s_moved;
+  __shared__ unsigned long long s_aux;
+  __shared__ Key3 s_near[kLaqBlock / G][kFarList];
+  const uint32_t nb = gridDim.x;
+  const uint32_t L = q.num_lps;
+  const uint32_t lp = blockIdx.x * kLaqBlock + threadIdx.x;
+  const uint32_t C = 1u << q.cap_log2;
+  const uint32_t refill_below = min(kOpenRefill, max(1u, C / 4u));
+  if (threadIdx.x == 0) {
+    s_count = 0;
+    s_inserted = 0;
+    s_moved = 0;
+    s_aux = 0;
+  }
+  __syncthreads();
+  if (lp < L) {
+    const uint32_t staged = q.ins[lp];
+    const uint32_t f = q.nfar[lp];
+    const uint32_t n = q.len[lp];
+    if (staged || (f && n < refill_below)) {
+      const uint32_t slot = atomicAdd(&s_count, 1u);
+      s_work[slot] = threadIdx.x;
+      s_k[threadIdx.x] = staged < q.stage_cap ? staged : q.stage_cap;
+      s_f[threadIdx.x] = f;
+      s_n[threadIdx.x] = n;
+      s_r[threadIdx.x] = q.rd[lp];
+      if (q.stats) atomicAdd(&s_inserted, staged);
+    }
+  }
+  __syncthreads();
+  const int lane = lane_id();
+  const int g = lane % G;
+  const int gbase = lane - g;
+  const int group = (int)threadIdx.x / G;
+  const uint64_t gmask = (G == kWave ? ~0ull : ((1ull << G) - 1ull)) << gbase;
+  uint32_t moved = 0;
+  for (uint32_t i = (uint32_t)group; i < s_count; i += kLaqBlock / G) {
+    const uint32_t t = s_work[i];
+    const uint32_t lpt = blockIdx.x * kLaqBlock + t;
+    const uint32_t k = s_k[t], f = s_f[t], n0 = s_n[t], r = s_r[t];
+    uint32_t n = n0;
+    Key3 H{q.near_lp[lpt], q.near_lp[L + lpt], q.near_lp[2 * L + lpt]};
+    bool refill = f && n < refill_below;
+    if (refill) {
+      const uint32_t room = min(C / 2u, kFarList);
+      const uint32_t budget = room > n + k ? room - n - k : 0u;
+      if (!budget && n) {
+        refill = false;                            // no room: the ring's records are popped first
+      } else {
+        H = open_refill_key<G>(q, lpt, f, budget ? budget : 1u, &s_bkt[group][0], g, gmask);
+        if (q.stats && g == 0) atomicAdd(&s_aux, 32ull * f);   // far records read and written once
+      }
+    }
+    const Event* stg = q.stage + (size_t)lpt * q.stage_cap;
+    uint32_t n_far, fmin;
+    const uint32_t n_near = split_candidates_by<G>(
+        q, lpt, stg, k, f, refill, [H](const Event& e) { return key_less3(Key3{e.ts, e.src, e.seq}, H); },
+        &s_near[group][0], g, gmask, &n_far, &fmin);
+    wave_lds_sync();
+    uint32_t nn = n_near < kFarList ? n_near : kFarList;
+    if (n + nn > C && n) {
+      open_evict<G>(q, lpt, r, &n, &s_near[group][0], &nn, &n_far, &fmin, &H, g, gmask);
+      wave_lds_sync();
+    }
+    const bool fits = n + nn <= C;                 // else: more than C records of the LP's smallest key
+    if (g == 0) {
+      if (!fits) atomicOr(&q.ws->error, kErrQueueOverflow);
+      q.ins[lpt] = 0;
+      q.nfar[lpt] = n_far < q.far_cap ? n_far : q.far_cap;
+      q.far_min[lpt] = fmin;
+      q.near_lp[lpt] = H.ts;
+      q.near_lp[L + lpt] = H.src;
+      q.near_lp[2 * L + lpt] = H.seq;
+      if (n != n0) {
+        q.len[lpt] = n;
+        if (!n) q.head[lpt] = kInfBits;            // a record merged at index 0 sets it again
+      }
+    }
+    if (nn && fits) moved += laq_merge_group<G, SL>(q, lpt, &s_near[group][0], nn, n, r, g, gbase, s_hist + group * G);
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (q.stats && g == 0 && moved) atomicAdd(&s_moved, moved);
+  __syncthreads();
+  if (q.stats && threadIdx.x == 0) {
+    const uint32_t lps_here = min((uint32_t)kLaqBlock, q.num_lps - blockIdx.x * kLaqBlock);
+    atomicAdd(&q.stats[0], (unsigned long long)s_inserted);
+    atomicAdd(&q.stats[1], (unsigned long long)s_moved);
+    atomicAdd(&q.stats[2], (unsigned long long)s_count);
+    atomicAdd(&q.stats[3], (unsigned long long)lps_here);
+    if (s_aux) atomicAdd(&q.stats[4], s_aux);
+  }
+  // partials: ring-head minimum, far minimum, far count
+  __shared__ uint32_t s_w[3][kLaqBlock / kWave];
+  uint32_t hv = kInfBits, fv = kInfBits, fc = 0;
+  if (lp < q.num_lps) {
+    hv = q.head[lp];
+    fv = q.far_min[lp];
+    fc = q.nfar[lp];
+  }
+  hv = wave_min_u32(hv);
+  fv = wave_min_u32(fv);
+  fc = wave_sum_u32(fc);
+  if (lane == 0) {
+    s_w[0][threadIdx.x / kWave] = hv;
+    s_w[1][threadIdx.x / kWave] = fv;
+    s_w[2][threadIdx.x / kWave] = fc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t a = s_w[0][0], b = s_w[1][0], cc = s_w[2][0];
+    for (int w = 1; w < kLaqBlock / kWave; ++w) {
+      a = s_w[0][w] < a ? s_w[0][w] : a;
+      b = s_w[1][w] < b ? s_w[1][w] : b;
+      cc += s_w[2][w];
+    }
+    partial[blockIdx.x] = a;
+    partial[nb + blockIdx.x] = b;
+    partial[2 * nb + blockIdx.x] = cc;
+  }
+}
+
 // One block: gmin = min(ring heads, far minima); the window ends at
 // min(gmin + lookahead, horizon, near horizon); when the next window could reach the near
 // horizon, its move to gmin + 2 lookahead + step is scheduled for the next post.
@@ -289,12 +606,17 @@ __global__ void laq_init_kernel(LaqView q) {
       q.nfar[lp] = 0;
       q.far_min[lp] = kInfBits;
     }
+    if (q.near_lp) {                           // open lists: everything to the ring until it fills
+      q.near_lp[lp] = 0xFFFFFFFFu;
+      q.near_lp[q.num_lps + lp] = 0xFFFFFFFFu;
+      q.near_lp[2 * q.num_lps + lp] = 0xFFFFFFFFu;
+    }
   }
   if (lp == 0) {
     q.ws->gmin = kInfBits;
     q.ws->wend = 0;
     q.ws->done = 0;
-    q.ws->near_end = 0;
+    q.ws->near_end = q.near_lp ? kInfBits : 0u;   // open lists: no global horizon
     q.ws->near_next = 0;
     q.ws->windows = 0;
     q.ws->error = 0;
@@ -560,16 +882,22 @@ gpq_status LaqQueue::init() {
   v_.stats = stats;
   partial_slots = ceil_div(L, kLaqBlock);
   if (cfg.far_per_lp) {
-    GPQ_CHECK_ARG(std::isfinite(cfg.lookahead) && cfg.lookahead > 0.0f, "LAQ two-tier: needs a finite lookahead");
+    // lookahead = +inf: open lists (a horizon per LP); else a global near horizon
+    const bool open = std::isinf(cfg.lookahead) && cfg.lookahead > 0.0f;
+    GPQ_CHECK_ARG(open || (std::isfinite(cfg.lookahead) && cfg.lookahead > 0.0f),
+                  "LAQ two-tier: needs a finite lookahead > 0, or lookahead = +inf (open lists)");
     GPQ_CHECK_ARG(cfg.far_per_lp <= 4096, "LAQ two-tier: far_per_lp must be <= 4096");
     GPQ_CHECK_ARG(group_ <= 16, "LAQ two-tier: capacity_per_lp must be <= 128");
     GPQ_CHECK_ARG(cfg.stage_per_lp <= kFarList, "LAQ two-tier: stage_per_lp must be <= 64");
     GPQ_CHECK_ARG((uint64_t)L * cfg.far_per_lp < (1ull << 32), "LAQ two-tier: num_lps * far_per_lp too large");
+    GPQ_CHECK_ARG(!open || cfg.capacity_per_lp >= 2u * cfg.stage_per_lp,
+                  "LAQ two-tier open lists: capacity_per_lp must be >= 2 * stage_per_lp");
     v_.far_cap = cfg.far_per_lp;
-    far_step_ = cfg.far_step > 0.0f ? cfg.far_step : 4.0f * cfg.lookahead;
+    far_step_ = open ? 0.0f : (cfg.far_step > 0.0f ? cfg.far_step : 4.0f * cfg.lookahead);
     GPQ_TRY(arena.alloc(&v_.far, (size_t)L * v_.far_cap));
     GPQ_TRY(arena.alloc(&v_.nfar, L));
     GPQ_TRY(arena.alloc(&v_.far_min, L));
+    if (open) GPQ_TRY(arena.alloc(&v_.near_lp, 3 * (size_t)L));   // the horizon's three key words per LP
   }
   if (cfg.update_ids) {
     GPQ_CHECK_ARG(!cfg.far_per_lp, "LAQ decrease-key mode needs the one-tier LAQ (far_per_lp = 0)");
@@ -715,6 +1043,14 @@ gpq_status LaqQueue::merge_publish() {
 gpq_status LaqQueue::merge_only() {
   static_assert(kLaqBlock == (int)LaqQueue::kBlockLps, "LAQ post block size");
   const dim3 grid(partial_slots), block(kLaqBlock);
+  if (v_.near_lp) {
+    if (group_ == 8)
+      hipLaunchKernelGGL((laq_post_open_kernel<8, kSlots>), grid, block, 0, stream, v_, partial);
+    else
+      hipLaunchKernelGGL((laq_post_open_kernel<16, kSlots>), grid, block, 0, stream, v_, partial);
+    GPQ_HIP(hipGetLastError());
+    return GPQ_OK;
+  }
   if (v_.far_cap) {
     if (group_ == 8)
       hipLaunchKernelGGL((laq_post_far_kernel<8, kSlots>), grid, block, 0, stream, v_, partial);

@@ -10,7 +10,12 @@
 // Two-tier mode (far_cap > 0): the ring keeps only events below the near horizon
 // ws->near_end; the others wait unsorted in
 //   far   [L][F]  Event   count nfar[lp], smallest ts bits far_min[lp] (+inf when empty)
-// and move into the ring when the horizon advances.
+// and move into the ring when the horizon advances.  With lookahead = +inf (open lists, the
+// A* model's) each LP has a horizon of its own, a full key (ts, src, seq) in near_lp[0/1/2][L]:
+// the ring keeps the LP's records below it, so the ring head is the LP's minimum.  A post moves
+// the smallest far records into a ring that holds fewer than 8, and the ring's largest into the
+// far store when it would overflow (the heap-backed LAQ's exchange,
+// src/local_array_queue.cu:150-180,515-681).
 // Inserts never touch the ring: they land in the staging slots and are merged by
 // queue_post.  The reference appends after fel_write and insertion-sorts in place
 // (src/local_array_queue.cu:486-513, src/sort.cu:329-356).
@@ -82,6 +87,7 @@ struct LaqView {
   Event* far;                   // two-tier mode only (far_cap > 0)
   uint32_t* nfar;
   uint32_t* far_min;
+  uint32_t* near_lp;            // two-tier open-list mode (lookahead = +inf) only: per-LP horizon, [3][L]
   uint32_t far_cap;
   uint32_t num_lps;
   uint32_t cap_log2;
@@ -138,6 +144,11 @@ struct LaqView {
     if (far_cap) {
       nfar[lp] = 0;
       far_min[lp] = kInfBits;
+    }
+    if (near_lp) {
+      near_lp[lp] = 0xFFFFFFFFu;
+      near_lp[num_lps + lp] = 0xFFFFFFFFu;
+      near_lp[2 * num_lps + lp] = 0xFFFFFFFFu;
     }
   }
   // queue_insert_or_update (src/local_array_queue.cu:830-846).  Without decrease-key mode a

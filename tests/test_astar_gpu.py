@@ -16,6 +16,9 @@ INF = float("inf")
 
 
 def open_lists(gpq, kind, lists=64, cap=64, stage=32):
+    if kind == "laq2":        # two-tier LAQ: a short ring per list backed by an unsorted far store
+        return gpq.Queue("laq", num_lps=lists, capacity_per_lp=32, stage_per_lp=16, far_per_lp=1024, lookahead=INF,
+                         horizon=INF)
     return gpq.Queue(kind, num_lps=lists, capacity_per_lp=cap, stage_per_lp=stage, lookahead=INF, horizon=INF)
 
 
@@ -30,7 +33,7 @@ def oracle_grid(oracle, w, h, seed, tag, q, t, forced=()):
     return g
 
 
-@pytest.mark.parametrize("kind", ["laq", "lhq", "lsq"])
+@pytest.mark.parametrize("kind", ["laq", "laq2", "lhq", "lsq"])
 @pytest.mark.parametrize("conn", [4, 8])
 def test_single_query_matches_oracle(gpq, oracle, kind, conn):
     w, h = 97, 61
@@ -49,6 +52,36 @@ def test_single_query_matches_oracle(gpq, oracle, kind, conn):
             if cost >= 0:
                 check_path(a.path(), ref_grid, w, conn, s, t, cost)
             a.model_finish()
+
+
+@pytest.mark.parametrize("conn,ring,stage", [(4, 32, 16), (4, 64, 32), (8, 64, 32)])
+def test_single_query_two_tier_open_lists(gpq, oracle, conn, ring, stage):
+    """The two-tier LAQ as open lists (the heap-backed LAQ's A* exchange, src/local_array_queue.cu:
+    150-180,612-627): each list keeps its best entries in a short sorted ring, the rest in an
+    unsorted far store; rings refill from the far stores as they run short.  Costs exact, plain ==
+    graph, refills counted, no overflow.  Eight lists: their open sets outgrow the rings, with
+    many entries of one f on the obstacle-free grid.  A list's staging takes the inserts of one
+    step (up to lists x connectivity in all)."""
+    cases = [(97, 61, (3, 5), (90, 57), 0.25, 1), (300, 200, (0, 0), (299, 199), 0.3, 4),
+             (256, 256, (128, 0), (128, 255), 0.0, 2), (1000, 700, (5, 690), (990, 3), 0.35, 11)]
+    refilled = 0
+    for (w, h, s, t, p, seed) in cases:
+        with gpq.Queue("laq", num_lps=8, capacity_per_lp=ring, stage_per_lp=stage, far_per_lp=4096, lookahead=INF,
+                       horizon=INF, collect_stats=True) as q:
+            a = gpq.Astar(q, w, h, connectivity=conn, source=s, target=t, seed=seed, threshold=p)
+            a.model_init()
+            ref_grid = oracle_grid(oracle, w, h, seed, 0, 0, thr(p), forced=(s, t))
+            rep = a.run("graph", 16)
+            cost = oracle.dijkstra(ref_grid, w, h, conn, s, t)
+            assert rep["cost"] == cost
+            assert rep["error_flags"] == 0
+            if cost >= 0:
+                check_path(a.path(), ref_grid, w, conn, s, t, cost)
+            refilled += q.stats()["aux_bytes"]
+            a.model_init()
+            assert a.run("plain")["cost"] == cost
+            a.model_finish()
+    assert refilled > 0                               # the far stores were used and refilled the rings
 
 
 @pytest.mark.parametrize("conn", [4, 8])

@@ -189,6 +189,121 @@ def test_laq_decrease_key(gpq):
         assert q.queue_check_phold() == 3 and q.error_flags() == 0
 
 
+def int_keys(gpq, items):
+    """items: (key, dst, src, seq) with integer keys (the open-list mode's, A*'s f values)."""
+    a = np.zeros(len(items), dtype=gpq.event_dtype)
+    for i, it in enumerate(items):
+        a[i] = it
+    return a
+
+
+def test_equal_keys_in_one_post_are_all_kept(gpq):
+    # a model may stage one key twice in a step: both copies stay, side by side
+    with gpq.Queue("laq", num_lps=1, capacity_per_lp=16, stage_per_lp=8, lookahead=1.0, horizon=1e9) as q:
+        q.insert(ev(gpq, [(3, 0, 1, 1), (3, 0, 1, 1), (2, 0, 0, 0), (3, 0, 1, 1)]))
+        q.queue_post()
+        q.insert(ev(gpq, [(3, 0, 1, 1), (1, 0, 0, 0)]))
+        q.queue_post()
+        got = [(gpq.bits_f32(e["ts"]), int(e["src"]), int(e["seq"])) for e in q.lp_contents(0)]
+        assert got == [(1.0, 0, 0), (2.0, 0, 0)] + [(3.0, 1, 1)] * 4
+        assert q.error_flags() == 0
+
+
+@pytest.mark.parametrize("cap,stage,far,dups", [(16, 8, 256, False), (64, 16, 512, False),
+                                                (128, 32, 1024, False), (16, 8, 256, True)])
+def test_laq_open_lists_two_tier_random_rounds(gpq, cap, stage, far, dups):
+    """Two-tier LAQ with lookahead = +inf (open lists, a horizon per LP): random insert rounds
+    with many equal timestamps (and, with dups, equal full keys), a pop of every LP's head on
+    some rounds only (so rings overflow into the far stores), then pops until every LP is empty.
+    Every pop must be the LP's smallest entry of a per-LP sorted-list model, and the contents
+    must equal the model's."""
+    rng = np.random.default_rng(cap + far + dups)
+    L = 37
+    model = [[] for _ in range(L)]
+    seqs = np.zeros(L, np.int64)
+    inf = float("inf")
+
+    def pop_round():
+        items, _ = q.peek(set_done=True)
+        expect = sorted(m[0] for m in model if m)
+        assert sorted(key_list(items)) == expect
+        for m in model:
+            if m:
+                m.pop(0)
+
+    with gpq.Queue("laq", num_lps=L, capacity_per_lp=cap, stage_per_lp=stage, far_per_lp=far, lookahead=inf,
+                   horizon=inf, collect_stats=True) as q:
+        for rnd in range(20):
+            batch = []
+            for lp in range(L):
+                for _ in range(int(rng.integers(0, stage + 1))):
+                    src = int(rng.integers(0, 5))
+                    seq = int(rng.integers(0, 3)) if dups else int(seqs[src])
+                    batch.append((int(rng.integers(rnd, rnd + (8 if dups else 40))), lp, src, seq))
+                    seqs[src] += 1
+            order = rng.permutation(len(batch))
+            q.insert(int_keys(gpq, [batch[i] for i in order]))
+            for (key, lp, src, seq) in batch:
+                model[lp].append((key, src, seq))
+            q.queue_post()
+            for m in model:
+                m.sort()
+            for lp in (0, 11, L - 1):
+                assert key_list(q.lp_contents(lp)) == model[lp]
+            if rnd % 3 == 2:
+                pop_round()
+                q.queue_post()
+        assert q.queue_check_phold() == sum(len(m) for m in model)
+        while any(model):
+            pop_round()
+            q.queue_post()
+        assert q.queue_check_phold() == 0
+        assert q.error_flags() == 0
+        assert q.stats()["aux_bytes"] > 0          # refills from the far stores happened
+
+
+def test_laq_open_lists_equal_keys_pass_through_a_short_ring(gpq):
+    """48 entries with one ts through a ring of 32: the horizon is a full key, so the far store
+    takes part of them and refills the ring in (ts, src, seq) order."""
+    inf = float("inf")
+    with gpq.Queue("laq", num_lps=1, capacity_per_lp=32, stage_per_lp=16, far_per_lp=64, lookahead=inf,
+                   horizon=inf, collect_stats=True) as q:
+        want = []
+        for r in range(3):
+            items = [(5, 0, (7 * i) % 3, 16 * r + i) for i in range(16)]
+            want += [(5, src, seq) for (_, _, src, seq) in items]
+            q.insert(int_keys(gpq, items))
+            q.queue_post()
+        assert key_list(q.lp_contents(0)) == sorted(want)
+        got = []
+        while len(got) < 48:
+            items, _ = q.peek(set_done=True)
+            assert len(items) == 1
+            got += key_list(items)
+            q.queue_post()
+        assert got == sorted(want)
+        assert q.error_flags() == 0 and q.queue_check_phold() == 0 and q.stats()["aux_bytes"] > 0
+
+
+def test_laq_open_lists_far_overflow_is_reported(gpq):
+    inf = float("inf")
+    with gpq.Queue("laq", num_lps=1, capacity_per_lp=32, stage_per_lp=16, far_per_lp=16, lookahead=inf,
+                   horizon=inf) as q:
+        for r in range(2):                         # 16 + 16 fill the ring
+            q.insert(int_keys(gpq, [(5, 0, 0, 16 * r + i) for i in range(16)]))
+            q.queue_post()
+        assert q.queue_check_phold() == 32 and q.error_flags() == 0
+        q.insert(int_keys(gpq, [(5, 0, 0, 32 + i) for i in range(16)]))
+        q.queue_post()                             # 48 > 32: more than 16 go to the far store
+        assert q.error_flags() & 2
+
+
+def test_laq_open_lists_needs_room_for_two_stages(gpq):
+    with pytest.raises(gpq.GpqError):
+        gpq.Queue("laq", num_lps=4, capacity_per_lp=16, stage_per_lp=16, far_per_lp=64, lookahead=float("inf"),
+                  horizon=float("inf"))
+
+
 def test_laq_decrease_key_needs_one_tier(gpq):
     with pytest.raises(gpq.GpqError):
         gpq.Queue("laq", num_lps=4, capacity_per_lp=64, stage_per_lp=4, lookahead=0.1, horizon=1.0, far_per_lp=64,
