@@ -495,10 +495,13 @@ def astar_single_bench(gpq, log, cpu=True, tile=32, drive="graph"):
     w = h = 4096
     inf = float("inf")
 
-    def one(lists, tile_side, decrease_key=False):
+    def one(lists, tile_side, decrease_key=False, two_tier=False):
         if decrease_key:      # one entry per cell in a LAQ, cheaper paths update it
             q = gpq.Queue("laq", num_lps=lists, capacity_per_lp=64, stage_per_lp=16, lookahead=inf, horizon=inf,
                           update_ids=w * h)
+        elif two_tier:        # a 32-record ring per list backed by its far store
+            q = gpq.Queue("laq", num_lps=lists, capacity_per_lp=32, stage_per_lp=16, far_per_lp=256, lookahead=inf,
+                          horizon=inf)
         else:
             q = gpq.Queue("lhq", num_lps=lists, capacity_per_lp=64, stage_per_lp=16, lookahead=inf, horizon=inf)
         a = gpq.Astar(q, w, h, connectivity=4, seed=7, threshold=0.2, tile=tile_side)
@@ -514,6 +517,7 @@ def astar_single_bench(gpq, log, cpu=True, tile=32, drive="graph"):
     rep, grid = one(64, tile)
     rep_q, _ = one(1 << 16, 0)
     rep_u, _ = one(1 << 16, 0, decrease_key=True)
+    rep_t, _ = one(1 << 16, 0, two_tier=True)
     out = {"grid": "4096x4096, 4-connected, Bernoulli(0.2), seed 7", "search": "tiles %dx%d" % (tile, tile),
            "drive": drive,
            "cost": rep["cost"], "seconds": rep["seconds"], "steps": rep["steps"], "expanded": rep["expanded"],
@@ -522,10 +526,13 @@ def astar_single_bench(gpq, log, cpu=True, tile=32, drive="graph"):
                            "steps": rep_q["steps"], "expanded": rep_q["expanded"]},
            "queue_lists_decrease_key": {"open_lists": 1 << 16, "queue": "laq, decrease-key", "cost": rep_u["cost"],
                                         "seconds": rep_u["seconds"], "steps": rep_u["steps"],
-                                        "expanded": rep_u["expanded"]}}
-    if not rep_q["cost"] == rep_u["cost"] == rep["cost"]:
-        raise RuntimeError("A* single query: tile search %d, queue lists %d, decrease-key lists %d" %
-                           (rep["cost"], rep_q["cost"], rep_u["cost"]))
+                                        "expanded": rep_u["expanded"]},
+           "queue_lists_two_tier": {"open_lists": 1 << 16, "queue": "laq two-tier (ring 32, far 256)",
+                                    "cost": rep_t["cost"], "seconds": rep_t["seconds"], "steps": rep_t["steps"],
+                                    "expanded": rep_t["expanded"]}}
+    if not rep_q["cost"] == rep_u["cost"] == rep_t["cost"] == rep["cost"]:
+        raise RuntimeError("A* single query: tile search %d, queue lists %d, decrease-key lists %d, two-tier %d" %
+                           (rep["cost"], rep_q["cost"], rep_u["cost"], rep_t["cost"]))
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_lib import Oracle

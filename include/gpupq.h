@@ -65,9 +65,15 @@ typedef struct {
   uint32_t block_threads;    /* launch width of the per-LP kernels (multiple of 64, <= 1024; 0 = 256) */
   uint32_t collect_stats;    /* 1: post kernels count algorithmic work (gpq_queue_stats_get); costs atomics */
   uint32_t far_per_lp;       /* LAQ two-tier mode (0 = off): unsorted far store per LP for events at or
-                                beyond the near horizon; the sorted ring keeps only events below it */
-  float far_step;            /* LAQ two-tier: the near horizon moves to (minimum + lookahead + far_step)
-                                when a window reaches it (0 = 4 * lookahead) */
+                                beyond the near horizon; the sorted ring keeps only events below it.
+                                With lookahead = +inf (open lists, as A*'s) each LP has a horizon of its
+                                own (a full key): a ring holding fewer than 8 records refills from its far
+                                store, a full ring moves its largest records there (the heap-backed LAQ's
+                                exchange, src/local_array_queue.cu:150-180,515-681); needs
+                                capacity_per_lp >= 2 * stage_per_lp */
+  float far_step;            /* LAQ two-tier with a finite lookahead: the near horizon moves to
+                                (minimum + lookahead + far_step) when a window reaches it
+                                (0 = 4 * lookahead) */
   uint32_t update_ids;       /* LAQ decrease-key mode (0 = off; one-tier LAQ only): item ids are
                                 gpq_event.seq < update_ids (the A* cell); queue_insert_or_update of an
                                 id already in the queue replaces its entry when the new key is smaller
@@ -81,12 +87,12 @@ typedef struct {
                                 global (the minimum over all heaps) */
   uint32_t calendar_windows;  /* LAQ calendar mode (0 = off; a power of two in [8, 256]; needs far_per_lp):
                                 for the PHOLD merge-and-commit step (drain 2) only, whose windows end one
-                                lookahead apart: each LP keeps an unsorted bucket of capacity_per_lp
-                                records (<= 64) for each of the next calendar_windows windows, and a step
-                                reads only the bucket of the window it commits.  Events beyond the
-                                calendar, at or past the horizon, or for a full bucket pass through a
-                                per-LP inbox of stage_per_lp records into the far store.  Same
-                                committed trace. */
+                                lookahead apart: one unsorted append log for each of the next
+                                calendar_windows windows and each block of 256 LPs, holding
+                                capacity_per_lp (<= 64) records per LP of the block; a step reads only
+                                the logs of the window it commits.  Events beyond the calendar wait in
+                                two lists, and events at or past the horizon in a parked list, of
+                                num_lps * far_per_lp records each.  Same committed trace. */
   uint32_t sort_tile;         /* GAQ / HQ: records per block of the record sort and merge kernels:
                                 1024, 2048 or 4096 (0 = 2048); gpq_queue_autotune_sort picks one */
 } gpq_queue_config;
