@@ -379,6 +379,43 @@ __device__ __forceinline__ void commit_block(const LaqView& q, const FusedModel&
   if (threadIdx.x == 0) *log_count(q, c, blk) = 0u;     // the block's log of window c is consumed
 }
 
+// The end of the step that committed window c (its logs emptied by the commit blocks): the list
+// the step's list blocks read is emptied; the window advances by the lookahead (capped at the end
+// time) and the end of window c + 1 + B joins the ring; the window that reached the end time ends
+// the run.  Only the fields it owns are written: other blocks of the step may be setting the
+// error flag.
+__device__ __forceinline__ void publish(const LaqView& q, float lookahead, float end_time) {
+  WindowState* ws = q.ws;
+  CalState* st = q.cal_st;
+  const uint32_t B = q.cal_windows, M = 2u * B - 1u;
+  const uint32_t c = st->cur;
+  const uint32_t wend = ws->wend;
+  q.over_n[(c + 1u) & 1u] = 0;
+  ws->windows += 1;
+  ws->gmin = wend;                                 // a lower bound of the pending minimum
+  if (!(bits_to_f(wend) < end_time)) {
+    ws->done = 1;
+  } else {
+    const uint32_t nc = c + 1u;
+    st->cur = nc;
+    float nw = bits_to_f(st->w[(nc + B - 1u) & M]) + lookahead;
+    if (end_time < nw) nw = end_time;
+    st->w[(nc + B) & M] = f_to_bits(nw);
+    ws->wend = st->w[nc & M];
+  }
+}
+
+// Called by thread 0 of every block of a step once the block is done: the last block publishes.
+// No fence is needed: the words publish() reads are written by no block of the step, the list it
+// empties was read (and the values used) by the list blocks before they counted themselves, the
+// error flag it leaves alone, and its writes reach the next step across the kernel boundary.
+__device__ __forceinline__ void block_done(const LaqView& q, float lookahead, float end_time) {
+  if (atomicAdd(&q.cal_st->fin, 1u) == gridDim.x - 1u) {
+    publish(q, lookahead, end_time);
+    q.cal_st->fin = 0u;
+  }
+}
+
 // List block `blk` of `nblk` in the step that commits window x - 1: the events of over[x & 1]
 // (beyond the windows the step before could reach) are seen from window x over its first B - 1
 // windows -- x .. x + B - 2, all of them beyond the window being committed and short of the

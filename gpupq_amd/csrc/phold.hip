@@ -127,9 +127,10 @@ __global__ __launch_bounds__(kFusedBlock, 4) void phold_fused_kernel(LaqView q, 
 
 // The step on a calendar-mode LAQ (laq_calendar.cuh): blocks [0, list_blocks) move the events
 // beyond the calendar that it now reaches into their logs, the others commit window cur, one
-// block of LPs each.  With `inst` null the whole queue is one instance; otherwise commit block b
-// takes the model parameters and the end time of instance inst[block_inst[b]] (several
-// instances in one queue, every commit block in one instance).
+// block of LPs each; the last block to finish moves the window (queue_post's work).  With `inst`
+// null the whole queue is one instance; otherwise commit block b takes the model parameters and
+// the end time of instance inst[block_inst[b]] (several instances in one queue, every commit
+// block in one instance).  m.lookahead and end_time are the queue's lookahead and horizon.
 static_assert(cal::kBlock == (int)LaqQueue::kCalBlockLps, "calendar commit block size");
 __global__ __launch_bounds__(cal::kBlock) void phold_calendar_kernel(LaqView q, PholdDev m, float end_time,
                                                                      uint32_t list_blocks, const PoolInst* inst,
@@ -154,6 +155,8 @@ __global__ __launch_bounds__(cal::kBlock) void phold_calendar_kernel(LaqView q, 
     }
     cal::commit_block(q, fm, et, cur, b);
   }
+  __syncthreads();
+  if (threadIdx.x == 0) cal::block_done(q, m.lookahead, end_time);
 }
 
 // Initial event j of every LP of several instances in one calendar-mode LAQ (the populate of

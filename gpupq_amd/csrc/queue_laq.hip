@@ -740,6 +740,7 @@ __global__ void cal_start_kernel(LaqView q, float lookahead, float end_time) {
   CalState* st = q.cal_st;
   const uint32_t B = q.cal_windows;
   st->cur = 0;
+  st->fin = 0;
   uint32_t w = q.ws->wend;
   st->w[0] = w;
   for (uint32_t i = 1; i <= B; ++i) {
@@ -772,32 +773,6 @@ __global__ __launch_bounds__(cal::kBlock) void cal_fill_kernel(LaqView q, float 
 }
 
 __global__ void cal_fill_done_kernel(LaqView q) { q.over_n[0] = 0; }
-
-// queue_post after a calendar step (window c committed; its logs were emptied by the commit
-// blocks): the list the step read is emptied; the window advances by the lookahead (capped at
-// the end time) and the end of window c + 1 + B joins the ring; the window that reached the end
-// time ends the run.
-__global__ void cal_publish_kernel(LaqView q, float lookahead, float end_time) {
-  WindowState w = *q.ws;
-  if (w.done) return;
-  CalState* st = q.cal_st;
-  const uint32_t B = q.cal_windows, M = 2u * B - 1u;
-  const uint32_t c = st->cur;
-  q.over_n[(c + 1u) & 1u] = 0;
-  w.windows += 1;
-  w.gmin = w.wend;                                 // a lower bound of the pending minimum
-  if (!(bits_to_f(w.wend) < end_time)) {
-    w.done = 1;
-  } else {
-    const uint32_t nc = c + 1u;
-    st->cur = nc;
-    float nw = bits_to_f(st->w[(nc + B - 1u) & M]) + lookahead;
-    if (end_time < nw) nw = end_time;
-    st->w[(nc + B) & M] = f_to_bits(nw);
-    w.wend = st->w[nc & M];
-  }
-  *q.ws = w;
-}
 
 // Pending records: every log (grid-stride over the B x blocks counters), both beyond-the-calendar
 // lists and the parked ones (by the first thread), and the staging area (one LP per thread:
@@ -984,10 +959,8 @@ gpq_status LaqQueue::calendar_start() {
 }
 
 gpq_status LaqQueue::post() {
-  if (v_.wlog) {             // the calendar step committed its window: the window and calendar move
+  if (v_.wlog) {             // the calendar step's last block has moved the window already
     if (profile_mark) GPQ_HIP(hipEventRecord(profile_mark, stream));
-    hipLaunchKernelGGL(cal_publish_kernel, dim3(1), dim3(1), 0, stream, v_, cfg.lookahead, cfg.horizon);
-    GPQ_HIP(hipGetLastError());
     return GPQ_OK;
   }
   if (v_.stage_alt) {        // the model step merged and committed: only the window moves

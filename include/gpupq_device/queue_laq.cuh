@@ -29,11 +29,14 @@ constexpr uint32_t kCalMaxWindows = 256;
 
 // Calendar mode: the window index `cur` of the step that commits next, and the ends of the
 // windows x in [cur, cur + B] at w[x mod 2B] (float bits; every window after the one that
-// reaches the end time ends at the end time).
+// reaches the end time ends at the end time).  `fin` counts the blocks of the running step that
+// have finished (the last one moves the window and sets it back to 0); it has a cache line of its
+// own, away from the words every block reads when it starts.
 struct CalState {
   uint32_t cur;
   uint32_t pad[3];
   uint32_t w[2 * kCalMaxWindows];
+  alignas(128) uint32_t fin;
 };
 
 struct LaqView {
