@@ -118,8 +118,9 @@ def roofline(gpq, kind, wl, log):
         kernels = {
             "merge_commit_step": {"us": prof["step_kernel_us"], "alg_bytes": path_bytes / steps,
                                   "launches": steps, "overhead_bytes": overhead},
-            "window_publish": {"us": prof["window_kernel_us"], "alg_bytes": 24.0, "launches": steps},
         }
+        if not CAL:           # the calendar step's last block moves the window: no publish launch
+            kernels["window_publish"] = {"us": prof["window_kernel_us"], "alg_bytes": 24.0, "launches": steps}
     else:
         # queue post: the sorted insert of k records displacing d (16*(2k+2d)), 4 B per scanned
         # head and per block partial; model step: 88 B per committed event + 4 B per LP
