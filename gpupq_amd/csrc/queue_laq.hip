@@ -928,17 +928,20 @@ gpq_status LaqQueue::init_calendar() {
   GPQ_TRY(arena.alloc(&v_.ins, L));
   GPQ_TRY(arena.alloc(&v_.head, L));
   GPQ_TRY(arena.alloc(&sum_, 1));
-  const size_t counters = (size_t)B * blocks + 2;  // the logs' counters, then over_n[2]
+  // the logs' counters, then over_n[2] and parked_n, each on a 128-byte line of its own: every
+  // block of a step may reserve a run on them, and a line that such atomics share with words that
+  // blocks read (or with each other) slows every access to it
+  const size_t lists_at = ((size_t)B * blocks + 31u) & ~(size_t)31u;
+  const size_t counters = lists_at + 64u;
   GPQ_TRY(arena.alloc(&v_.wlog, (size_t)B * v_.wlog_cap));
   GPQ_TRY(arena.alloc(&v_.wlog_n, counters));
-  v_.over_n = v_.wlog_n + (size_t)B * blocks;
+  v_.over_n = v_.wlog_n + lists_at;
   GPQ_TRY(arena.alloc(&v_.over, 2 * (size_t)v_.over_cap));
   GPQ_TRY(arena.alloc(&v_.parked, (size_t)v_.parked_cap));
-  GPQ_TRY(arena.alloc(&v_.parked_n, 1));
   GPQ_TRY(arena.alloc(&v_.cal_st, 1));
   GPQ_TRY(arena.alloc(&gather_, kGatherCap + 1));
+  v_.parked_n = v_.wlog_n + lists_at + 32u;
   GPQ_HIP(hipMemsetAsync(v_.wlog_n, 0, sizeof(uint32_t) * counters, stream));
-  GPQ_HIP(hipMemsetAsync(v_.parked_n, 0, sizeof(uint32_t), stream));
   GPQ_HIP(hipMemsetAsync(v_.cal_st, 0, sizeof(CalState), stream));
   v_.ws = ws;
   v_.stats = stats;
