@@ -261,6 +261,17 @@ __device__ __forceinline__ void commit_block(const LaqView& q, const FusedModel&
   __shared__ uint32_t s_slow[kBlock];
   __shared__ uint32_t s_nslow;
   __shared__ Key s_keys[kBlock / kWave][kWaveCand];
+  const uint32_t lp0 = blk << q.cal_block_log2;
+  const uint32_t lp = lp0 + threadIdx.x;
+  const bool in_range = lp < q.num_lps && lp - m.base < m.lps;
+  const Event* lg = log_base(q, c, blk);
+  // issued together with the window ends' load: the log's length, the LP of the log record this
+  // thread groups first (slot threadIdx.x exists: a block's log has at least kBlock slots; used
+  // only below the length) and the thread's LP state, so that their latencies overlap
+  const uint32_t n0 = *log_count(q, c, blk);
+  const uint32_t d0 = lg[threadIdx.x].dst;
+  const uint32_t cnt_pre = in_range ? m.count[lp] : 0u;
+  const uint64_t h_pre = in_range ? m.hash[lp] : 0ull;
   load_windows(q, c, sw);
   const uint32_t B = q.cal_windows;
   s_cnt[threadIdx.x] = 0u;
@@ -268,12 +279,9 @@ __device__ __forceinline__ void commit_block(const LaqView& q, const FusedModel&
   if (threadIdx.x < 2) s_lcnt[threadIdx.x] = 0u;
   if (threadIdx.x == 0) s_nslow = 0;
   __syncthreads();
-  const uint32_t n0 = *log_count(q, c, blk);
   const uint32_t n = n0 < q.cal_sub_cap ? n0 : q.cal_sub_cap;
-  const Event* lg = log_base(q, c, blk);
-  const uint32_t lp0 = blk << q.cal_block_log2;
   for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
-    const uint32_t l = lg[i].dst - lp0;                 // < kBlock: the log holds this block's LPs only
+    const uint32_t l = (i == threadIdx.x ? d0 : lg[i].dst) - lp0;   // < kBlock: the log holds this block's LPs only
     if (l >= (uint32_t)kBlock) {
       atomicOr(&q.ws->error, kErrBadState);
       continue;
@@ -283,11 +291,10 @@ __device__ __forceinline__ void commit_block(const LaqView& q, const FusedModel&
   }
   __syncthreads();
   const uint32_t end_bits = f_to_bits(end_time);
-  const uint32_t lp = lp0 + threadIdx.x;
   uint4 ne[kRegs];      // list successors as uint4: an array of the 16-byte-aligned Event goes to scratch memory
   uint32_t route[kRegs], rank[kRegs];
   uint32_t cn = 0;
-  if (lp < q.num_lps && lp - m.base < m.lps) {
+  if (in_range) {
     const uint32_t k = s_cnt[threadIdx.x];
     if (k > (uint32_t)kRegs) {
       s_slow[atomicAdd(&s_nslow, 1u)] = lp;
@@ -305,8 +312,8 @@ __device__ __forceinline__ void commit_block(const LaqView& q, const FusedModel&
           kt[s] = ks[s] = kq[s] = 0xFFFFFFFFu;
         }
       }
-      const uint32_t cnt = m.count[lp];
-      uint64_t h = m.hash[lp];
+      const uint32_t cnt = cnt_pre;
+      uint64_t h = h_pre;
       uint32_t last = m.last_ts ? m.last_ts[lp] : 0u;
       sort8(kt, ks, kq);
 #pragma unroll
