@@ -741,7 +741,7 @@ def main():
                               if DRAIN == "fused" else
                               "drain each LP's safe window" if DRAIN else "one event per LP per window"),
                    "laq_far_per_lp": FAR,
-                   "laq_calendar": ({"windows": CAL, "slots_per_bucket": CAL_SLOTS} if CAL and DRAIN == "fused"
+                   "laq_calendar": ({"windows": CAL, "log_records_per_lp": CAL_SLOTS} if CAL and DRAIN == "fused"
                                     else None),
                    "trace_hash_vs_cpu": parity},
     }
