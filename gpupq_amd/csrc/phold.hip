@@ -132,6 +132,7 @@ __global__ __launch_bounds__(kFusedBlock, 4) void phold_fused_kernel(LaqView q, 
 // the end time of instance inst[block_inst[b]] (several instances in one queue, every commit
 // block in one instance).  m.lookahead and end_time are the queue's lookahead and horizon.
 static_assert(cal::kBlock == (int)LaqQueue::kCalBlockLps, "calendar commit block size");
+template <bool kStage>
 __global__ __launch_bounds__(cal::kBlock) void phold_calendar_kernel(LaqView q, PholdDev m, float end_time,
                                                                      uint32_t list_blocks, const PoolInst* inst,
                                                                      const uint32_t* block_inst) {
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(cal::kBlock) void phold_calendar_kernel(LaqView q, 
       fm.lookahead = in.lookahead;
       et = in.end_time;
     }
-    cal::commit_block(q, fm, et, cur, b);
+    cal::commit_block<kStage>(q, fm, et, cur, b);
   }
   __syncthreads();
   if (threadIdx.x == 0) cal::block_done(q, m.lookahead, end_time);
@@ -349,7 +350,7 @@ gpq_status PholdModel::launch_fused(uint32_t settle) {
   if (laq->calendar()) {     // nothing is staged between calendar steps: no closing fold
     if (settle) return GPQ_OK;
     const uint32_t g = cal::kStepListBlocks;
-    hipLaunchKernelGGL(phold_calendar_kernel, dim3(g + ceil_div(cfg.num_lps, cal::kBlock)), dim3(cal::kBlock), 0,
+    hipLaunchKernelGGL(phold_calendar_kernel<false>, dim3(g + ceil_div(cfg.num_lps, cal::kBlock)), dim3(cal::kBlock), 0,
                        q->stream, laq->view(), dev_of(this), cfg.end_time, g, nullptr, nullptr);
     GPQ_HIP(hipGetLastError());
     return GPQ_OK;
@@ -575,7 +576,7 @@ gpq_status launch_calendar_instances(LaqQueue* q, const PoolInst* inst, const ui
   d.drain = 2;
   d.lookahead = q->cfg.lookahead;
   const uint32_t g = cal::kStepListBlocks;
-  hipLaunchKernelGGL(phold_calendar_kernel, dim3(g + ceil_div(q->cfg.num_lps, cal::kBlock)), dim3(cal::kBlock), 0,
+  hipLaunchKernelGGL(phold_calendar_kernel<true>, dim3(g + ceil_div(q->cfg.num_lps, cal::kBlock)), dim3(cal::kBlock), 0,
                      q->stream, q->view(), d, q->cfg.horizon, g, inst, block_inst);
   GPQ_HIP(hipGetLastError());
   return GPQ_OK;
