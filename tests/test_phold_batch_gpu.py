@@ -76,6 +76,21 @@ def test_calendar_batch_mixed_lookaheads(gpq, oracle):
         assert int(eg[i]) == res.events and int(hg[i]) == res.trace_hash, i
 
 
+def test_calendar_batch_dense_instance(gpq, oracle):
+    # 64 events per LP: about 6 per LP and window, so many LPs take the slow path, and a block's log
+    # of a window holds about 1,500 records, past the ones whose keys the step keeps in LDS
+    inst = [(256, 64, 21, 2.0), (300, 8, 101, 6.0), (600, 64, 22, 1.5)]
+    with gpq.PholdBatch(inst, **CAL) as b:
+        b.run("graph", batch=8)
+        hg, eg = b.results()
+        b.run("plain")
+        hp, ep = b.results()
+    assert np.array_equal(hg, hp) and np.array_equal(eg, ep)
+    for i, (L, k, seed, T) in enumerate(inst):
+        res, _, _ = oracle.phold("seq", L, k, T, seed=seed)
+        assert int(eg[i]) == res.events and int(hg[i]) == res.trace_hash, i
+
+
 def test_calendar_batch_runs_twice_and_rejects_bad_settings(gpq, oracle):
     with gpq.PholdBatch(INSTANCES[:3], **CAL) as b:
         b.run("graph", batch=16)
