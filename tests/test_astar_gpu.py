@@ -215,13 +215,20 @@ def test_single_query_plain_equals_graph(gpq, oracle):
         a.model_finish()
 
 
-@pytest.mark.parametrize("tile", [0, 16, 32])
+@pytest.mark.parametrize("tile", [0, 16, 32, "laq2"])
 def test_single_query_4096_grid(gpq, oracle, tile):
     """The 4096 x 4096, 4-connected, Bernoulli(0.2) seed 7 query, (0,0) -> (4095,4095): open
-    lists in a local heap queue (tile 0) and the tile search."""
+    lists in a local heap queue (tile 0), in two-tier local array queues (the benchmark's: a
+    32-record ring and a 256-record far store per list) and the tile search."""
     w = h = 4096
-    with gpq.Queue("lhq", num_lps=64 if tile else 1 << 16, capacity_per_lp=64, stage_per_lp=16, lookahead=INF,
-                   horizon=INF) as q:
+    if tile == "laq2":
+        tile = 0
+        q = gpq.Queue("laq", num_lps=1 << 16, capacity_per_lp=32, stage_per_lp=16, far_per_lp=256, lookahead=INF,
+                      horizon=INF)
+    else:
+        q = gpq.Queue("lhq", num_lps=64 if tile else 1 << 16, capacity_per_lp=64, stage_per_lp=16, lookahead=INF,
+                      horizon=INF)
+    with q:
         a = gpq.Astar(q, w, h, connectivity=4, seed=7, threshold=0.2, tile=tile)
         a.model_init()
         rep = a.run("graph", 32)
