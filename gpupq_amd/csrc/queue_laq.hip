@@ -808,7 +808,7 @@ __global__ void cal_count_kernel(LaqView q, unsigned long long* sum) {
   unsigned long long n = 0;
   if (t < q.num_lps) n = min(q.ins[t], q.stage_cap);
   const uint32_t logs = q.cal_windows * q.cal_blocks;
-  for (uint32_t i = t; i < logs; i += gridDim.x * blockDim.x) n += min(q.wlog_n[i], q.cal_sub_cap);
+  for (uint32_t i = t; i < logs; i += gridDim.x * blockDim.x) n += min(q.wlog_n[(size_t)i << cal::kLogCountShift], q.cal_sub_cap);
   if (t == 0) {
     for (uint32_t p = 0; p < 2; ++p) n += min(q.over_n[p], q.over_cap);
     n += min(*q.parked_n, q.parked_cap);
@@ -953,10 +953,10 @@ gpq_status LaqQueue::init_calendar() {
   GPQ_TRY(arena.alloc(&v_.ins, L));
   GPQ_TRY(arena.alloc(&v_.head, L));
   GPQ_TRY(arena.alloc(&sum_, 1));
-  // the logs' counters, then over_n[2] and parked_n, each on a 128-byte line of its own: every
+  // the logs' counters (16 words apart), then over_n[2] and parked_n, each on a 128-byte line of its own: every
   // block of a step may reserve a run on them, and a line that such atomics share with words that
   // blocks read (or with each other) slows every access to it
-  const size_t lists_at = ((size_t)B * blocks + 31u) & ~(size_t)31u;
+  const size_t lists_at = (((size_t)B * blocks << cal::kLogCountShift) + 31u) & ~(size_t)31u;
   const size_t counters = lists_at + 64u;
   GPQ_TRY(arena.alloc(&v_.wlog, (size_t)B * v_.wlog_cap));
   GPQ_TRY(arena.alloc(&v_.wlog_n, counters));

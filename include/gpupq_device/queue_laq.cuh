@@ -68,7 +68,7 @@ struct LaqView {
   // Calendar mode (cal_windows > 0; the PHOLD merge-and-commit step only): one append log per
   // upcoming window and block of 2^cal_block_log2 LPs: the events of window cur + o for the LPs of
   // block b are at wlog + ((cur + o) mod cal_windows) * wlog_cap + b * cal_sub_cap, count
-  // wlog_n[((cur + o) mod cal_windows) * cal_blocks + b]; events beyond the calendar in over[p]
+  // wlog_n[(((cur + o) mod cal_windows) * cal_blocks + b) * 16]; events beyond the calendar in over[p]
   // (p = parity of the step that appended them), count over_n[p]; events at or past the end
   // time in parked, count *parked_n.  A step's commit blocks read the log of their own LPs for
   // its window and group it by LP in LDS.
