@@ -108,7 +108,7 @@ __device__ __forceinline__ uint32_t route_of(const uint32_t* sw, uint32_t n, uin
 // The log of window x for LP block b: its counter and its first record.  Counters sit 16 words
 // (64 bytes) apart: packed, the counters of one window for 32 neighbouring LP blocks shared a
 // 128-byte line, and every block's successors for those blocks' logs met on it (about 2,000
-// atomics per line and step at 2^20 LPs).  Spread out, the step took 153 against 159 us (nine
+// atomics per line and step at 2^20 LPs).  Spread out, the step took 154 against 159 us (nine
 // A/B pairs at 2^20 LPs) and the instances of one calendar ran 9% faster; 32 words apart gave no
 // more.
 constexpr uint32_t kLogCountShift = 4;
