@@ -102,9 +102,9 @@ gpq_status GaqQueue::init() {
   GPQ_TRY(arena.alloc(&fel_[1], cap));
   GPQ_TRY(arena.alloc(&rest_, cap));
   GPQ_TRY(arena.alloc(&v_.enq, (size_t)kInsertStripes * v_.seg_cap + enq_cap));
-  GPQ_TRY(arena.alloc(&v_.seg_m, kInsertStripes + 1));
+  GPQ_TRY(arena.alloc(&v_.seg_m, kStripeCounterWords));
   GPQ_TRY(arena.alloc(&seg_pre_, kInsertStripes + 2));
-  GPQ_HIP(hipMemsetAsync(v_.seg_m, 0, sizeof(uint32_t) * (kInsertStripes + 1), stream));
+  GPQ_HIP(hipMemsetAsync(v_.seg_m, 0, sizeof(uint32_t) * kStripeCounterWords, stream));
   GPQ_TRY(arena.alloc(&sort_[0], enq_cap));
   GPQ_TRY(arena.alloc(&sort_[1], enq_cap));
   GPQ_TRY(arena.alloc(&v_.deq, cfg.num_lps));

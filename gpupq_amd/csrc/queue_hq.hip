@@ -335,9 +335,9 @@ gpq_status HqQueue::init() {
   v_.seg_cap = insert_stripe_cap(enq_cap);
   v_.area = (uint64_t)kInsertStripes * v_.seg_cap + enq_cap;
   GPQ_TRY(arena.alloc(&v_.enq, (size_t)heaps_ * v_.area));
-  GPQ_TRY(arena.alloc(&v_.seg_m, (size_t)heaps_ * (kInsertStripes + 1)));
+  GPQ_TRY(arena.alloc(&v_.seg_m, (size_t)heaps_ * kStripeCounterWords));
   GPQ_TRY(arena.alloc(&seg_pre_, (size_t)heaps_ * (kInsertStripes + 2)));
-  GPQ_HIP(hipMemsetAsync(v_.seg_m, 0, sizeof(uint32_t) * heaps_ * (kInsertStripes + 1), stream));
+  GPQ_HIP(hipMemsetAsync(v_.seg_m, 0, sizeof(uint32_t) * heaps_ * kStripeCounterWords, stream));
   // working buffers, one set per heap so that the heaps' steps can run concurrently
   scratch_n_ = 2 * (size_t)ceil_div(mcap_, 1024) + 2;
   tasks_n_ = 3 * (size_t)slots_ + 8;
@@ -418,7 +418,7 @@ gpq_status HqQueue::merge_root(uint32_t h, hipStream_t s) {
   MergeTask* t = tasks_ + (size_t)h * tasks_n_;
   Event* sorted = nullptr;
   uint32_t* pre = seg_pre_ + (size_t)h * (kInsertStripes + 2);
-  GPQ_TRY(gather_inserts(v_.seg_m + (size_t)h * (kInsertStripes + 1), v_.seg_cap, v_.enq_cap, v_.enq_cap, pre, &st->m,
+  GPQ_TRY(gather_inserts(v_.seg_m + (size_t)h * kStripeCounterWords, v_.seg_cap, v_.enq_cap, v_.enq_cap, pre, &st->m,
                          ws, s));
   GPQ_TRY(sort_records(v_.enq + (size_t)h * v_.area, sort_[0] + (size_t)h * v_.enq_cap,
                        sort_[1] + (size_t)h * v_.enq_cap, &st->m, v_.enq_cap, s, &sorted, tile_,

@@ -48,10 +48,11 @@ inline uint32_t insert_stripe_cap(uint64_t enq_cap) {
   return (uint32_t)(c < 1024 ? 1024 : c);
 }
 
-// The step's inserts in a striped insert area (counters seg_m[0 .. kInsertStripes], stripes of
-// seg_cap records, spill segment of spill_cap): seg_pre[k] = records in the segments before k
-// (kInsertStripes + 2 words), *d_m = their total, capped at max_total (beyond it the queue's
-// error flag is set and the last segments are cut short); the counters are reset to zero.
+// The step's inserts in a striped insert area (counters seg_m[k * kStripeWords], k <=
+// kInsertStripes; stripes of seg_cap records, spill segment of spill_cap): seg_pre[k] = records
+// in the segments before k (kInsertStripes + 2 words), *d_m = their total, capped at max_total
+// (beyond it the queue's error flag is set and the last segments are cut short); the counters are
+// reset to zero.
 gpq_status gather_inserts(uint32_t* seg_m, uint32_t seg_cap, uint32_t spill_cap, uint32_t max_total, uint32_t* seg_pre,
                           uint32_t* d_m, WindowState* ws, hipStream_t stream);
 

@@ -326,9 +326,9 @@ __global__ __launch_bounds__(64) void gather_inserts_kernel(uint32_t* seg_m, uin
   const uint32_t k = threadIdx.x;
   uint32_t n = 0;
   if (k <= kInsertStripes) {
-    const uint32_t c = seg_m[k], cap = k < kInsertStripes ? seg_cap : spill_cap;
+    const uint32_t c = seg_m[k * kStripeWords], cap = k < kInsertStripes ? seg_cap : spill_cap;
     n = c < cap ? c : cap;
-    seg_m[k] = 0;
+    seg_m[k * kStripeWords] = 0;
   }
   uint32_t inc = n;
   for (int off = 1; off < kWave; off <<= 1) {

@@ -179,22 +179,25 @@ __device__ __forceinline__ uint32_t wave_append_slot(uint32_t* counter) {
 }
 
 // Striped insert area of a global queue (GAQ, one HQ heap): kInsertStripes segments of seg_cap
-// records, then a spill segment of spill_cap records; counters seg_m[0 .. kInsertStripes].  A
+// records, then a spill segment of spill_cap records; counter k at seg_m[k * kStripeWords],
+// k = 0 .. kInsertStripes, each on a 64-byte line of its own (kStripeCounterWords words in all).  A
 // workgroup appends to stripe blockIdx.x mod kInsertStripes (one counter atomic per wave), so
 // that waves of different workgroups do not all contend for one counter; a lane whose stripe is
 // full appends to the spill segment.  queue_post reads the stripes and then the spill segment,
 // in that order, as one run of inserts.  All active lanes of the wave must take part.
 constexpr uint32_t kInsertStripes = 32;
+constexpr uint32_t kStripeWords = 16;
+constexpr uint32_t kStripeCounterWords = (kInsertStripes + 1u) * kStripeWords;
 
 __device__ __forceinline__ bool striped_append(Event* area, uint32_t* seg_m, uint32_t seg_cap, uint32_t spill_cap,
                                                const Event& e) {
   const uint32_t s = blockIdx.x & (kInsertStripes - 1u);
-  const uint32_t pos = wave_append_slot(&seg_m[s]);
+  const uint32_t pos = wave_append_slot(&seg_m[s * kStripeWords]);
   if (pos < seg_cap) {
     store_event(area + (size_t)s * seg_cap + pos, e);
     return true;
   }
-  const uint32_t sp = wave_append_slot(&seg_m[kInsertStripes]);   // the lanes whose stripe is full
+  const uint32_t sp = wave_append_slot(&seg_m[kInsertStripes * kStripeWords]);   // the lanes whose stripe is full
   if (sp < spill_cap) {
     store_event(area + (size_t)kInsertStripes * seg_cap + sp, e);
     return true;

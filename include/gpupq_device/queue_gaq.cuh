@@ -31,7 +31,7 @@ struct GaqView {
   Event* deq;
   GaqState* st;
   WindowState* ws;
-  uint32_t* seg_m;      // kInsertStripes + 1 append counters
+  uint32_t* seg_m;      // kInsertStripes + 1 append counters, kStripeWords apart (core.cuh)
   uint32_t enq_cap;     // inserts one step may hold (and the spill segment's size)
   uint32_t seg_cap;
   uint32_t num_lps;

@@ -40,7 +40,7 @@ struct HqView {
   Event* deq;
   HqState* st;
   WindowState* ws;
-  uint32_t* seg_m;         // heap h's kInsertStripes + 1 append counters at seg_m + h * (kInsertStripes + 1)
+  uint32_t* seg_m;         // heap h's kInsertStripes + 1 append counters at seg_m + h * kStripeCounterWords
   uint32_t enq_cap;        // per heap: inserts one step may hold (and the spill segment's size)
   uint32_t seg_cap;
   uint32_t num_lps;
@@ -63,7 +63,7 @@ struct HqView {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lh = (uint32_t)__shfl((int)h, leader, kWave);
         const uint64_t same = __ballot(h == lh) & pending;
-        if (h == lh) ok = striped_append(enq + lh * area, seg_m + lh * (kInsertStripes + 1u), seg_cap, enq_cap, e);
+        if (h == lh) ok = striped_append(enq + lh * area, seg_m + lh * kStripeCounterWords, seg_cap, enq_cap, e);
         pending &= ~same;
       }
     }
