@@ -953,9 +953,9 @@ gpq_status LaqQueue::init_calendar() {
   GPQ_TRY(arena.alloc(&v_.ins, L));
   GPQ_TRY(arena.alloc(&v_.head, L));
   GPQ_TRY(arena.alloc(&sum_, 1));
-  // the logs' counters (16 words apart), then over_n[2] and parked_n, each on a 128-byte line of its own: every
-  // block of a step may reserve a run on them, and a line that such atomics share with words that
-  // blocks read (or with each other) slows every access to it
+  // the logs' counters (16 words apart), then over_n[2] and parked_n, each on a 128-byte line of
+  // its own: every block of a step may reserve a run on them, and a line that such atomics share
+  // with words that blocks read (or with each other) slows every access to it
   const size_t lists_at = (((size_t)B * blocks << cal::kLogCountShift) + 31u) & ~(size_t)31u;
   const size_t counters = lists_at + 64u;
   GPQ_TRY(arena.alloc(&v_.wlog, (size_t)B * v_.wlog_cap));

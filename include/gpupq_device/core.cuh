@@ -180,7 +180,8 @@ __device__ __forceinline__ uint32_t wave_append_slot(uint32_t* counter) {
 
 // Striped insert area of a global queue (GAQ, one HQ heap): kInsertStripes segments of seg_cap
 // records, then a spill segment of spill_cap records; counter k at seg_m[k * kStripeWords],
-// k = 0 .. kInsertStripes, each on a 64-byte line of its own (kStripeCounterWords words in all).  A
+// k = 0 .. kInsertStripes, 64 bytes apart (kStripeCounterWords words in all): packed into two
+// lines they took every wave's append atomic (GAQ 6.85e8 against 7.03e8 events/s at 2^20 LPs).  A
 // workgroup appends to stripe blockIdx.x mod kInsertStripes (one counter atomic per wave), so
 // that waves of different workgroups do not all contend for one counter; a lane whose stripe is
 // full appends to the spill segment.  queue_post reads the stripes and then the spill segment,
